@@ -1,0 +1,193 @@
+/*
+ * radhip.h -- C ABI of radiacuda_amd, an MI355X (gfx950) implementation of
+ * Radia's three GPU-bound paths: interaction-matrix assembly, the relaxation
+ * solve, and field evaluation of magnetized blocks and polyhedra.
+ *
+ * Plain C types only (pointers, sizes, enums); no HIP or torch types cross
+ * this boundary.  Streams are passed as opaque `void*` (a hipStream_t, or NULL
+ * for the null stream).  Every function returns an rh_status; on failure
+ * rh_last_error() returns a thread-local message.
+ *
+ * Each entry point names the reference interface it replaces
+ * (DanielWinklehner/RadiaCUDA, paths relative to the repository root).
+ *
+ * Units follow Radia: lengths in mm, B, H (as mu0*H) and M in tesla, current
+ * density in A/mm^2.
+ */
+#ifndef RADHIP_H
+#define RADHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    RH_OK = 0,
+    RH_ERR_ARG = 1,         /* invalid argument (shape, index, null pointer)     */
+    RH_ERR_HIP = 2,         /* a HIP runtime call failed                          */
+    RH_ERR_NOMEM = 3,       /* device allocation failed                           */
+    RH_ERR_UNSUPPORTED = 4, /* element / material / option not supported          */
+    RH_ERR_GEOMETRY = 5,    /* degenerate or non-convex polyhedron                */
+    RH_ERR_NOTCONV = 6      /* relaxation did not reach the requested precision   */
+} rh_status;
+
+typedef enum {
+    RH_FP64 = 0,  /* FP64 arithmetic, FP64 matrix storage                          */
+    RH_MIXED = 1  /* FP64 arithmetic, FP32 matrix storage (Radia's own storage)    */
+} rh_precision;
+
+/* ------------------------------------------------------------------------ */
+/* Model: magnetizable elements uploaded once, reused by assembly and field. */
+/* ------------------------------------------------------------------------ */
+
+/* Element geometry in each element's own frame plus its transform copies.
+ * Copy 0 of element e is its main transform (defines the collocation point and
+ * the row transform); further copies come from symmetries.  A copy maps a
+ * point x to M*x + V and a field vector h to s*M*h (gmTrans,
+ * cpp/src/ext/genmath/gmtrans.h:30-85).  All arrays are host pointers. */
+typedef struct {
+    int64_t n_elem;
+    const int32_t *elem_type;  /* [n]  0 = rectangular block, 1 = polyhedron          */
+    const double *rec_center;  /* [3n] block centre (ignored for polyhedra)           */
+    const double *rec_dims;    /* [3n] block full dimensions                          */
+    const int64_t *vert_off;   /* [n+1] polyhedron vertex ranges                      */
+    const double *verts;       /* [3*V] polyhedron vertices                           */
+    const int64_t *face_off;   /* [n+1] polyhedron face ranges                        */
+    const int32_t *face_len;   /* [F]  vertices per face                              */
+    const int32_t *face_idx;   /* [sum face_len] 0-based vertex index within element  */
+    const int64_t *copy_off;   /* [n+1] transform-copy ranges                         */
+    const double *copy_m;      /* [9*C] copy matrices, row-major                      */
+    const double *copy_v;      /* [3*C] copy translations                             */
+    const double *copy_s;      /* [C]  field multipliers (+1, or -1 for inversion)   */
+} rh_model_desc;
+
+typedef struct rh_model rh_model;
+
+/* Builds face frames, outward normals, edge tables and copy transforms on the
+ * host and uploads them.  Replaces the packing half of the reference seam:
+ * radGPU_PackGeometryForAsm (cpp/src/core/radgpu_asm.cpp:206-444) and the
+ * per-call geometry flattening of radGPU_ComputeField
+ * (cpp/src/core/radgpu_fld.cpp:194-536). */
+rh_status rh_model_create(const rh_model_desc *desc, int device, void *stream, rh_model **out);
+rh_status rh_model_destroy(rh_model *m);
+/* Collocation points (lab frame) of the elements, [3n] host doubles:
+ * MainTrans(centroid), cpp/src/core/radintrc.cpp:852. */
+rh_status rh_model_colloc(const rh_model *m, double *colloc_out);
+/* Counts: [0] elements, [1] copies, [2] tetrahedra, [3] hexahedra,
+ * [4] other polyhedra, [5] blocks, [6] edges, [7] faces. */
+rh_status rh_model_counts(const rh_model *m, int64_t *counts8);
+
+/* ------------------------------------------------------------------------ */
+/* Interaction matrix                                                         */
+/* ------------------------------------------------------------------------ */
+typedef struct rh_matrix rh_matrix;
+
+/* Assembles the dense 3n x 3n interaction matrix (H = Q*M, element frames) on
+ * the device and keeps it there.  Replaces radGPU_AssembleMatrix
+ * (cpp/src/core/radgpu_asm.cu:889-1158) and the matrix hand-off
+ * radGPU_PublishAssembledMatrix / radGPU_AdoptMatrixCache
+ * (cpp/src/core/radgpu_asm.cu:40-51, cpp/src/core/radgpurlx.cu:728-746): the
+ * handle owns the device matrix, no host copy is made. */
+rh_status rh_matrix_assemble(rh_model *m, rh_precision prec, rh_matrix **out);
+rh_status rh_matrix_destroy(rh_matrix *q);
+/* Copies rows [row0, row0+nrows) of the 3n x 3n matrix to host doubles. */
+rh_status rh_matrix_export(const rh_matrix *q, int64_t row0, int64_t nrows, double *out);
+/* Raw device pointer and element size (4 or 8 bytes) of the stored matrix. */
+rh_status rh_matrix_device_ptr(const rh_matrix *q, void **dptr, int32_t *elem_bytes);
+/* Re-assembles into an existing matrix (same model); the timed unit of work. */
+rh_status rh_matrix_reassemble(rh_matrix *q);
+
+/* ------------------------------------------------------------------------ */
+/* Relaxation                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Per-element material record, 16 doubles (radmater.h:164-696):
+ *   [0] kind: 0 linear isotropic, 1 linear anisotropic, 2 nonlinear isotropic
+ *       (tanh-sum formula MatSatIsoFrm)
+ *   kind 0: [1] ksi
+ *   kind 1: [4..12] susceptibility tensor, row-major (element frame)
+ *   kind 2: [1] number of terms (1..3), [2..4] ms_i, [5..7] ksi_i
+ *   [13..15] remanent magnetization (element frame) */
+#define RH_MAT_STRIDE 16
+
+typedef struct {
+    double prec;       /* stop when the RMS magnetization residual (T) <= prec */
+    int32_t max_iter;  /* iteration limit                                      */
+    int32_t keep_m;    /* 1: start from M_inout ('ZeroM->False'), 0: from M_r  */
+    double omega0;     /* mixing factor of the iteration (0 -> 0.5)            */
+    int32_t method;    /* 9 (or 0): Anderson-accelerated block Jacobi          */
+    int32_t reserved;  /* must be 0                                            */
+} rh_relax_opts;
+
+typedef struct {
+    double misfit;     /* RMS |M_proposal - M| over elements (T)   */
+    double max_m;      /* max |M| (T)                               */
+    double max_h;      /* max |H| (T)                               */
+    int32_t iters;     /* iterations performed                      */
+    int32_t matvecs;   /* matrix passes (each streams the matrix)   */
+    int32_t converged; /* 1 if misfit <= prec                       */
+} rh_relax_result;
+
+/* Relaxes the magnetization to M = M_mat(H_ext + Q*M).  mats: [16n] host
+ * records; hext: [3n] external field at the collocation points (element
+ * frames); M_inout, H_out: [3n] host doubles.  Replaces radGPU_RelaxAuto
+ * (cpp/src/core/radgpurlx.cu:799-1349, header cpp/src/core/radgpurlx.h:69-86)
+ * and the method-9 driver radGPU_AutoRelax (cpp/src/core/radrlmet_gpu.cpp:9-53). */
+rh_status rh_relax(rh_matrix *q, const double *mats, const double *hext, double *M_inout,
+                   double *H_out, const rh_relax_opts *opts, rh_relax_result *res);
+
+/* Same on device-resident vectors (d_mats [16n], d_hext, d_M, d_H [3n] doubles). */
+rh_status rh_relax_dev(rh_matrix *q, const double *d_mats, const double *d_hext, double *d_M,
+                       double *d_H, const rh_relax_opts *opts, rh_relax_result *res);
+
+/* ------------------------------------------------------------------------ */
+/* Field evaluation                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* B and/or H of the model's elements with magnetizations M ([3n], element
+ * frames) at np lab-frame points (B = H + M inside an element).  B_out or
+ * H_out may be NULL.  prec = RH_FP64 evaluates the integrals in FP64;
+ * RH_MIXED evaluates them in FP32 and accumulates in FP64.  Host arrays.
+ * Replaces radGPU_ComputeField (cpp/src/core/radgpu_fld.h:172-184,
+ * radgpu_fld.cpp:623-857; use_gpu 1 = FP64, 2 = FP32) and the kernels behind
+ * radGPU_Fld{AllocAndCopy,LaunchKernel,RetrieveAndFree}
+ * (cpp/src/core/radgpu_fld.cu:381-1023). */
+rh_status rh_field(rh_model *m, const double *M, const double *pts, int64_t np, double *B_out,
+                   double *H_out, rh_precision prec);
+/* Same on device arrays; the library keeps a growable scratch buffer per model. */
+rh_status rh_field_dev(rh_model *m, const double *d_M, const double *d_pts, int64_t np,
+                       double *d_B, double *d_H, rh_precision prec);
+
+/* Current sources.  Record kinds (12 doubles each):
+ *   0 block current: centre[3], dims[3], J[3]          (radrec.cpp:49-315, J branch)
+ *   1 arc current:   centre[3], rmin, rmax, phimin, phimax, height, j, nseg
+ *                    (trapezoid rule, radarccu.cpp:30-234)
+ * each with one transform (M [9], V [3], s).  Adds their B (= H outside
+ * magnetic material) at np points into B_acc ([3np] host doubles).
+ * Replaces the host loop ComputeCoilFieldCPU (cpp/src/core/radgpu_fld.cpp:542-617)
+ * and the external-field set-up of radTInteraction::SetupExternFieldArray
+ * (cpp/src/core/radintrc.cpp:1208-1236). */
+rh_status rh_current_field(int device, void *stream, int64_t nsrc, const int32_t *kind,
+                           const double *par, const double *tm, const double *tv,
+                           const double *ts, const double *pts, int64_t np, double *B_acc);
+/* Same with pts and B_acc in device memory (source records stay on the host). */
+rh_status rh_current_field_dev(int device, void *stream, int64_t nsrc, const int32_t *kind,
+                               const double *par, const double *tm, const double *tv,
+                               const double *ts, const double *d_pts, int64_t np, double *d_B_acc);
+
+/* Diagnostics */
+const char *rh_last_error(void);
+int32_t rh_device_count(void);
+/* Average duration (ms) and count of the most recent timed launches of the
+ * named kernel family ("asm", "field", "matvec"), measured with HIP events on
+ * the launch stream when rh_set_timing(1) is on. */
+rh_status rh_set_timing(int32_t on);
+rh_status rh_kernel_time(const char *family, double *avg_ms, int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RADHIP_H */

@@ -1,0 +1,238 @@
+"""ctypes binding of libradhip.so (the C ABI in include/radhip.h).
+
+The library is built in-tree by ``radiacuda_amd.build``.  There is no CPU
+fallback: if the library or a GPU is missing every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import RH_FP64, RH_MIXED, RH_OK  # noqa: F401  (re-exported)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libradhip.so")
+_L = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i32 = ctypes.POINTER(ctypes.c_int32)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+
+class RadHipError(RuntimeError):
+    pass
+
+
+def lib():
+    global _L
+    if _L is not None:
+        return _L
+    if not os.path.exists(LIB_PATH):
+        raise RadHipError("libradhip.so is not built (run: python -m radiacuda_amd.build); "
+                          "the HIP path has no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    st = ctypes.c_int
+    L.rh_last_error.restype = ctypes.c_char_p
+    L.rh_device_count.restype = ctypes.c_int32
+    L.rh_model_create.argtypes = [ctypes.POINTER(_abi.ModelDesc), ctypes.c_int, _vp,
+                                  ctypes.POINTER(_vp)]
+    L.rh_model_destroy.argtypes = [_vp]
+    L.rh_model_colloc.argtypes = [_vp, _d]
+    L.rh_model_counts.argtypes = [_vp, _i64]
+    L.rh_matrix_assemble.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
+    L.rh_matrix_destroy.argtypes = [_vp]
+    L.rh_matrix_export.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, _d]
+    L.rh_matrix_device_ptr.argtypes = [_vp, ctypes.POINTER(_vp), _i32]
+    L.rh_matrix_reassemble.argtypes = [_vp]
+    L.rh_relax.argtypes = [_vp, _d, _d, _d, _d, ctypes.POINTER(_abi.RelaxOpts),
+                           ctypes.POINTER(_abi.RelaxResult)]
+    L.rh_relax_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_abi.RelaxOpts),
+                               ctypes.POINTER(_abi.RelaxResult)]
+    L.rh_field.argtypes = [_vp, _d, _d, ctypes.c_int64, _d, _d, ctypes.c_int]
+    L.rh_field_dev.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, _vp, ctypes.c_int]
+    L.rh_current_field.argtypes = [ctypes.c_int, _vp, ctypes.c_int64, _i32, _d, _d, _d, _d, _d,
+                                   ctypes.c_int64, _d]
+    L.rh_current_field_dev.argtypes = [ctypes.c_int, _vp, ctypes.c_int64, _i32, _d, _d, _d, _d,
+                                       _vp, ctypes.c_int64, _vp]
+    L.rh_set_timing.argtypes = [ctypes.c_int32]
+    L.rh_kernel_time.argtypes = [ctypes.c_char_p, _d, _i64]
+    for name in ["rh_model_create", "rh_model_destroy", "rh_model_colloc", "rh_model_counts",
+                 "rh_matrix_assemble", "rh_matrix_destroy", "rh_matrix_export",
+                 "rh_matrix_device_ptr", "rh_matrix_reassemble", "rh_relax", "rh_relax_dev",
+                 "rh_field", "rh_field_dev", "rh_current_field", "rh_current_field_dev",
+                 "rh_set_timing", "rh_kernel_time"]:
+        getattr(L, name).restype = st
+    if L.rh_device_count() < 1:
+        raise RadHipError("no HIP device visible; the HIP path has no CPU fallback")
+    _L = L
+    return L
+
+
+def check(rc):
+    if rc != RH_OK:
+        raise RadHipError("radhip error %d: %s" % (rc, lib().rh_last_error().decode()))
+
+
+def _p(a):
+    return a.ctypes.data_as(_d)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def stream_handle():
+    """The current torch stream when the caller has already initialised torch's
+    GPU context, else the null stream (torch is never imported from here)."""
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is not None:
+        try:
+            if torch.cuda.is_initialized():
+                return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        except Exception:
+            pass
+    return ctypes.c_void_p(0)
+
+
+class Model:
+    """Device-resident magnetizable elements (rh_model)."""
+
+    def __init__(self, packed, device=0, stream=None):
+        L = lib()
+        self.packed = packed
+        self.n = len(packed["type"])
+        desc, self._keep = _abi.model_desc(packed)
+        self.h = _vp()
+        self.device = device
+        check(L.rh_model_create(ctypes.byref(desc), device,
+                                stream if stream is not None else stream_handle(),
+                                ctypes.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().rh_model_destroy(self.h)
+            self.h = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def colloc(self):
+        X = np.zeros((self.n, 3))
+        check(lib().rh_model_colloc(self.h, _p(X)))
+        return X
+
+    def counts(self):
+        c = np.zeros(8, np.int64)
+        check(lib().rh_model_counts(self.h, c.ctypes.data_as(_i64)))
+        return c
+
+    def assemble(self, prec=RH_MIXED):
+        return Matrix(self, prec)
+
+    def field(self, M, pts, want_b=True, want_h=True, precision=RH_FP64):
+        M = _c(M).reshape(-1)
+        pts = _c(pts).reshape(-1, 3)
+        B = np.zeros_like(pts) if want_b else None
+        H = np.zeros_like(pts)
+        check(lib().rh_field(self.h, _p(M), _p(pts), len(pts), _p(B) if want_b else None,
+                             _p(H), precision))
+        return B, H
+
+
+class Matrix:
+    """Device-resident interaction matrix (rh_matrix)."""
+
+    def __init__(self, model, prec=RH_MIXED):
+        self.model = model
+        self.n = model.n
+        self.h = _vp()
+        check(lib().rh_matrix_assemble(model.h, int(prec), ctypes.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().rh_matrix_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reassemble(self):
+        check(lib().rh_matrix_reassemble(self.h))
+
+    def export(self, row0=0, nrows=None):
+        n3 = 3 * self.n
+        nrows = n3 - row0 if nrows is None else nrows
+        out = np.zeros((nrows, n3))
+        check(lib().rh_matrix_export(self.h, row0, nrows, _p(out)))
+        return out
+
+    def relax(self, mats, hext, M0, prec, max_iter, keep_m=False, omega0=0.0):
+        mats = _c(mats).reshape(-1)
+        hext = _c(hext).reshape(-1)
+        M = _c(M0).reshape(-1).copy()
+        H = np.zeros_like(M)
+        opts = _abi.RelaxOpts(float(prec), int(max_iter), 1 if keep_m else 0, float(omega0), 9, 0)
+        res = _abi.RelaxResult()
+        check(lib().rh_relax(self.h, _p(mats), _p(hext), _p(M), _p(H), ctypes.byref(opts),
+                             ctypes.byref(res)))
+        return res, M.reshape(-1, 3), H.reshape(-1, 3)
+
+
+def current_field(cur, pts, device=0):
+    """B of packed current sources (objects.pack_currents) at lab points."""
+    pts = _c(pts).reshape(-1, 3)
+    B = np.zeros_like(pts)
+    ns = len(cur["kind"])
+    if ns == 0:
+        return B
+    kind = np.ascontiguousarray(cur["kind"], dtype=np.int32)
+    par = _c(cur["par"])
+    tm, tv, ts = _c(cur["cM"]), _c(cur["cV"]), _c(cur["cS"])
+    check(lib().rh_current_field(device, stream_handle(), ns, kind.ctypes.data_as(_i32), _p(par),
+                                 _p(tm), _p(tv), _p(ts), _p(pts), len(pts), _p(B)))
+    return B
+
+
+def mat_M(rec, H):
+    """M(H) of a material record (radmater.h:224, :351, :644-657)."""
+    rec = np.asarray(rec, dtype=np.float64)
+    H = np.asarray(H, dtype=np.float64)
+    kind = int(rec[0])
+    mr = rec[13:16]
+    if kind == 0:
+        return rec[1] * H + mr
+    if kind == 1:
+        return rec[4:13].reshape(3, 3) @ H + mr
+    h = float(np.linalg.norm(H))
+    f = 0.0
+    for t in range(int(rec[1])):
+        ms, ks = rec[2 + t], rec[5 + t]
+        if ms != 0:
+            f += ms * np.tanh(ks * h / ms)
+    return (f / h) * H + mr if h != 0 else mr.copy()
+
+
+def set_timing(on):
+    check(lib().rh_set_timing(1 if on else 0))
+
+
+def kernel_time(family):
+    avg = ctypes.c_double()
+    cnt = ctypes.c_int64()
+    check(lib().rh_kernel_time(family.encode(), ctypes.byref(avg), ctypes.byref(cnt)))
+    return avg.value, cnt.value

@@ -1,0 +1,789 @@
+// rh_api.cpp -- the C ABI (include/radhip.h): device-resident models and
+// matrices, the relaxation driver and the field entry points.  Host C++ that
+// calls the gfx950 kernels through rh_kernels.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/radhip.h"
+#include "rh_geom.hpp"
+#include "rh_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+rh_status fail(rh_status st, const std::string& msg)
+{
+    g_err = msg;
+    return st;
+}
+
+#define RH_HIP(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(e_ == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_HIP,           \
+                        std::string(#call) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+// ---------------------------------------------------------------- timing
+bool g_timing = false;
+struct Family {
+    double total_ms = 0.0;
+    int64_t count = 0;
+};
+std::map<std::string, Family> g_fam;
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+// Records launches of one kernel family inside one API call and folds their
+// durations into the family statistics at the end of the call.
+struct Timer {
+    std::string fam;
+    hipStream_t s;
+    std::vector<EventPair> ev;
+    Timer(const char* f, hipStream_t st) : fam(f), s(st) {}
+    void begin()
+    {
+        if (!g_timing) return;
+        EventPair p;
+        (void)hipEventCreate(&p.a);
+        (void)hipEventCreate(&p.b);
+        (void)hipEventRecord(p.a, s);
+        ev.push_back(p);
+    }
+    void end()
+    {
+        if (!g_timing || ev.empty()) return;
+        (void)hipEventRecord(ev.back().b, s);
+    }
+    ~Timer()
+    {
+        if (ev.empty()) return;
+        Family& F = g_fam[fam];
+        for (EventPair& p : ev) {
+            (void)hipEventSynchronize(p.b);
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                F.total_ms += ms;
+                F.count++;
+            }
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+    }
+};
+
+template <class T>
+hipError_t dalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+template <class T>
+hipError_t upload(T** p, const std::vector<T>& v, hipStream_t s)
+{
+    hipError_t e = dalloc(p, v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- handles
+struct rh_model {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rh::PackedModel host;
+    rh::DevModel dev;
+    int32_t* d_kind = nullptr;
+    int64_t* d_off = nullptr;
+    double* d_data = nullptr;
+    int64_t* d_copy_off = nullptr;
+    double* d_copies = nullptr;
+    double* d_colloc = nullptr;
+    double* d_rowt = nullptr;
+    // growable workspace for the field entry points
+    double* d_work = nullptr;
+    size_t work_cap = 0;
+};
+
+struct rh_matrix {
+    rh_model* m = nullptr;
+    rh_precision prec = RH_MIXED;
+    int elem_bytes = 4;
+    int64_t n3 = 0, ld = 0;
+    void* d_Q = nullptr;
+    double* d_qdiag = nullptr;
+};
+
+namespace {
+
+hipError_t grow(rh_model* m, size_t doubles)
+{
+    if (doubles <= m->work_cap) return hipSuccess;
+    dfree(m->d_work);
+    m->d_work = nullptr;
+    m->work_cap = 0;
+    hipError_t e = dalloc(&m->d_work, doubles);
+    if (e == hipSuccess) m->work_cap = doubles;
+    return e;
+}
+
+void free_model(rh_model* m)
+{
+    if (!m) return;
+    (void)hipSetDevice(m->device);
+    dfree(m->d_kind);
+    dfree(m->d_off);
+    dfree(m->d_data);
+    dfree(m->d_copy_off);
+    dfree(m->d_copies);
+    dfree(m->d_colloc);
+    dfree(m->d_rowt);
+    dfree(m->d_work);
+    delete m;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rh_last_error(void) { return g_err.c_str(); }
+
+int32_t rh_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+rh_status rh_set_timing(int32_t on)
+{
+    g_timing = on != 0;
+    g_fam.clear();
+    return RH_OK;
+}
+
+rh_status rh_kernel_time(const char* family, double* avg_ms, int64_t* count)
+{
+    if (!family || !avg_ms || !count) return fail(RH_ERR_ARG, "null argument");
+    auto it = g_fam.find(family);
+    if (it == g_fam.end() || it->second.count == 0) {
+        *avg_ms = 0.0;
+        *count = 0;
+        return RH_OK;
+    }
+    *avg_ms = it->second.total_ms / double(it->second.count);
+    *count = it->second.count;
+    return RH_OK;
+}
+
+rh_status rh_model_create(const rh_model_desc* desc, int device, void* stream, rh_model** out)
+{
+    if (!desc || !out) return fail(RH_ERR_ARG, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(RH_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(RH_ERR_ARG, "device index out of range");
+    RH_HIP(hipSetDevice(device));
+    rh_model* m = new rh_model();
+    m->device = device;
+    m->stream = static_cast<hipStream_t>(stream);
+    rh_status st = rh::pack_model(desc, &m->host, &g_err);
+    if (st != RH_OK) {
+        delete m;
+        return st;
+    }
+    const rh::PackedModel& P = m->host;
+    hipStream_t s = m->stream;
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = upload(&m->d_kind, P.kind, s);
+    if (e == hipSuccess) e = upload(&m->d_off, P.off, s);
+    if (e == hipSuccess) e = upload(&m->d_data, P.data, s);
+    if (e == hipSuccess) e = upload(&m->d_copy_off, P.copy_off, s);
+    if (e == hipSuccess) e = upload(&m->d_copies, P.copies, s);
+    if (e == hipSuccess) e = upload(&m->d_colloc, P.colloc, s);
+    if (e == hipSuccess) e = upload(&m->d_rowt, P.rowt, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        std::string msg = std::string("model upload: ") + hipGetErrorString(e);
+        free_model(m);
+        return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_HIP, msg);
+    }
+    rh::DevModel& d = m->dev;
+    d.n = P.n;
+    d.kind = m->d_kind;
+    d.off = m->d_off;
+    d.data = m->d_data;
+    d.copy_off = m->d_copy_off;
+    d.copies = m->d_copies;
+    d.colloc = m->d_colloc;
+    d.rowt = m->d_rowt;
+    d.sym = P.sym;
+    d.rowt_identity = P.rowt_identity;
+    *out = m;
+    return RH_OK;
+}
+
+rh_status rh_model_destroy(rh_model* m)
+{
+    free_model(m);
+    return RH_OK;
+}
+
+rh_status rh_model_colloc(const rh_model* m, double* colloc_out)
+{
+    if (!m || !colloc_out) return fail(RH_ERR_ARG, "null argument");
+    std::memcpy(colloc_out, m->host.colloc.data(), m->host.colloc.size() * sizeof(double));
+    return RH_OK;
+}
+
+rh_status rh_model_counts(const rh_model* m, int64_t* counts8)
+{
+    if (!m || !counts8) return fail(RH_ERR_ARG, "null argument");
+    for (int k = 0; k < 8; k++) counts8[k] = m->host.counts[k];
+    return RH_OK;
+}
+
+// ---------------------------------------------------------------- assembly
+static rh_status assemble_into(rh_matrix* q)
+{
+    rh_model* m = q->m;
+    RH_HIP(hipSetDevice(m->device));
+    Timer t("asm", m->stream);
+    t.begin();
+    RH_HIP(rh::launch_assemble(m->dev, q->d_Q, q->elem_bytes, q->ld, q->d_qdiag, m->stream));
+    t.end();
+    RH_HIP(hipStreamSynchronize(m->stream));
+    return RH_OK;
+}
+
+rh_status rh_matrix_assemble(rh_model* m, rh_precision prec, rh_matrix** out)
+{
+    if (!m || !out) return fail(RH_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (prec != RH_FP64 && prec != RH_MIXED) return fail(RH_ERR_ARG, "unknown precision");
+    RH_HIP(hipSetDevice(m->device));
+    rh_matrix* q = new rh_matrix();
+    q->m = m;
+    q->prec = prec;
+    q->elem_bytes = (prec == RH_FP64) ? 8 : 4;
+    q->n3 = 3 * m->host.n;
+    q->ld = std::max<int64_t>(256, (q->n3 + 255) / 256 * 256);
+    const size_t bytes = size_t(q->n3) * size_t(q->ld) * size_t(q->elem_bytes);
+    hipError_t e = hipMalloc(&q->d_Q, std::max<size_t>(bytes, 16));
+    if (e == hipSuccess) e = dalloc(&q->d_qdiag, size_t(9 * m->host.n));
+    // padded columns must read as zero in the matrix pass
+    if (e == hipSuccess) e = hipMemsetAsync(q->d_Q, 0, bytes, m->stream);
+    if (e != hipSuccess) {
+        std::string msg = std::string("matrix allocation (") + std::to_string(bytes) +
+                          " bytes): " + hipGetErrorString(e);
+        dfree(q->d_Q);
+        dfree(q->d_qdiag);
+        delete q;
+        return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_HIP, msg);
+    }
+    rh_status st = assemble_into(q);
+    if (st != RH_OK) {
+        rh_matrix_destroy(q);
+        return st;
+    }
+    *out = q;
+    return RH_OK;
+}
+
+rh_status rh_matrix_reassemble(rh_matrix* q)
+{
+    if (!q) return fail(RH_ERR_ARG, "null argument");
+    return assemble_into(q);
+}
+
+rh_status rh_matrix_destroy(rh_matrix* q)
+{
+    if (!q) return RH_OK;
+    (void)hipSetDevice(q->m->device);
+    dfree(q->d_Q);
+    dfree(q->d_qdiag);
+    delete q;
+    return RH_OK;
+}
+
+rh_status rh_matrix_device_ptr(const rh_matrix* q, void** dptr, int32_t* elem_bytes)
+{
+    if (!q || !dptr || !elem_bytes) return fail(RH_ERR_ARG, "null argument");
+    *dptr = q->d_Q;
+    *elem_bytes = q->elem_bytes;
+    return RH_OK;
+}
+
+rh_status rh_matrix_export(const rh_matrix* q, int64_t row0, int64_t nrows, double* out)
+{
+    if (!q || !out) return fail(RH_ERR_ARG, "null argument");
+    if (row0 < 0 || nrows < 0 || row0 + nrows > q->n3) return fail(RH_ERR_ARG, "row range out of bounds");
+    rh_model* m = q->m;
+    RH_HIP(hipSetDevice(m->device));
+    const int64_t n3 = q->n3;
+    std::vector<char> buf(size_t(q->ld) * size_t(q->elem_bytes));
+    for (int64_t r = row0; r < row0 + nrows; r++) {
+        const char* src = static_cast<const char*>(q->d_Q) + size_t(r) * q->ld * q->elem_bytes;
+        RH_HIP(hipMemcpy(buf.data(), src, size_t(n3) * q->elem_bytes, hipMemcpyDeviceToHost));
+        double* o = out + (r - row0) * n3;
+        for (int64_t c = 0; c < n3; c++)
+            o[c] = (q->elem_bytes == 4) ? double(reinterpret_cast<float*>(buf.data())[c])
+                                        : reinterpret_cast<double*>(buf.data())[c];
+    }
+    // diagonal blocks are kept in FP64 beside the matrix
+    std::vector<double> qd(size_t(9 * m->host.n));
+    RH_HIP(hipMemcpy(qd.data(), q->d_qdiag, qd.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t r = row0; r < row0 + nrows; r++) {
+        const int64_t e = r / 3, a = r % 3;
+        for (int b = 0; b < 3; b++) out[(r - row0) * n3 + 3 * e + b] = qd[size_t(9 * e + 3 * a + b)];
+    }
+    return RH_OK;
+}
+
+// ---------------------------------------------------------------- relaxation
+namespace {
+
+// Anderson-accelerated (type II, depth kHist) under-relaxed block-Jacobi
+// iteration on M = G(M), G = per-element implicit material update given the
+// field of all other elements (the fixed point of radGPU_RelaxAuto,
+// cpp/src/core/radgpurlx.cu:799-1349, with an acceleration of our own).
+constexpr int kHist = 8;
+
+struct RelaxWork {
+    double* x = nullptr;   // [ld] current iterate (padded)
+    double* y = nullptr;   // [n3]
+    double* F = nullptr;   // [n3]
+    double* Fp = nullptr;  // [n3] previous F
+    double* xp = nullptr;  // [n3] previous x
+    double* dX[kHist] = {nullptr};
+    double* dF[kHist] = {nullptr};
+    double* part = nullptr;
+    double* dots = nullptr;
+    ~RelaxWork()
+    {
+        dfree(x);
+        dfree(y);
+        dfree(F);
+        dfree(Fp);
+        dfree(xp);
+        for (int k = 0; k < kHist; k++) {
+            dfree(dX[k]);
+            dfree(dF[k]);
+        }
+        dfree(part);
+        dfree(dots);
+    }
+};
+
+// Solves (G + lambda I) g = r for a small symmetric positive semi-definite G.
+bool small_solve(int m, std::vector<double> G, std::vector<double> r, std::vector<double>& g)
+{
+    double tr = 0.0;
+    for (int i = 0; i < m; i++) tr += G[i * m + i];
+    const double lam = 1e-10 * (tr > 0 ? tr / m : 1.0);
+    for (int i = 0; i < m; i++) G[i * m + i] += lam;
+    // Cholesky
+    std::vector<double> L(size_t(m * m), 0.0);
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j <= i; j++) {
+            double s = G[i * m + j];
+            for (int k = 0; k < j; k++) s -= L[i * m + k] * L[j * m + k];
+            if (i == j) {
+                if (!(s > 0.0)) return false;
+                L[i * m + i] = std::sqrt(s);
+            } else {
+                L[i * m + j] = s / L[j * m + j];
+            }
+        }
+    std::vector<double> z(m);
+    for (int i = 0; i < m; i++) {
+        double s = r[i];
+        for (int k = 0; k < i; k++) s -= L[i * m + k] * z[k];
+        z[i] = s / L[i * m + i];
+    }
+    g.assign(m, 0.0);
+    for (int i = m - 1; i >= 0; i--) {
+        double s = z[i];
+        for (int k = i + 1; k < m; k++) s -= L[k * m + i] * g[k];
+        g[i] = s / L[i * m + i];
+    }
+    return true;
+}
+
+}  // namespace
+
+rh_status rh_relax_dev(rh_matrix* q, const double* d_mats, const double* d_hext, double* d_M,
+                       double* d_H, const rh_relax_opts* opts, rh_relax_result* res)
+{
+    if (!q || !d_mats || !d_hext || !d_M || !d_H || !opts || !res) return fail(RH_ERR_ARG, "null argument");
+    if (opts->method != 9 && opts->method != 0) return fail(RH_ERR_UNSUPPORTED, "relaxation method must be 9");
+    rh_model* m = q->m;
+    RH_HIP(hipSetDevice(m->device));
+    hipStream_t s = m->stream;
+    const int64_t n = m->host.n, n3 = q->n3;
+    std::memset(res, 0, sizeof(*res));
+    if (n == 0) {
+        res->converged = 1;
+        return RH_OK;
+    }
+    RelaxWork w;
+    RH_HIP(dalloc(&w.x, size_t(q->ld)));
+    RH_HIP(dalloc(&w.y, size_t(n3)));
+    RH_HIP(dalloc(&w.F, size_t(n3)));
+    RH_HIP(dalloc(&w.Fp, size_t(n3)));
+    RH_HIP(dalloc(&w.xp, size_t(n3)));
+    for (int k = 0; k < kHist; k++) {
+        RH_HIP(dalloc(&w.dX[k], size_t(n3)));
+        RH_HIP(dalloc(&w.dF[k], size_t(n3)));
+    }
+    const int64_t nb = rh::relax_blocks(n);
+    const int64_t db = rh::dot_blocks(n3);
+    RH_HIP(dalloc(&w.part, size_t(3 * nb)));
+    RH_HIP(dalloc(&w.dots, size_t(17 * db)));
+    RH_HIP(hipMemsetAsync(w.x, 0, size_t(q->ld) * sizeof(double), s));
+    if (opts->keep_m) {
+        RH_HIP(hipMemcpyAsync(w.x, d_M, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+        // start from the remanent magnetization (radintrc.h:385-395)
+        for (int k = 0; k < 3; k++)
+            RH_HIP(hipMemcpy2DAsync(w.x + k, 3 * sizeof(double), d_mats + 13 + k, 16 * sizeof(double),
+                                    sizeof(double), size_t(n), hipMemcpyDeviceToDevice, s));
+    }
+    const double prec = opts->prec;
+    const int max_iter = std::max(1, int(opts->max_iter));
+    double beta = (opts->omega0 > 0.0) ? opts->omega0 : 0.5;
+    std::vector<double> hp(size_t(3 * nb)), hd(size_t(17 * db));
+    std::vector<double> G(size_t(kHist * kHist), 0.0);
+    int hist = 0, head = 0;  // ring of the last `hist` differences, newest at head-1
+    double best = INFINITY;
+    Timer tm("matvec", s);
+    int it = 0;
+    double misfit = INFINITY, maxM = 0.0, maxH = 0.0;
+    bool done = false;
+    while (!done) {
+        it++;
+        tm.begin();
+        RH_HIP(rh::launch_matvec(q->d_Q, q->elem_bytes, n3, q->ld, w.x, nullptr, w.y, s));
+        tm.end();
+        RH_HIP(rh::launch_relax_update(n, d_mats, d_hext, q->d_qdiag, w.y, w.x, w.F, d_H, w.part, s));
+        RH_HIP(hipMemcpyAsync(hp.data(), w.part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        RH_HIP(hipStreamSynchronize(s));
+        double f2 = 0.0;
+        maxM = 1e-17;
+        maxH = 1e-17;
+        for (int64_t b = 0; b < nb; b++) {
+            f2 += hp[size_t(3 * b)];
+            maxM = std::max(maxM, hp[size_t(3 * b + 1)]);
+            maxH = std::max(maxH, hp[size_t(3 * b + 2)]);
+        }
+        misfit = std::sqrt(f2 / double(n));
+        if (!std::isfinite(misfit)) return fail(RH_ERR_NOTCONV, "relaxation produced a non-finite residual");
+        if (misfit <= prec || it >= max_iter) {
+            // final magnetization: the last proposal x + F
+            const double* vv[2] = {w.x, w.F};
+            const double cc[2] = {1.0, 1.0};
+            RH_HIP(rh::launch_lincomb(2, vv, cc, n3, d_M, s));
+            done = true;
+            break;
+        }
+        // safeguard: restart the history when the residual grows a lot
+        if (misfit > 1e4 * best) {
+            hist = 0;
+            head = 0;
+            beta = std::max(0.05, 0.5 * beta);
+        }
+        best = std::min(best, misfit);
+        if (it > 1) {
+            // new difference column at slot `head`
+            const double* v1[2] = {w.F, w.Fp};
+            const double c1[2] = {1.0, -1.0};
+            RH_HIP(rh::launch_lincomb(2, v1, c1, n3, w.dF[head], s));
+            const double* v2[2] = {w.x, w.xp};
+            RH_HIP(rh::launch_lincomb(2, v2, c1, n3, w.dX[head], s));
+            head = (head + 1) % kHist;
+            hist = std::min(hist + 1, kHist);
+        }
+        RH_HIP(hipMemcpyAsync(w.Fp, w.F, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        RH_HIP(hipMemcpyAsync(w.xp, w.x, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (hist == 0) {
+            const double* v[2] = {w.x, w.F};
+            const double c[2] = {1.0, beta};
+            RH_HIP(rh::launch_lincomb(2, v, c, n3, w.x, s));
+            continue;
+        }
+        // slots in age order: oldest..newest
+        int slot[kHist];
+        for (int k = 0; k < hist; k++) slot[k] = (head - hist + k + kHist) % kHist;
+        const int newest = slot[hist - 1];
+        // dots: newest dF with every dF, and every dF with F
+        const double* a[17];
+        const double* b[17];
+        int nd = 0;
+        for (int k = 0; k < hist; k++) {
+            a[nd] = w.dF[newest];
+            b[nd] = w.dF[slot[k]];
+            nd++;
+        }
+        for (int k = 0; k < hist; k++) {
+            a[nd] = w.dF[slot[k]];
+            b[nd] = w.F;
+            nd++;
+        }
+        RH_HIP(rh::launch_dots(nd, a, b, n3, w.dots, s));
+        RH_HIP(hipMemcpyAsync(hd.data(), w.dots, size_t(nd * db) * sizeof(double), hipMemcpyDeviceToHost, s));
+        RH_HIP(hipStreamSynchronize(s));
+        std::vector<double> dv(size_t(nd), 0.0);
+        for (int d = 0; d < nd; d++)
+            for (int64_t bb = 0; bb < db; bb++) dv[size_t(d)] += hd[size_t(d * db + bb)];
+        for (int k = 0; k < hist; k++) {
+            G[size_t(newest * kHist + slot[k])] = dv[size_t(k)];
+            G[size_t(slot[k] * kHist + newest)] = dv[size_t(k)];
+        }
+        std::vector<double> Gm(static_cast<size_t>(hist) * hist);
+        std::vector<double> r(static_cast<size_t>(hist));
+        std::vector<double> gam;
+        for (int i = 0; i < hist; i++) {
+            r[size_t(i)] = dv[size_t(hist + i)];
+            for (int j = 0; j < hist; j++) Gm[size_t(i * hist + j)] = G[size_t(slot[i] * kHist + slot[j])];
+        }
+        if (!small_solve(hist, Gm, r, gam)) {
+            hist = 0;
+            head = 0;
+            const double* v[2] = {w.x, w.F};
+            const double c[2] = {1.0, beta};
+            RH_HIP(rh::launch_lincomb(2, v, c, n3, w.x, s));
+            continue;
+        }
+        // x <- x + beta F - sum_k gam_k (dX_k + beta dF_k)
+        const double* v[34];
+        double c[34];
+        int nv = 0;
+        v[nv] = w.x;
+        c[nv++] = 1.0;
+        v[nv] = w.F;
+        c[nv++] = beta;
+        for (int k = 0; k < hist; k++) {
+            v[nv] = w.dX[slot[k]];
+            c[nv++] = -gam[size_t(k)];
+            v[nv] = w.dF[slot[k]];
+            c[nv++] = -beta * gam[size_t(k)];
+        }
+        RH_HIP(rh::launch_lincomb(nv, v, c, n3, w.x, s));
+    }
+    RH_HIP(hipStreamSynchronize(s));
+    res->misfit = misfit;
+    res->max_m = maxM;
+    res->max_h = maxH;
+    res->iters = it;
+    res->matvecs = it;
+    res->converged = misfit <= prec ? 1 : 0;
+    return RH_OK;
+}
+
+rh_status rh_relax(rh_matrix* q, const double* mats, const double* hext, double* M_inout,
+                   double* H_out, const rh_relax_opts* opts, rh_relax_result* res)
+{
+    if (!q || !mats || !hext || !M_inout || !H_out) return fail(RH_ERR_ARG, "null argument");
+    rh_model* m = q->m;
+    RH_HIP(hipSetDevice(m->device));
+    const int64_t n = m->host.n;
+    double *dm = nullptr, *dh = nullptr, *dM = nullptr, *dH = nullptr;
+    hipStream_t s = m->stream;
+    auto cleanup = [&]() {
+        dfree(dm);
+        dfree(dh);
+        dfree(dM);
+        dfree(dH);
+    };
+    hipError_t e = dalloc(&dm, size_t(16 * n));
+    if (e == hipSuccess) e = dalloc(&dh, size_t(3 * n));
+    if (e == hipSuccess) e = dalloc(&dM, size_t(3 * n));
+    if (e == hipSuccess) e = dalloc(&dH, size_t(3 * n));
+    if (e == hipSuccess) e = hipMemcpyAsync(dm, mats, size_t(16 * n) * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dh, hext, size_t(3 * n) * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dM, M_inout, size_t(3 * n) * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(RH_ERR_HIP, std::string("relax upload: ") + hipGetErrorString(e));
+    }
+    rh_status st = rh_relax_dev(q, dm, dh, dM, dH, opts, res);
+    if (st == RH_OK) {
+        e = hipMemcpyAsync(M_inout, dM, size_t(3 * n) * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(H_out, dH, size_t(3 * n) * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = fail(RH_ERR_HIP, std::string("relax download: ") + hipGetErrorString(e));
+    }
+    cleanup();
+    return st;
+}
+
+// ---------------------------------------------------------------- field
+static rh_status field_dev_impl(rh_model* m, const double* d_M, const double* d_pts, int64_t np,
+                                double* d_B, double* d_H, bool fp32)
+{
+    const int64_t splits = rh::field_splits(np, m->host.n);
+    double* work = nullptr;
+    if (splits > 1) {
+        RH_HIP(grow(m, size_t(6 * np * splits)));
+        work = m->d_work;
+    }
+    Timer t("field", m->stream);
+    t.begin();
+    RH_HIP(rh::launch_field(m->dev, d_M, d_pts, np, d_B, d_H, work, splits, fp32, m->stream));
+    t.end();
+    return RH_OK;
+}
+
+rh_status rh_field_dev(rh_model* m, const double* d_M, const double* d_pts, int64_t np,
+                       double* d_B, double* d_H, rh_precision prec)
+{
+    if (!m || (!d_B && !d_H) || np < 0) return fail(RH_ERR_ARG, "bad argument");
+    if (np > 0 && (!d_M || !d_pts)) return fail(RH_ERR_ARG, "null argument");
+    if (prec != RH_FP64 && prec != RH_MIXED) return fail(RH_ERR_ARG, "unknown precision");
+    RH_HIP(hipSetDevice(m->device));
+    rh_status st = field_dev_impl(m, d_M, d_pts, np, d_B, d_H, prec == RH_MIXED);
+    if (st != RH_OK) return st;
+    RH_HIP(hipStreamSynchronize(m->stream));
+    return RH_OK;
+}
+
+rh_status rh_field(rh_model* m, const double* M, const double* pts, int64_t np, double* B_out,
+                   double* H_out, rh_precision prec)
+{
+    if (prec != RH_FP64 && prec != RH_MIXED) return fail(RH_ERR_ARG, "unknown precision");
+    if (!m || !pts || np < 0 || (!B_out && !H_out)) return fail(RH_ERR_ARG, "bad argument");
+    RH_HIP(hipSetDevice(m->device));
+    const int64_t n = m->host.n;
+    if (np == 0) return RH_OK;
+    double *dM = nullptr, *dP = nullptr, *dB = nullptr, *dH = nullptr;
+    auto cleanup = [&]() {
+        dfree(dM);
+        dfree(dP);
+        dfree(dB);
+        dfree(dH);
+    };
+    hipStream_t s = m->stream;
+    hipError_t e = dalloc(&dM, size_t(3 * std::max<int64_t>(n, 1)));
+    if (e == hipSuccess) e = dalloc(&dP, size_t(3 * np));
+    if (e == hipSuccess) e = dalloc(&dB, size_t(3 * np));
+    if (e == hipSuccess) e = dalloc(&dH, size_t(3 * np));
+    if (e == hipSuccess && n > 0) e = hipMemcpyAsync(dM, M, size_t(3 * n) * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dP, pts, size_t(3 * np) * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_HIP,
+                    std::string("field upload: ") + hipGetErrorString(e));
+    }
+    rh_status st = field_dev_impl(m, dM, dP, np, dB, dH, prec == RH_MIXED);
+    if (st == RH_OK) {
+        if (B_out) e = hipMemcpyAsync(B_out, dB, size_t(3 * np) * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && H_out)
+            e = hipMemcpyAsync(H_out, dH, size_t(3 * np) * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = fail(RH_ERR_HIP, std::string("field download: ") + hipGetErrorString(e));
+    }
+    cleanup();
+    return st;
+}
+
+// ---------------------------------------------------------------- currents
+static rh_status pack_currents(int64_t nsrc, const double* tm, const double* tv, const double* ts,
+                               std::vector<double>& cp)
+{
+    cp.assign(size_t(24 * nsrc), 0.0);
+    for (int64_t k = 0; k < nsrc; k++) {
+        double* o = cp.data() + 24 * k;
+        rh::inv3(tm + 9 * k, o);
+        for (int i = 0; i < 3; i++) o[9 + i] = tv[3 * k + i];
+        for (int i = 0; i < 9; i++) o[12 + i] = ts[k] * tm[9 * k + i];
+        for (int i = 0; i < 9; i++)
+            if (!std::isfinite(o[i])) return fail(RH_ERR_ARG, "singular current-source transform");
+    }
+    return RH_OK;
+}
+
+rh_status rh_current_field_dev(int device, void* stream, int64_t nsrc, const int32_t* kind,
+                               const double* par, const double* tm, const double* tv,
+                               const double* ts, const double* d_pts, int64_t np, double* d_B)
+{
+    if (nsrc < 0 || np < 0) return fail(RH_ERR_ARG, "negative size");
+    if (nsrc == 0 || np == 0) return RH_OK;
+    if (!kind || !par || !tm || !tv || !ts || !d_pts || !d_B) return fail(RH_ERR_ARG, "null argument");
+    for (int64_t k = 0; k < nsrc; k++) {
+        if (kind[k] != 0 && kind[k] != 1) return fail(RH_ERR_UNSUPPORTED, "unknown current-source kind");
+        if (kind[k] == 1 && !(par[12 * k + 9] >= 1.0)) return fail(RH_ERR_ARG, "arc needs nseg >= 1");
+    }
+    std::vector<double> cp;
+    rh_status st = pack_currents(nsrc, tm, tv, ts, cp);
+    if (st != RH_OK) return st;
+    RH_HIP(hipSetDevice(device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int32_t* dk = nullptr;
+    double *dp = nullptr, *dc = nullptr;
+    auto cleanup = [&]() {
+        dfree(dk);
+        dfree(dp);
+        dfree(dc);
+    };
+    hipError_t e = dalloc(&dk, size_t(nsrc));
+    if (e == hipSuccess) e = dalloc(&dp, size_t(12 * nsrc));
+    if (e == hipSuccess) e = dalloc(&dc, size_t(24 * nsrc));
+    if (e == hipSuccess) e = hipMemcpyAsync(dk, kind, size_t(nsrc) * sizeof(int32_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dp, par, size_t(12 * nsrc) * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dc, cp.data(), cp.size() * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = rh::launch_current(nsrc, dk, dp, dc, d_pts, np, d_B, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    cleanup();
+    if (e != hipSuccess) return fail(RH_ERR_HIP, std::string("current field: ") + hipGetErrorString(e));
+    return RH_OK;
+}
+
+rh_status rh_current_field(int device, void* stream, int64_t nsrc, const int32_t* kind,
+                           const double* par, const double* tm, const double* tv, const double* ts,
+                           const double* pts, int64_t np, double* B_acc)
+{
+    if (np == 0 || nsrc == 0) return RH_OK;
+    if (!pts || !B_acc) return fail(RH_ERR_ARG, "null argument");
+    RH_HIP(hipSetDevice(device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double *dP = nullptr, *dB = nullptr;
+    hipError_t e = dalloc(&dP, size_t(3 * np));
+    if (e == hipSuccess) e = dalloc(&dB, size_t(3 * np));
+    if (e == hipSuccess) e = hipMemcpyAsync(dP, pts, size_t(3 * np) * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dB, B_acc, size_t(3 * np) * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+        dfree(dP);
+        dfree(dB);
+        return fail(RH_ERR_HIP, std::string("current upload: ") + hipGetErrorString(e));
+    }
+    rh_status st = rh_current_field_dev(device, stream, nsrc, kind, par, tm, tv, ts, dP, np, dB);
+    if (st == RH_OK) {
+        e = hipMemcpyAsync(B_acc, dB, size_t(3 * np) * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = fail(RH_ERR_HIP, std::string("current download: ") + hipGetErrorString(e));
+    }
+    dfree(dP);
+    dfree(dB);
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,102 @@
+// rh_eval.hpp -- interaction blocks and point fields over transform copies.
+//
+// block_Q:  Q_ij = rowT_i * sum_c F_c Q_j(Minv_c (x_i - V_c))
+//           (cpp/src/core/radintrc.cpp:831-878: TrMatrix of each symmetry copy,
+//           TrMatrix_inv of the row element's main transform)
+// field:    B(x) = sum_j sum_c F_c [Q_j(x') M_j + inside * M_j],
+//           x' = Minv_c (x - V_c)   (cpp/src/core/radg3d.cpp:57-115)
+#pragma once
+
+#include "rh_math.hpp"
+
+namespace rh {
+
+struct ModelView {
+    const int32_t* kind;
+    const int64_t* off;
+    const double* data;
+    const int64_t* copy_off;
+    const double* copies;  // kCopySize (24) doubles each: Minv, V, F, pad
+};
+
+RH_HD void apply_inv(const double* c, const double x[3], double o[3])
+{
+    const double d0 = x[0] - c[9], d1 = x[1] - c[10], d2 = x[2] - c[11];
+    o[0] = c[0] * d0 + c[1] * d1 + c[2] * d2;
+    o[1] = c[3] * d0 + c[4] * d1 + c[5] * d2;
+    o[2] = c[6] * d0 + c[7] * d1 + c[8] * d2;
+}
+
+RH_HD void mat3_vec(const double* m, const double v[3], double o[3])
+{
+    o[0] = m[0] * v[0] + m[1] * v[1] + m[2] * v[2];
+    o[1] = m[3] * v[0] + m[4] * v[1] + m[5] * v[2];
+    o[2] = m[6] * v[0] + m[7] * v[1] + m[8] * v[2];
+}
+
+// Sum over copies of F_c Q_j(x'), without the row transform.
+template <class R, bool SYM>
+RH_HD void block_Q(const ModelView& mv, int64_t j, const double x[3], double q9[9])
+{
+    const int kind = mv.kind[j];
+    const double* rec = mv.data + mv.off[j];
+    if (!SYM) {
+        double q6[6];
+        eval_elem<R>(kind, rec, x, q6);
+        q6_to_q9(q6, q9);
+        return;
+    }
+    for (int k = 0; k < 9; k++) q9[k] = 0.0;
+    for (int64_t c = mv.copy_off[j]; c < mv.copy_off[j + 1]; c++) {
+        const double* cp = mv.copies + 24 * c;
+        double xl[3], q6[6], ql[9];
+        apply_inv(cp, x, xl);
+        eval_elem<R>(kind, rec, xl, q6);
+        q6_to_q9(q6, ql);
+        const double* F = cp + 12;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                q9[3 * a + b] += F[3 * a] * ql[b] + F[3 * a + 1] * ql[3 + b] + F[3 * a + 2] * ql[6 + b];
+    }
+}
+
+// Adds the B and H of elements [j0, j1) at x to B, H.
+template <class R, bool SYM>
+RH_HD void field_add(const ModelView& mv, const double* M, int64_t j0, int64_t j1, const double x[3],
+                     double B[3], double H[3])
+{
+    for (int64_t j = j0; j < j1; j++) {
+        const int kind = mv.kind[j];
+        const double* rec = mv.data + mv.off[j];
+        const double m[3] = {M[3 * j], M[3 * j + 1], M[3 * j + 2]};
+        if (!SYM) {
+            double q6[6];
+            bool in = eval_elem<R>(kind, rec, x, q6);
+            double h0 = q6[0] * m[0] + q6[3] * m[1] + q6[4] * m[2];
+            double h1 = q6[3] * m[0] + q6[1] * m[1] + q6[5] * m[2];
+            double h2 = q6[4] * m[0] + q6[5] * m[1] + q6[2] * m[2];
+            H[0] += h0; H[1] += h1; H[2] += h2;
+            B[0] += h0 + (in ? m[0] : 0.0);
+            B[1] += h1 + (in ? m[1] : 0.0);
+            B[2] += h2 + (in ? m[2] : 0.0);
+            continue;
+        }
+        for (int64_t c = mv.copy_off[j]; c < mv.copy_off[j + 1]; c++) {
+            const double* cp = mv.copies + 24 * c;
+            double xl[3], q6[6];
+            apply_inv(cp, x, xl);
+            bool in = eval_elem<R>(kind, rec, xl, q6);
+            double hl[3] = {q6[0] * m[0] + q6[3] * m[1] + q6[4] * m[2],
+                            q6[3] * m[0] + q6[1] * m[1] + q6[5] * m[2],
+                            q6[4] * m[0] + q6[5] * m[1] + q6[2] * m[2]};
+            double bl[3] = {hl[0] + (in ? m[0] : 0.0), hl[1] + (in ? m[1] : 0.0),
+                            hl[2] + (in ? m[2] : 0.0)};
+            double hg[3], bg[3];
+            mat3_vec(cp + 12, hl, hg);
+            mat3_vec(cp + 12, bl, bg);
+            for (int k = 0; k < 3; k++) { H[k] += hg[k]; B[k] += bg[k]; }
+        }
+    }
+}
+
+}  // namespace rh

@@ -1,0 +1,333 @@
+// rh_geom.cpp -- packing of blocks and convex polyhedra into kernel records.
+//
+// Polyhedron preparation restates what the reference does when it builds a
+// polyhedron (cpp/src/core/radvlpgn.cpp:98-345): face normals, outward
+// orientation, the convexity check (Radia::Error106) and the centroid as the
+// mean of the given vertices (radvlpgn.h:321-327).  The per-edge tables are
+// this implementation's own (rh_math.hpp).
+#include "rh_geom.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <utility>
+
+#include "rh_math.hpp"
+
+namespace rh {
+
+void inv3(const double* m, double* o)
+{
+    double c[9] = {m[4] * m[8] - m[5] * m[7], m[2] * m[7] - m[1] * m[8], m[1] * m[5] - m[2] * m[4],
+                   m[5] * m[6] - m[3] * m[8], m[0] * m[8] - m[2] * m[6], m[2] * m[3] - m[0] * m[5],
+                   m[3] * m[7] - m[4] * m[6], m[1] * m[6] - m[0] * m[7], m[0] * m[4] - m[1] * m[3]};
+    double det = m[0] * c[0] + m[1] * c[3] + m[2] * c[6];
+    double inv = 1.0 / det;
+    for (int i = 0; i < 9; i++) o[i] = c[i] * inv;
+}
+
+namespace {
+
+struct Vec {
+    double x, y, z;
+};
+Vec sub(const Vec& a, const Vec& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+Vec crs(const Vec& a, const Vec& b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+double dt(const Vec& a, const Vec& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+double nrm(const Vec& a) { return std::sqrt(dt(a, a)); }
+Vec scl(const Vec& a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+
+// Accumulates m n^T (symmetrized) into w6 = xx yy zz xy xz yz.
+void add_mn(double* w6, const Vec& m, const Vec& n)
+{
+    w6[0] += m.x * n.x;
+    w6[1] += m.y * n.y;
+    w6[2] += m.z * n.z;
+    w6[3] += 0.5 * (m.x * n.y + m.y * n.x);
+    w6[4] += 0.5 * (m.x * n.z + m.z * n.x);
+    w6[5] += 0.5 * (m.y * n.z + m.z * n.y);
+}
+
+struct Face {
+    std::vector<int> v;  // counter-clockwise about n
+    Vec n;
+};
+
+// Normals (Newell), outward orientation and convexity of one polyhedron.
+rh_status orient_faces(const std::vector<Vec>& V, std::vector<Face>& F, std::string* err)
+{
+    Vec c{0, 0, 0};
+    for (const Vec& p : V) c = {c.x + p.x, c.y + p.y, c.z + p.z};
+    c = scl(c, 1.0 / double(V.size()));
+    double size = 0;
+    for (const Vec& p : V) size = std::max(size, nrm(sub(p, c)));
+    const double tol = 1e-9 * size;
+    for (Face& f : F) {
+        Vec n{0, 0, 0};
+        const size_t k = f.v.size();
+        for (size_t i = 0; i < k; i++) {
+            const Vec& a = V[f.v[i]];
+            const Vec& b = V[f.v[(i + 1) % k]];
+            n.x += (a.y - b.y) * (a.z + b.z);
+            n.y += (a.z - b.z) * (a.x + b.x);
+            n.z += (a.x - b.x) * (a.y + b.y);
+        }
+        double ln = nrm(n);
+        if (!(ln > 1e-300)) {
+            *err = "degenerate polyhedron face (zero area)";
+            return RH_ERR_GEOMETRY;
+        }
+        n = scl(n, 1.0 / ln);
+        // planarity (Radia::Error047)
+        for (size_t i = 1; i < k; i++) {
+            if (std::fabs(dt(sub(V[f.v[i]], V[f.v[0]]), n)) > 1e-6 * size) {
+                *err = "polyhedron face is not planar";
+                return RH_ERR_GEOMETRY;
+            }
+        }
+        if (dt(sub(V[f.v[0]], c), n) < 0) {
+            n = scl(n, -1.0);
+            std::reverse(f.v.begin(), f.v.end());
+        }
+        f.n = n;
+        // convexity (Radia::Error106): every vertex on the inner side
+        for (const Vec& p : V) {
+            if (dt(sub(p, V[f.v[0]]), n) > tol + 1e-7 * size) {
+                *err = "polyhedron is not convex";
+                return RH_ERR_GEOMETRY;
+            }
+        }
+    }
+    return RH_OK;
+}
+
+void append(std::vector<double>& d, const Vec& v)
+{
+    d.push_back(v.x);
+    d.push_back(v.y);
+    d.push_back(v.z);
+}
+
+// Canonical tetrahedron record (rh_math.hpp: kTetSize doubles).
+void pack_tet(std::vector<Vec> V, std::vector<double>& d)
+{
+    if (dt(crs(sub(V[1], V[0]), sub(V[2], V[0])), sub(V[3], V[0])) < 0) std::swap(V[2], V[3]);
+    for (int v = 0; v < 4; v++) append(d, V[v]);
+    Vec n[4];
+    for (int f = 0; f < 4; f++) {
+        const int* fv = kTetFace[f];
+        Vec c = crs(sub(V[fv[1]], V[fv[0]]), sub(V[fv[2]], V[fv[0]]));
+        n[f] = scl(c, 1.0 / nrm(c));
+    }
+    for (int e = 0; e < 6; e++) {
+        const int a = kTetEdgeA[e], b = kTetEdgeB[e];
+        Vec t = sub(V[b], V[a]);
+        double len = nrm(t);
+        t = scl(t, 1.0 / len);
+        double w6[6] = {0, 0, 0, 0, 0, 0};
+        for (int f = 0; f < 4; f++) {
+            const int* fv = kTetFace[f];
+            for (int i = 0; i < 3; i++) {
+                int p = fv[i], q = fv[(i + 1) % 3];
+                if ((p == a && q == b) || (p == b && q == a)) {
+                    Vec tf = (p == a) ? t : scl(t, -1.0);
+                    add_mn(w6, crs(tf, n[f]), n[f]);
+                }
+            }
+        }
+        append(d, t);
+        d.push_back(len);
+        for (int k = 0; k < 6; k++) d.push_back(w6[k]);
+    }
+    for (int f = 0; f < 4; f++) append(d, n[f]);
+}
+
+void pack_gen(const std::vector<Vec>& V, const std::vector<Face>& F, std::vector<double>& d)
+{
+    // unique undirected edges with their W
+    std::map<std::pair<int, int>, int> eid;
+    std::vector<std::pair<int, int>> edges;
+    std::vector<std::array<double, 6>> W;
+    for (const Face& f : F) {
+        const size_t k = f.v.size();
+        for (size_t i = 0; i < k; i++) {
+            int p = f.v[i], q = f.v[(i + 1) % k];
+            auto key = std::make_pair(std::min(p, q), std::max(p, q));
+            auto it = eid.find(key);
+            int id;
+            if (it == eid.end()) {
+                id = int(edges.size());
+                eid[key] = id;
+                edges.push_back(key);
+                W.push_back({0, 0, 0, 0, 0, 0});
+            } else {
+                id = it->second;
+            }
+            Vec tf = sub(V[q], V[p]);
+            tf = scl(tf, 1.0 / nrm(tf));
+            add_mn(W[id].data(), crs(tf, f.n), f.n);
+        }
+    }
+    int nfv = 0;
+    for (const Face& f : F) nfv += int(f.v.size());
+    d.push_back(double(V.size()));
+    d.push_back(double(edges.size()));
+    d.push_back(double(F.size()));
+    d.push_back(double(nfv));
+    for (const Vec& v : V) append(d, v);
+    for (size_t e = 0; e < edges.size(); e++) {
+        int a = edges[e].first, b = edges[e].second;
+        Vec t = sub(V[b], V[a]);
+        double len = nrm(t);
+        t = scl(t, 1.0 / len);
+        d.push_back(double(a));
+        d.push_back(double(b));
+        append(d, t);
+        d.push_back(len);
+        for (int k = 0; k < 6; k++) d.push_back(W[e][k]);
+    }
+    int b0 = 0;
+    for (const Face& f : F) {
+        append(d, f.n);
+        d.push_back(double(b0));
+        d.push_back(double(f.v.size()));
+        b0 += int(f.v.size());
+    }
+    for (const Face& f : F)
+        for (int v : f.v) d.push_back(double(v));
+}
+
+}  // namespace
+
+rh_status pack_model(const rh_model_desc* d, PackedModel* out, std::string* err)
+{
+    if (!d || d->n_elem < 0) {
+        *err = "null model description";
+        return RH_ERR_ARG;
+    }
+    const int64_t n = d->n_elem;
+    PackedModel& P = *out;
+    P = PackedModel();
+    P.n = n;
+    P.kind.resize(n);
+    P.off.resize(n);
+    P.copy_off.assign(d->copy_off, d->copy_off + n + 1);
+    P.colloc.resize(3 * n);
+    P.rowt.resize(9 * n);
+    std::vector<double> cent(3 * n);
+    const int64_t nf_all = n > 0 ? d->face_off[n] : 0;
+    std::vector<int64_t> fstart(nf_all + 1, 0);
+    for (int64_t f = 0; f < nf_all; f++) fstart[f + 1] = fstart[f] + d->face_len[f];
+    for (int64_t e = 0; e < n; e++) {
+        P.off[e] = int64_t(P.data.size());
+        if (d->elem_type[e] == 0) {
+            P.kind[e] = EK_REC;
+            const double* c = d->rec_center + 3 * e;
+            const double* w = d->rec_dims + 3 * e;
+            if (!(w[0] > 0 && w[1] > 0 && w[2] > 0)) {
+                *err = "block with non-positive dimension";
+                return RH_ERR_GEOMETRY;
+            }
+            for (int k = 0; k < 3; k++) P.data.push_back(c[k]);
+            for (int k = 0; k < 3; k++) P.data.push_back(0.5 * w[k]);
+            for (int k = 0; k < 3; k++) cent[3 * e + k] = c[k];
+            P.counts[5]++;
+            continue;
+        }
+        if (d->elem_type[e] != 1) {
+            *err = "unknown element type";
+            return RH_ERR_UNSUPPORTED;
+        }
+        const int64_t v0 = d->vert_off[e], v1 = d->vert_off[e + 1];
+        std::vector<Vec> V;
+        for (int64_t v = v0; v < v1; v++)
+            V.push_back({d->verts[3 * v], d->verts[3 * v + 1], d->verts[3 * v + 2]});
+        // centroid: mean of the vertices in the order given
+        Vec s{0, 0, 0};
+        for (const Vec& p : V) s = {s.x + p.x, s.y + p.y, s.z + p.z};
+        double b = 1.0 / double(V.size());
+        cent[3 * e] = b * s.x;
+        cent[3 * e + 1] = b * s.y;
+        cent[3 * e + 2] = b * s.z;
+        // faces of this element (face_idx is flat over all faces of all elements)
+        std::vector<Face> F;
+        for (int64_t f = d->face_off[e]; f < d->face_off[e + 1]; f++) {
+            Face fc;
+            if (d->face_len[f] < 3) {
+                *err = "face with fewer than 3 vertices";
+                return RH_ERR_GEOMETRY;
+            }
+            for (int k = 0; k < d->face_len[f]; k++) {
+                int vi = d->face_idx[fstart[f] + k];
+                if (vi < 0 || vi >= int(V.size())) {
+                    *err = "face vertex index out of range";
+                    return RH_ERR_ARG;
+                }
+                fc.v.push_back(vi);
+            }
+            F.push_back(fc);
+        }
+        rh_status st = orient_faces(V, F, err);
+        if (st != RH_OK) return st;
+        bool tet = (V.size() == 4 && F.size() == 4);
+        for (const Face& f : F) tet = tet && (f.v.size() == 3);
+        if (tet) {
+            P.kind[e] = EK_TET;
+            pack_tet(V, P.data);
+            P.counts[2]++;
+            P.counts[6] += 6;
+            P.counts[7] += 4;
+        } else {
+            P.kind[e] = EK_GEN;
+            size_t before = P.data.size();
+            pack_gen(V, F, P.data);
+            P.counts[4]++;
+            P.counts[6] += int64_t(P.data[before + 1]);
+            P.counts[7] += int64_t(F.size());
+        }
+    }
+    // copies
+    const int64_t C = d->copy_off[n];
+    P.counts[0] = n;
+    P.counts[1] = C;
+    P.copies.assign(size_t(kCopySize * C), 0.0);
+    for (int64_t c = 0; c < C; c++) {
+        const double* m = d->copy_m + 9 * c;
+        double* o = P.copies.data() + kCopySize * c;
+        inv3(m, o);
+        for (int k = 0; k < 3; k++) o[9 + k] = d->copy_v[3 * c + k];
+        for (int k = 0; k < 9; k++) o[12 + k] = d->copy_s[c] * m[k];
+        bool ident = d->copy_s[c] == 1.0 && d->copy_v[3 * c] == 0 && d->copy_v[3 * c + 1] == 0 &&
+                     d->copy_v[3 * c + 2] == 0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) ident = ident && (m[3 * i + j] == (i == j ? 1.0 : 0.0));
+        if (!ident) P.sym = true;
+    }
+    for (int64_t e = 0; e < n; e++) {
+        if (d->copy_off[e + 1] - d->copy_off[e] != 1) P.sym = true;
+        if (d->copy_off[e + 1] <= d->copy_off[e]) {
+            *err = "element without a transform copy";
+            return RH_ERR_ARG;
+        }
+        const int64_t c = d->copy_off[e];
+        const double* m = d->copy_m + 9 * c;
+        const double* v = d->copy_v + 3 * c;
+        const double* ce = cent.data() + 3 * e;
+        for (int i = 0; i < 3; i++)
+            P.colloc[3 * e + i] = m[3 * i] * ce[0] + m[3 * i + 1] * ce[1] + m[3 * i + 2] * ce[2] + v[i];
+        double mi[9];
+        inv3(m, mi);
+        for (int k = 0; k < 9; k++) {
+            P.rowt[9 * e + k] = d->copy_s[c] * mi[k];
+            if (P.rowt[9 * e + k] != ((k % 4 == 0) ? 1.0 : 0.0)) P.rowt_identity = false;
+        }
+    }
+    return RH_OK;
+}
+
+}  // namespace rh

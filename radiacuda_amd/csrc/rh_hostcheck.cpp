@@ -1,0 +1,89 @@
+// rh_hostcheck.cpp -- host build of the packing and of the exact device math,
+// for CPU unit tests of the formulation (no GPU).  This library is a test
+// aid: the product entry points in libradhip.so never call it.
+#include <string>
+#include <vector>
+
+#include "rh_current.hpp"
+#include "rh_eval.hpp"
+#include "rh_geom.hpp"
+
+namespace {
+thread_local std::string g_err;
+
+rh::ModelView view(const rh::PackedModel& P)
+{
+    return rh::ModelView{P.kind.data(), P.off.data(), P.data.data(), P.copy_off.data(),
+                         P.copies.data()};
+}
+}  // namespace
+
+extern "C" {
+
+const char* rhc_last_error(void) { return g_err.c_str(); }
+
+// Interaction matrix rows [i0, i1) (3 rows each) in FP64 via the host build.
+int rhc_assemble(const rh_model_desc* d, int64_t i0, int64_t i1, int32_t fp32, double* out)
+{
+    rh::PackedModel P;
+    rh_status st = rh::pack_model(d, &P, &g_err);
+    if (st != RH_OK) return st;
+    rh::ModelView mv = view(P);
+    const int64_t n3 = 3 * P.n;
+    for (int64_t i = i0; i < i1; i++) {
+        const double* x = P.colloc.data() + 3 * i;
+        for (int64_t j = 0; j < P.n; j++) {
+            double q[9];
+            if (fp32) rh::block_Q<float, true>(mv, j, x, q);
+            else rh::block_Q<double, true>(mv, j, x, q);
+            const double* rt = P.rowt.data() + 9 * i;
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    out[(3 * (i - i0) + a) * n3 + 3 * j + b] =
+                        rt[3 * a] * q[b] + rt[3 * a + 1] * q[3 + b] + rt[3 * a + 2] * q[6 + b];
+        }
+    }
+    return RH_OK;
+}
+
+int rhc_field(const rh_model_desc* d, const double* M, const double* pts, int64_t np, int32_t fp32,
+              double* B, double* H)
+{
+    rh::PackedModel P;
+    rh_status st = rh::pack_model(d, &P, &g_err);
+    if (st != RH_OK) return st;
+    rh::ModelView mv = view(P);
+    for (int64_t p = 0; p < np; p++) {
+        double b[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+        if (fp32) rh::field_add<float, true>(mv, M, 0, P.n, pts + 3 * p, b, h);
+        else rh::field_add<double, true>(mv, M, 0, P.n, pts + 3 * p, b, h);
+        for (int k = 0; k < 3; k++) {
+            if (B) B[3 * p + k] = b[k];
+            if (H) H[3 * p + k] = h[k];
+        }
+    }
+    return RH_OK;
+}
+
+int rhc_colloc(const rh_model_desc* d, double* X)
+{
+    rh::PackedModel P;
+    rh_status st = rh::pack_model(d, &P, &g_err);
+    if (st != RH_OK) return st;
+    for (size_t k = 0; k < P.colloc.size(); k++) X[k] = P.colloc[k];
+    return RH_OK;
+}
+
+// Current sources: kind[ns], par[12 ns], copies[24 ns] (Minv, V, s*M).
+int rhc_current(int64_t ns, const int32_t* kind, const double* par, const double* cp,
+                const double* pts, int64_t np, double* B)
+{
+    for (int64_t p = 0; p < np; p++) {
+        double b[3] = {0, 0, 0};
+        for (int64_t s = 0; s < ns; s++) rh::current_add(kind[s], par + 12 * s, cp + 24 * s, pts + 3 * p, b);
+        for (int k = 0; k < 3; k++) B[3 * p + k] = b[k];
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,554 @@
+// rh_kernels.hip -- gfx950 kernels: interaction-matrix assembly, field
+// evaluation, the relaxation's matrix pass and per-element update, and
+// current-source fields.
+//
+// Mapping (wave64): in the assembly and field kernels each lane owns one
+// observation point (a matrix row's collocation point, or a field point) and
+// every lane of the workgroup sweeps the same source elements, so element
+// records are read with wave-uniform addresses (scalar-cache loads, no
+// per-lane gathers) and the per-element branches are uniform.  These kernels
+// are bound by FP64 transcendental/VALU throughput.  The matrix pass streams
+// the matrix once per iteration with 16-byte non-temporal loads and FP64
+// accumulation; it is bound by HBM bandwidth.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "rh_current.hpp"
+#include "rh_eval.hpp"
+#include "rh_kernels.hpp"
+
+namespace rh {
+
+namespace {
+
+__host__ __device__ __forceinline__ ModelView view(const DevModel& m)
+{
+    return ModelView{m.kind, m.off, m.data, m.copy_off, m.copies};
+}
+
+template <class T>
+__device__ __forceinline__ void store_out(T* p, double v) { *p = T(v); }
+
+// ---------------------------------------------------------------- assembly
+template <class R, bool SYM, bool ROWT, class OUT>
+__global__ __launch_bounds__(64) void k_assemble(DevModel m, OUT* __restrict__ Q, int64_t ld,
+                                                 double* __restrict__ qdiag, int64_t jt)
+{
+    const int64_t i = int64_t(blockIdx.x) * 64 + threadIdx.x;
+    const int64_t j0 = int64_t(blockIdx.y) * jt;
+    const int64_t j1 = (j0 + jt < m.n) ? (j0 + jt) : m.n;
+    if (i >= m.n) return;
+    const ModelView mv = view(m);
+    const double x[3] = {m.colloc[3 * i], m.colloc[3 * i + 1], m.colloc[3 * i + 2]};
+    double rt[9];
+    if (ROWT)
+        for (int k = 0; k < 9; k++) rt[k] = m.rowt[9 * i + k];
+    OUT* row = Q + (3 * i) * ld;
+    for (int64_t j = j0; j < j1; j++) {
+        double q[9];
+        block_Q<R, SYM>(mv, j, x, q);
+        if (ROWT) {
+            double t[9];
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    t[3 * a + b] = rt[3 * a] * q[b] + rt[3 * a + 1] * q[3 + b] + rt[3 * a + 2] * q[6 + b];
+            for (int k = 0; k < 9; k++) q[k] = t[k];
+        }
+        if (j == i) {
+            for (int k = 0; k < 9; k++) {
+                qdiag[9 * i + k] = q[k];
+                q[k] = 0.0;
+            }
+        }
+        OUT* o = row + 3 * j;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) store_out(o + a * ld + b, q[3 * a + b]);
+    }
+}
+
+template <bool SYM, bool ROWT, class OUT>
+hipError_t asm_dispatch(const DevModel& m, OUT* Q, int64_t ld, double* qdiag, dim3 grid, int64_t jt,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- field
+template <class R, bool SYM>
+__global__ __launch_bounds__(256) void k_field(DevModel m, const double* __restrict__ M,
+                                               const double* __restrict__ pts, int64_t np,
+                                               int64_t js, double* __restrict__ B,
+                                               double* __restrict__ H, double* __restrict__ part)
+{
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= np) return;
+    const int64_t j0 = int64_t(blockIdx.y) * js;
+    const int64_t j1 = (j0 + js < m.n) ? (j0 + js) : m.n;
+    const double x[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+    double b[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+    field_add<R, SYM>(view(m), M, j0, j1, x, b, h);
+    if (part) {
+        double* o = part + (int64_t(blockIdx.y) * np + p) * 6;
+        for (int k = 0; k < 3; k++) {
+            o[k] = b[k];
+            o[3 + k] = h[k];
+        }
+    } else {
+        for (int k = 0; k < 3; k++) {
+            if (B) B[3 * p + k] = b[k];
+            if (H) H[3 * p + k] = h[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_field_reduce(int64_t np, int64_t splits,
+                                                      const double* __restrict__ part,
+                                                      double* __restrict__ B, double* __restrict__ H)
+{
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= np) return;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t s = 0; s < splits; s++) {
+        const double* o = part + (s * np + p) * 6;
+        for (int k = 0; k < 6; k++) acc[k] += o[k];
+    }
+    for (int k = 0; k < 3; k++) {
+        if (B) B[3 * p + k] = acc[k];
+        if (H) H[3 * p + k] = acc[3 + k];
+    }
+}
+
+// ---------------------------------------------------------------- matrix pass
+typedef float nf4 __attribute__((ext_vector_type(4)));
+typedef double nd2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int RW>
+__global__ __launch_bounds__(256) void k_matvec_f32(const float* __restrict__ Q, int64_t nrows,
+                                                    int64_t ld, const double* __restrict__ x,
+                                                    const double* __restrict__ add,
+                                                    double* __restrict__ y)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = (int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * RW;
+    if (r0 >= nrows) return;
+    double acc[RW];
+#pragma unroll
+    for (int r = 0; r < RW; r++) acc[r] = 0.0;
+    const nf4* rows[RW];
+#pragma unroll
+    for (int r = 0; r < RW; r++) {
+        const int64_t rr = (r0 + r < nrows) ? (r0 + r) : r0;
+        rows[r] = reinterpret_cast<const nf4*>(Q + rr * ld);
+    }
+    const int64_t nv = ld / 4;  // float4 per row (ld multiple of 256)
+    const nd2* x2 = reinterpret_cast<const nd2*>(x);
+    for (int64_t c = lane; c < nv; c += 64) {
+        const nd2 xa = x2[2 * c], xb = x2[2 * c + 1];
+#pragma unroll
+        for (int r = 0; r < RW; r++) {
+            const nf4 q = __builtin_nontemporal_load(rows[r] + c);
+            acc[r] += double(q.x) * xa.x + double(q.y) * xa.y + double(q.z) * xb.x + double(q.w) * xb.y;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RW; r++) {
+        const double v = wave_sum(acc[r]);
+        if (lane == 0 && r0 + r < nrows) y[r0 + r] = v + (add ? add[r0 + r] : 0.0);
+    }
+}
+
+template <int RW>
+__global__ __launch_bounds__(256) void k_matvec_f64(const double* __restrict__ Q, int64_t nrows,
+                                                    int64_t ld, const double* __restrict__ x,
+                                                    const double* __restrict__ add,
+                                                    double* __restrict__ y)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = (int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * RW;
+    if (r0 >= nrows) return;
+    double acc[RW];
+#pragma unroll
+    for (int r = 0; r < RW; r++) acc[r] = 0.0;
+    const nd2* rows[RW];
+#pragma unroll
+    for (int r = 0; r < RW; r++) {
+        const int64_t rr = (r0 + r < nrows) ? (r0 + r) : r0;
+        rows[r] = reinterpret_cast<const nd2*>(Q + rr * ld);
+    }
+    const int64_t nv = ld / 2;
+    const nd2* x2 = reinterpret_cast<const nd2*>(x);
+    for (int64_t c = lane; c < nv; c += 64) {
+        const nd2 xa = x2[c];
+#pragma unroll
+        for (int r = 0; r < RW; r++) {
+            const nd2 q = __builtin_nontemporal_load(rows[r] + c);
+            acc[r] += q.x * xa.x + q.y * xa.y;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RW; r++) {
+        const double v = wave_sum(acc[r]);
+        if (lane == 0 && r0 + r < nrows) y[r0 + r] = v + (add ? add[r0 + r] : 0.0);
+    }
+}
+
+// ---------------------------------------------------------------- relaxation update
+__device__ __forceinline__ void inv3d(const double* m, double* o)
+{
+    const double c0 = m[4] * m[8] - m[5] * m[7], c1 = m[2] * m[7] - m[1] * m[8],
+                 c2 = m[1] * m[5] - m[2] * m[4];
+    const double c3 = m[5] * m[6] - m[3] * m[8], c4 = m[0] * m[8] - m[2] * m[6],
+                 c5 = m[2] * m[3] - m[0] * m[5];
+    const double c6 = m[3] * m[7] - m[4] * m[6], c7 = m[1] * m[6] - m[0] * m[7],
+                 c8 = m[0] * m[4] - m[1] * m[3];
+    const double inv = 1.0 / (m[0] * c0 + m[1] * c3 + m[2] * c6);
+    o[0] = c0 * inv; o[1] = c1 * inv; o[2] = c2 * inv;
+    o[3] = c3 * inv; o[4] = c4 * inv; o[5] = c5 * inv;
+    o[6] = c6 * inv; o[7] = c7 * inv; o[8] = c8 * inv;
+}
+
+// H = (I - A)^{-1} b with A = chi_s * Q (iso) or Q * K (aniso).
+__device__ __forceinline__ void solve_IminusA(const double* A, const double* b, double* H)
+{
+    double m[9];
+    for (int k = 0; k < 9; k++) m[k] = -A[k];
+    m[0] += 1.0; m[4] += 1.0; m[8] += 1.0;
+    double inv[9];
+    inv3d(m, inv);
+    mat3_vec(inv, b, H);
+}
+
+// secant susceptibility f(h)/h of the tanh-sum law (radmater.h:644-657), h >= 0
+__device__ __forceinline__ double frm_chi(const double* mat, double h)
+{
+    const int nt = int(mat[1]);
+    double chi = 0.0;
+    for (int t = 0; t < nt; t++) {
+        const double ms = mat[2 + t], ks = mat[5 + t];
+        if (ms == 0.0) continue;
+        if (h > 0.0) {
+            const double u = ks * h / ms;
+            chi += (u < 1e-8) ? ks : ms * tanh(u) / h;
+        } else {
+            chi += ks;
+        }
+    }
+    return chi;
+}
+
+__device__ void local_solve(const double* mat, const double* q, const double* heff, double* Mp,
+                            double* Hl)
+{
+    const int kind = int(mat[0]);
+    const double* mr = mat + 13;
+    double qmr[3];
+    mat3_vec(q, mr, qmr);
+    const double b[3] = {heff[0] + qmr[0], heff[1] + qmr[1], heff[2] + qmr[2]};
+    if (kind == 0) {
+        double A[9];
+        for (int k = 0; k < 9; k++) A[k] = mat[1] * q[k];
+        solve_IminusA(A, b, Hl);
+        for (int k = 0; k < 3; k++) Mp[k] = mat[1] * Hl[k] + mr[k];
+        return;
+    }
+    if (kind == 1) {
+        const double* K = mat + 4;
+        double A[9];
+        for (int a = 0; a < 3; a++)
+            for (int c = 0; c < 3; c++)
+                A[3 * a + c] = q[3 * a] * K[c] + q[3 * a + 1] * K[3 + c] + q[3 * a + 2] * K[6 + c];
+        solve_IminusA(A, b, Hl);
+        double kh[3];
+        mat3_vec(K, Hl, kh);
+        for (int k = 0; k < 3; k++) Mp[k] = kh[k] + mr[k];
+        return;
+    }
+    // nonlinear isotropic: find h = |H(h)|, H(h) = (I - chi_s(h) Q)^{-1} b,
+    // bracketed in [0, |b|] (Illinois false position).
+    const double bn = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    if (!(bn > 0.0)) {
+        Hl[0] = Hl[1] = Hl[2] = 0.0;
+        for (int k = 0; k < 3; k++) Mp[k] = mr[k];
+        return;
+    }
+    auto phi = [&](double h, double* Hv) -> double {
+        const double chi = frm_chi(mat, h);
+        double A[9];
+        for (int k = 0; k < 9; k++) A[k] = chi * q[k];
+        solve_IminusA(A, b, Hv);
+        return sqrt(Hv[0] * Hv[0] + Hv[1] * Hv[1] + Hv[2] * Hv[2]);
+    };
+    double Hv[3];
+    double lo = 0.0, glo = phi(0.0, Hv) - 0.0;
+    double hi = bn, ghi = phi(hi, Hv) - hi;
+    for (int k = 0; k < 60 && ghi > 0.0; k++) {
+        lo = hi;
+        glo = ghi;
+        hi *= 2.0;
+        ghi = phi(hi, Hv) - hi;
+    }
+    double h = hi;
+    int side = 0;
+    for (int it = 0; it < 100; it++) {
+        if (!(glo > 0.0) || !(ghi < 0.0)) {
+            h = (glo <= 0.0) ? lo : hi;
+            break;
+        }
+        h = (lo * ghi - hi * glo) / (ghi - glo);
+        const double g = phi(h, Hv) - h;
+        if (fabs(hi - lo) <= 1e-15 * bn || g == 0.0) break;
+        if (g > 0.0) {
+            lo = h;
+            glo = g;
+            if (side == 1) ghi *= 0.5;
+            side = 1;
+        } else {
+            hi = h;
+            ghi = g;
+            if (side == -1) glo *= 0.5;
+            side = -1;
+        }
+    }
+    phi(h, Hv);
+    const double hn = sqrt(Hv[0] * Hv[0] + Hv[1] * Hv[1] + Hv[2] * Hv[2]);
+    const double chi = frm_chi(mat, hn);
+    for (int k = 0; k < 3; k++) {
+        Hl[k] = Hv[k];
+        Mp[k] = chi * Hv[k] + mr[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_relax_update(int64_t n, const double* __restrict__ mats,
+                                                      const double* __restrict__ hext,
+                                                      const double* __restrict__ qdiag,
+                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ M,
+                                                      double* __restrict__ F,
+                                                      double* __restrict__ Hout,
+                                                      double* __restrict__ partials)
+{
+    __shared__ double red[3][256];
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    double f2 = 0.0, mm = 0.0, hh = 0.0;
+    if (i < n) {
+        const double heff[3] = {y[3 * i] + hext[3 * i], y[3 * i + 1] + hext[3 * i + 1],
+                                y[3 * i + 2] + hext[3 * i + 2]};
+        double mp[3], hl[3];
+        local_solve(mats + 16 * i, qdiag + 9 * i, heff, mp, hl);
+        for (int k = 0; k < 3; k++) {
+            const double d = mp[k] - M[3 * i + k];
+            F[3 * i + k] = d;
+            Hout[3 * i + k] = hl[k];
+            f2 += d * d;
+        }
+        mm = sqrt(mp[0] * mp[0] + mp[1] * mp[1] + mp[2] * mp[2]);
+        hh = sqrt(hl[0] * hl[0] + hl[1] * hl[1] + hl[2] * hl[2]);
+    }
+    red[0][threadIdx.x] = f2;
+    red[1][threadIdx.x] = mm;
+    red[2][threadIdx.x] = hh;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
+            red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x] = red[0][0];
+        partials[3 * blockIdx.x + 1] = red[1][0];
+        partials[3 * blockIdx.x + 2] = red[2][0];
+    }
+}
+
+// ---------------------------------------------------------------- vector helpers
+struct DotPack {
+    const double* a[17];
+    const double* b[17];
+};
+
+__global__ __launch_bounds__(256) void k_dots(int nd, DotPack pk, int64_t len,
+                                              double* __restrict__ partials)
+{
+    __shared__ double red[256];
+    for (int d = 0; d < nd; d++) {
+        double acc = 0.0;
+        for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < len;
+             i += int64_t(gridDim.x) * 256)
+            acc += pk.a[d][i] * pk.b[d][i];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partials[d * gridDim.x + blockIdx.x] = red[0];
+        __syncthreads();
+    }
+}
+
+struct CombPack {
+    const double* v[34];
+    double c[34];
+};
+
+__global__ __launch_bounds__(256) void k_lincomb(int nv, CombPack pk, int64_t len,
+                                                 double* __restrict__ y)
+{
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < len;
+         i += int64_t(gridDim.x) * 256) {
+        double acc = 0.0;
+        for (int k = 0; k < nv; k++) acc += pk.c[k] * pk.v[k][i];
+        y[i] = acc;
+    }
+}
+
+// ---------------------------------------------------------------- currents
+__global__ __launch_bounds__(256) void k_current(int64_t ns, const int32_t* __restrict__ kind,
+                                                 const double* __restrict__ par,
+                                                 const double* __restrict__ cp,
+                                                 const double* __restrict__ pts, int64_t np,
+                                                 double* __restrict__ B)
+{
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= np) return;
+    const double x[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+    double b[3] = {0, 0, 0};
+    for (int64_t s = 0; s < ns; s++) current_add(kind[s], par + 12 * s, cp + 24 * s, x, b);
+    for (int k = 0; k < 3; k++) B[3 * p + k] += b[k];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_assemble(const DevModel& m, void* Q, int elem_bytes, int64_t ld, double* qdiag,
+                           hipStream_t s)
+{
+    if (m.n == 0) return hipSuccess;
+    const int64_t gx = (m.n + 63) / 64;
+    // enough workgroups to fill 256 CUs several times over
+    int64_t gy_target = std::max<int64_t>(1, (8192 + gx - 1) / gx);
+    int64_t jt = std::max<int64_t>(4, (m.n + gy_target - 1) / gy_target);
+    jt = std::min<int64_t>(jt, 256);
+    const int64_t gy = (m.n + jt - 1) / jt;
+    const dim3 grid{unsigned(gx), unsigned(gy), 1u};
+    if (elem_bytes == 4) {
+        float* q = static_cast<float*>(Q);
+        if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, grid, jt, s);
+        if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, grid, jt, s);
+        return asm_dispatch<true, true>(m, q, ld, qdiag, grid, jt, s);
+    }
+    double* q = static_cast<double*>(Q);
+    if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, grid, jt, s);
+    if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, grid, jt, s);
+    return asm_dispatch<true, true>(m, q, ld, qdiag, grid, jt, s);
+}
+
+int64_t field_splits(int64_t np, int64_t n)
+{
+    const int64_t blocks = (np + 255) / 256;
+    if (blocks >= 2048 || n < 64) return 1;
+    int64_t s = (2048 + blocks - 1) / blocks;
+    s = std::min<int64_t>(s, std::max<int64_t>(1, n / 16));
+    return std::max<int64_t>(1, s);
+}
+
+hipError_t launch_field(const DevModel& m, const double* M, const double* pts, int64_t np,
+                        double* B, double* H, double* work, int64_t splits, bool fp32,
+                        hipStream_t s)
+{
+    if (np == 0) return hipSuccess;
+    const int64_t gx = (np + 255) / 256;
+    const int64_t js = (m.n + splits - 1) / splits;
+    const dim3 grid{unsigned(gx), unsigned(splits), 1u};
+    double* part = (splits > 1) ? work : nullptr;
+    if (fp32) {
+        if (m.sym) hipLaunchKernelGGL((k_field<float, true>), grid, dim3(256), 0, s, m, M, pts, np, js, B, H, part);
+        else hipLaunchKernelGGL((k_field<float, false>), grid, dim3(256), 0, s, m, M, pts, np, js, B, H, part);
+    } else {
+        if (m.sym) hipLaunchKernelGGL((k_field<double, true>), grid, dim3(256), 0, s, m, M, pts, np, js, B, H, part);
+        else hipLaunchKernelGGL((k_field<double, false>), grid, dim3(256), 0, s, m, M, pts, np, js, B, H, part);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || splits == 1) return e;
+    hipLaunchKernelGGL(k_field_reduce, dim3(unsigned(gx)), dim3(256), 0, s, np, splits, part, B, H);
+    return hipGetLastError();
+}
+
+hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t ld, const double* x,
+                         const double* add, double* y, hipStream_t s)
+{
+    constexpr int RW = 4;
+    const int64_t waves = (nrows + RW - 1) / RW;
+    const int64_t blocks = (waves + 3) / 4;
+    if (elem_bytes == 4)
+        hipLaunchKernelGGL((k_matvec_f32<RW>), dim3(unsigned(blocks)), dim3(256), 0, s,
+                           static_cast<const float*>(Q), nrows, ld, x, add, y);
+    else
+        hipLaunchKernelGGL((k_matvec_f64<RW>), dim3(unsigned(blocks)), dim3(256), 0, s,
+                           static_cast<const double*>(Q), nrows, ld, x, add, y);
+    return hipGetLastError();
+}
+
+int64_t relax_blocks(int64_t n) { return (n + 255) / 256; }
+
+hipError_t launch_relax_update(int64_t n, const double* mats, const double* hext,
+                               const double* qdiag, const double* y, const double* M, double* F,
+                               double* Hout, double* partials, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_relax_update, dim3(unsigned(relax_blocks(n))), dim3(256), 0, s, n, mats,
+                       hext, qdiag, y, M, F, Hout, partials);
+    return hipGetLastError();
+}
+
+int64_t dot_blocks(int64_t len) { return std::max<int64_t>(1, std::min<int64_t>(256, (len + 4095) / 4096)); }
+
+hipError_t launch_dots(int nd, const double* const* a, const double* const* b, int64_t len,
+                       double* partials, hipStream_t s)
+{
+    if (nd < 1 || nd > 17) return hipErrorInvalidValue;
+    DotPack pk;
+    for (int d = 0; d < nd; d++) {
+        pk.a[d] = a[d];
+        pk.b[d] = b[d];
+    }
+    hipLaunchKernelGGL(k_dots, dim3(unsigned(dot_blocks(len))), dim3(256), 0, s, nd, pk, len, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_lincomb(int nv, const double* const* v, const double* c, int64_t len, double* y,
+                          hipStream_t s)
+{
+    if (nv < 1 || nv > 34) return hipErrorInvalidValue;
+    CombPack pk;
+    for (int k = 0; k < nv; k++) {
+        pk.v[k] = v[k];
+        pk.c[k] = c[k];
+    }
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(4096, (len + 255) / 256));
+    hipLaunchKernelGGL(k_lincomb, dim3(unsigned(blocks)), dim3(256), 0, s, nv, pk, len, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_current(int64_t ns, const int32_t* kind, const double* par, const double* cp,
+                          const double* pts, int64_t np, double* B, hipStream_t s)
+{
+    if (np == 0 || ns == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_current, dim3(unsigned((np + 255) / 256)), dim3(256), 0, s, ns, kind, par,
+                       cp, pts, np, B);
+    return hipGetLastError();
+}
+
+}  // namespace rh

@@ -1,0 +1,62 @@
+// rh_kernels.hpp -- launch interface of the gfx950 kernels (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rh {
+
+struct DevModel {
+    int64_t n = 0;
+    const int32_t* kind = nullptr;
+    const int64_t* off = nullptr;
+    const double* data = nullptr;
+    const int64_t* copy_off = nullptr;
+    const double* copies = nullptr;
+    const double* colloc = nullptr;  // [3n] lab
+    const double* rowt = nullptr;    // [9n]
+    bool sym = false;
+    bool rowt_identity = true;
+};
+
+// Interaction matrix columns [col0, col1) of all rows into Q (row-major, leading
+// dimension ld elements of elem_bytes = 4 or 8); the 3x3 diagonal blocks go to
+// qdiag [9n] in FP64 and are stored as zeros in Q.
+hipError_t launch_assemble(const DevModel& m, void* Q, int elem_bytes, int64_t ld, double* qdiag,
+                           hipStream_t s);
+
+// Number of element splits the field kernel uses for np points (>= 1).
+int64_t field_splits(int64_t np, int64_t n);
+// B, H (either may be null) at np points; work holds 6*np*splits doubles when splits > 1.
+hipError_t launch_field(const DevModel& m, const double* M, const double* pts, int64_t np,
+                        double* B, double* H, double* work, int64_t splits, bool fp32,
+                        hipStream_t s);
+
+// y = Q x (+ add), Q: nrows x ld (ld multiple of 256, padded columns zero,
+// x padded with zeros to ld).
+hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t ld, const double* x,
+                         const double* add, double* y, hipStream_t s);
+
+// Per-element implicit update: proposal M_prop = M_mat(H) with
+// H = y + hext + Qii M_prop solved locally; F = M_prop - M.  partials gets
+// per-block [sum |F|^2, max |M_prop|, max |H|].
+int64_t relax_blocks(int64_t n);
+hipError_t launch_relax_update(int64_t n, const double* mats, const double* hext,
+                               const double* qdiag, const double* y, const double* M, double* F,
+                               double* Hout, double* partials, hipStream_t s);
+
+// Batched dot products: out[k*nb + b] = partial of sum_i a_k[i]*b_k[i] over block b.
+int64_t dot_blocks(int64_t len);
+hipError_t launch_dots(int nd, const double* const* a, const double* const* b, int64_t len,
+                       double* partials, hipStream_t s);
+
+// y = sum_k c_k v_k (k < nv <= 34); y may alias v_0.
+hipError_t launch_lincomb(int nv, const double* const* v, const double* c, int64_t len, double* y,
+                          hipStream_t s);
+
+// B += field of current sources (kind [ns], par [12 ns], copies [24 ns]) at np points.
+hipError_t launch_current(int64_t ns, const int32_t* kind, const double* par, const double* cp,
+                          const double* pts, int64_t np, double* B, hipStream_t s);
+
+}  // namespace rh

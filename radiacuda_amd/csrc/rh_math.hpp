@@ -1,0 +1,359 @@
+// rh_math.hpp -- field integrals of uniformly magnetized elements.
+//
+// Shared by the gfx950 kernels and by the host build (RH_HD = host+device).
+// Everything here is per (observation point, element) and branch-uniform
+// across a wavefront, because every lane of a wave sweeps the same element
+// for its own point.
+//
+// Formulation (own design; the reference evaluates each polygon face in its
+// own 2D frame edge by edge, cpp/src/core/radplnr2.cpp:29-386).  For a convex
+// polyhedron with outward unit face normals n_f, surface charge
+// sigma_f = n_f . M, the field per unit magnetization is the symmetric block
+//
+//     Q(p) = 1/(4 pi) * [ sum_edges L_e W_e  -  sum_faces Omega_f n_f n_f^T ]
+//
+// with  L_e     = integral over the edge of dl/|x - p|
+//                = ln((R_b + s_b)/(R_a + s_a)), s = (v - p).t_e,
+//       W_e     = sum over the two faces meeting at e of m_{e,f} n_f^T,
+//                 m_{e,f} = in-plane outward normal of face f at edge e,
+//       Omega_f = signed solid angle integral of (x-p).n_f/|x-p|^3 over f,
+//                 from the Van Oosterom-Strackee triangle formula with the
+//                 fan of each face multiplied as complex numbers so that a
+//                 face costs one atan2.
+// W_e is stored symmetrized: the antisymmetric parts cancel in the sum since
+// Q is a Hessian.  Each edge's log is shared by both of its faces, so a
+// tetrahedron costs 4 sqrt, 6 log, 6 div and 4 atan2 per point.
+//
+// The rectangular block uses the same decomposition specialised to an
+// axis-aligned box: 3 logs (one per off-diagonal pair, as products of edge
+// ratios), 6 atan2 (one per face), 8 sqrt -- the quantities of the closed form
+// in cpp/src/core/radrec.cpp:49-315.
+//
+// H = Q M, and B = H + M when the point is strictly inside the element
+// (cpp/src/core/radvlpgn.cpp:497-503, radrec.cpp:307-312).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#define RH_HD __host__ __device__ __forceinline__
+
+namespace rh {
+
+constexpr double kInv4Pi = 0.079577471545947667884441881686257181;  // 1/(4 pi)
+
+// Element record kinds in the packed element data.
+enum ElemKind : int32_t { EK_REC = 0, EK_TET = 1, EK_GEN = 3 };
+
+// Record sizes (doubles).
+constexpr int kRecSize = 6;             // centre(3), half-dims(3)
+constexpr int kTetEdge = 10;            // t(3), len, W sym (xx yy zz xy xz yz)
+constexpr int kTetSize = 12 + 6 * kTetEdge + 12;  // verts, edges, face normals = 84
+constexpr int kGenHdr = 4;              // nv, ne, nf, nfv
+constexpr int kGenEdge = 12;            // a, b, t(3), len, W sym(6)
+constexpr int kGenFace = 5;             // n(3), fv begin, fv count
+
+// Canonical tetrahedron topology: vertices 0..3 positively oriented
+// (det(v1-v0, v2-v0, v3-v0) > 0); faces counter-clockwise about the outward
+// normal; edge k joins kTetEdgeA[k] -> kTetEdgeB[k].
+constexpr int kTetEdgeA[6] = {0, 0, 0, 1, 1, 2};
+constexpr int kTetEdgeB[6] = {1, 2, 3, 2, 3, 3};
+constexpr int kTetFace[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+
+template <class R>
+struct V3 {
+    R x, y, z;
+};
+template <class R>
+RH_HD V3<R> v3(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <class R>
+RH_HD R dot(const V3<R>& a, const V3<R>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R>
+RH_HD V3<R> cross(const V3<R>& a, const V3<R>& b)
+{
+    return V3<R>{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+RH_HD double rh_log(double x) { return log(x); }
+RH_HD float rh_log(float x) { return logf(x); }
+RH_HD double rh_sqrt(double x) { return sqrt(x); }
+RH_HD float rh_sqrt(float x) { return sqrtf(x); }
+RH_HD double rh_atan2(double y, double x) { return atan2(y, x); }
+RH_HD float rh_atan2(float y, float x) { return atan2f(y, x); }
+
+// Vertex relative to the observation point, formed in FP64 then rounded to R.
+template <class R>
+RH_HD V3<R> rel(const double* v, const double p[3])
+{
+    return V3<R>{R(v[0] - p[0]), R(v[1] - p[1]), R(v[2] - p[2])};
+}
+
+// ln of the edge integral of 1/|x-p| as (numerator, denominator) without
+// cancellation: whichever end has the larger projection along the edge gives
+// R + s directly; the other end uses R + s = d^2/(R - s) when s < 0.
+template <class R>
+RH_HD void edge_ratio(const V3<R>& ra, R Ra, const V3<R>& rb, R Rb, const V3<R>& t, R& num,
+                      R& den)
+{
+    R sa = dot(ra, t), sb = dot(rb, t);
+    bool pos = (sa + sb) >= R(0);
+    R Rn = pos ? Rb : Ra, sn = pos ? sb : -sa;
+    R Rm = pos ? Ra : Rb, sm = pos ? sa : -sb;
+    V3<R> c = cross(ra, t);
+    R d2 = dot(c, c);
+    bool good = sm >= R(0);
+    num = (Rn + sn) * (good ? R(1) : (Rm - sm));
+    den = good ? (Rm + sm) : d2;
+}
+
+// Solid-angle factor of triangle (r1, r2, r3) as a complex number whose
+// argument is half the signed solid angle (Van Oosterom & Strackee 1983).
+template <class R>
+RH_HD void tri_factor(const V3<R>& r1, R R1, const V3<R>& r2, R R2, const V3<R>& r3, R R3, R& re,
+                      R& im)
+{
+    im = dot(r1, cross(r2, r3));
+    re = R1 * R2 * R3 + dot(r1, r2) * R3 + dot(r1, r3) * R2 + dot(r2, r3) * R1;
+}
+
+// q6 layout: xx, yy, zz, xy, xz, yz.
+RH_HD void q6_to_q9(const double q6[6], double q9[9])
+{
+    q9[0] = q6[0]; q9[1] = q6[3]; q9[2] = q6[4];
+    q9[3] = q6[3]; q9[4] = q6[1]; q9[5] = q6[5];
+    q9[6] = q6[4]; q9[7] = q6[5]; q9[8] = q6[2];
+}
+
+// ---------------------------------------------------------------- block
+// rec: centre c, half-dims h.  Corner (i,j,k) in {0,1}^3 sits at
+// c + ((2i-1)hx, (2j-1)hy, (2k-1)hz).
+template <class R>
+RH_HD bool eval_rec(const double* __restrict__ rec, const double p[3], double q6[6])
+{
+    const double cx = rec[0] - p[0], cy = rec[1] - p[1], cz = rec[2] - p[2];
+    const R X[2] = {R(cx - rec[3]), R(cx + rec[3])};
+    const R Y[2] = {R(cy - rec[4]), R(cy + rec[4])};
+    const R Z[2] = {R(cz - rec[5]), R(cz + rec[5])};
+    R Rc[2][2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+                Rc[i][j][k] = rh_sqrt(X[i] * X[i] + Y[j] * Y[j] + Z[k] * Z[k]);
+
+    // Off-diagonals: Q_yz from the four x-edges, Q_xz from y-edges, Q_xy from z-edges.
+    // edge (j,k) along x contributes s_j s_k ln(ratio), s_0 = -1, s_1 = +1.
+    double lx, ly, lz;
+    {
+        R nP = R(1), dP = R(1);
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                R num, den;
+                edge_ratio(v3(X[0], Y[j], Z[k]), Rc[0][j][k], v3(X[1], Y[j], Z[k]), Rc[1][j][k],
+                           v3(R(1), R(0), R(0)), num, den);
+                if (j == k) { nP *= num; dP *= den; } else { nP *= den; dP *= num; }
+            }
+        lx = double(rh_log(nP / dP));
+    }
+    {
+        R nP = R(1), dP = R(1);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                R num, den;
+                edge_ratio(v3(X[i], Y[0], Z[k]), Rc[i][0][k], v3(X[i], Y[1], Z[k]), Rc[i][1][k],
+                           v3(R(0), R(1), R(0)), num, den);
+                if (i == k) { nP *= num; dP *= den; } else { nP *= den; dP *= num; }
+            }
+        ly = double(rh_log(nP / dP));
+    }
+    {
+        R nP = R(1), dP = R(1);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                R num, den;
+                edge_ratio(v3(X[i], Y[j], Z[0]), Rc[i][j][0], v3(X[i], Y[j], Z[1]), Rc[i][j][1],
+                           v3(R(0), R(0), R(1)), num, den);
+                if (i == j) { nP *= num; dP *= den; } else { nP *= den; dP *= num; }
+            }
+        lz = double(rh_log(nP / dP));
+    }
+    // Diagonals: Q_aa = -(Omega(-a face) + Omega(+a face)); faces as two triangles.
+    double om[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double s = 0.;
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            // corners of the face counter-clockwise about its outward normal
+            int c[4][3];
+            if (a == 0) {
+                int i = side;
+                int cc[4][3] = {{i, 0, 0}, {i, 0, 1}, {i, 1, 1}, {i, 1, 0}};
+                int cp[4][3] = {{i, 0, 0}, {i, 1, 0}, {i, 1, 1}, {i, 0, 1}};
+                for (int u = 0; u < 4; u++)
+                    for (int w = 0; w < 3; w++) c[u][w] = side ? cp[u][w] : cc[u][w];
+            } else if (a == 1) {
+                int j = side;
+                int cc[4][3] = {{0, j, 0}, {1, j, 0}, {1, j, 1}, {0, j, 1}};
+                int cp[4][3] = {{0, j, 0}, {0, j, 1}, {1, j, 1}, {1, j, 0}};
+                for (int u = 0; u < 4; u++)
+                    for (int w = 0; w < 3; w++) c[u][w] = side ? cp[u][w] : cc[u][w];
+            } else {
+                int k = side;
+                int cc[4][3] = {{0, 0, k}, {0, 1, k}, {1, 1, k}, {1, 0, k}};
+                int cp[4][3] = {{0, 0, k}, {1, 0, k}, {1, 1, k}, {0, 1, k}};
+                for (int u = 0; u < 4; u++)
+                    for (int w = 0; w < 3; w++) c[u][w] = side ? cp[u][w] : cc[u][w];
+            }
+            V3<R> r[4];
+            R rr[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                r[u] = v3(X[c[u][0]], Y[c[u][1]], Z[c[u][2]]);
+                rr[u] = Rc[c[u][0]][c[u][1]][c[u][2]];
+            }
+            R re1, im1, re2, im2;
+            tri_factor(r[0], rr[0], r[1], rr[1], r[2], rr[2], re1, im1);
+            tri_factor(r[0], rr[0], r[2], rr[2], r[3], rr[3], re2, im2);
+            R re = re1 * re2 - im1 * im2, im = re1 * im2 + im1 * re2;
+            s += 2.0 * double(rh_atan2(im, re));
+        }
+        om[a] = s;
+    }
+    q6[0] = -kInv4Pi * om[0];
+    q6[1] = -kInv4Pi * om[1];
+    q6[2] = -kInv4Pi * om[2];
+    q6[3] = kInv4Pi * lz;  // xy from z-edges
+    q6[4] = kInv4Pi * ly;  // xz from y-edges
+    q6[5] = kInv4Pi * lx;  // yz from x-edges
+    return (X[0] * X[1] < R(0)) && (Y[0] * Y[1] < R(0)) && (Z[0] * Z[1] < R(0));
+}
+
+// ---------------------------------------------------------------- tetrahedron
+template <class R>
+RH_HD bool eval_tet(const double* __restrict__ rec, const double p[3], double q6[6])
+{
+    V3<R> r[4];
+    R Rv[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        r[v] = rel<R>(rec + 3 * v, p);
+        Rv[v] = rh_sqrt(dot(r[v], r[v]));
+    }
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const double* ed = rec + 12;
+#pragma unroll
+    for (int e = 0; e < 6; e++) {
+        const double* E = ed + kTetEdge * e;
+        const int a = kTetEdgeA[e], b = kTetEdgeB[e];
+        V3<R> t = v3(R(E[0]), R(E[1]), R(E[2]));
+        R num, den;
+        edge_ratio(r[a], Rv[a], r[b], Rv[b], t, num, den);
+        double L = double(rh_log(num / den));
+#pragma unroll
+        for (int k = 0; k < 6; k++) acc[k] += L * E[4 + k];
+    }
+    const double* fn = rec + 12 + 6 * kTetEdge;
+    bool inside = true;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        const int a = kTetFace[f][0], b = kTetFace[f][1], c = kTetFace[f][2];
+        R re, im;
+        tri_factor(r[a], Rv[a], r[b], Rv[b], r[c], Rv[c], re, im);
+        double om = 2.0 * double(rh_atan2(im, re));
+        const double nx = fn[3 * f], ny = fn[3 * f + 1], nz = fn[3 * f + 2];
+        acc[0] -= om * nx * nx;
+        acc[1] -= om * ny * ny;
+        acc[2] -= om * nz * nz;
+        acc[3] -= om * nx * ny;
+        acc[4] -= om * nx * nz;
+        acc[5] -= om * ny * nz;
+        // strictly on the inner side of every face plane
+        double h = (rec[3 * a] - p[0]) * nx + (rec[3 * a + 1] - p[1]) * ny + (rec[3 * a + 2] - p[2]) * nz;
+        inside = inside && (h > 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) q6[k] = kInv4Pi * acc[k];
+    return inside;
+}
+
+// ---------------------------------------------------------------- general convex polyhedron
+// rec: nv, ne, nf, nfv | verts[3nv] | edges[12 ne] | faces[5 nf] | fv[nfv]
+template <class R>
+RH_HD bool eval_gen(const double* __restrict__ rec, const double p[3], double q6[6])
+{
+    const int nv = int(rec[0]), ne = int(rec[1]), nf = int(rec[2]);
+    const double* vt = rec + kGenHdr;
+    const double* ed = vt + 3 * nv;
+    const double* fc = ed + kGenEdge * ne;
+    const double* fv = fc + kGenFace * nf;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int e = 0; e < ne; e++) {
+        const double* E = ed + kGenEdge * e;
+        const int a = int(E[0]), b = int(E[1]);
+        V3<R> ra = rel<R>(vt + 3 * a, p), rb = rel<R>(vt + 3 * b, p);
+        R Ra = rh_sqrt(dot(ra, ra)), Rb = rh_sqrt(dot(rb, rb));
+        R num, den;
+        edge_ratio(ra, Ra, rb, Rb, v3(R(E[2]), R(E[3]), R(E[4])), num, den);
+        double L = double(rh_log(num / den));
+        for (int k = 0; k < 6; k++) acc[k] += L * E[6 + k];
+    }
+    bool inside = true;
+    for (int f = 0; f < nf; f++) {
+        const double* F = fc + kGenFace * f;
+        const int b0 = int(F[3]), cnt = int(F[4]);
+        const int i0 = int(fv[b0]);
+        V3<R> r0 = rel<R>(vt + 3 * i0, p);
+        R R0 = rh_sqrt(dot(r0, r0));
+        V3<R> r1 = rel<R>(vt + 3 * int(fv[b0 + 1]), p);
+        R R1 = rh_sqrt(dot(r1, r1));
+        R zr = R(1), zi = R(0);
+        for (int k = 2; k < cnt; k++) {
+            V3<R> r2 = rel<R>(vt + 3 * int(fv[b0 + k]), p);
+            R R2 = rh_sqrt(dot(r2, r2));
+            R re, im;
+            tri_factor(r0, R0, r1, R1, r2, R2, re, im);
+            R s = R(1) / (R0 * R1 * R2);  // keep the running product O(1)
+            re *= s;
+            im *= s;
+            R nr = zr * re - zi * im, ni = zr * im + zi * re;
+            zr = nr;
+            zi = ni;
+            r1 = r2;
+            R1 = R2;
+        }
+        double om = 2.0 * double(rh_atan2(zi, zr));
+        const double nx = F[0], ny = F[1], nz = F[2];
+        acc[0] -= om * nx * nx;
+        acc[1] -= om * ny * ny;
+        acc[2] -= om * nz * nz;
+        acc[3] -= om * nx * ny;
+        acc[4] -= om * nx * nz;
+        acc[5] -= om * ny * nz;
+        const double* v0 = vt + 3 * i0;
+        double h = (v0[0] - p[0]) * nx + (v0[1] - p[1]) * ny + (v0[2] - p[2]) * nz;
+        inside = inside && (h > 0.0);
+    }
+    for (int k = 0; k < 6; k++) q6[k] = kInv4Pi * acc[k];
+    return inside;
+}
+
+template <class R>
+RH_HD bool eval_elem(int kind, const double* __restrict__ rec, const double p[3], double q6[6])
+{
+    if (kind == EK_TET) return eval_tet<R>(rec, p, q6);
+    if (kind == EK_REC) return eval_rec<R>(rec, p, q6);
+    return eval_gen<R>(rec, p, q6);
+}
+
+}  // namespace rh

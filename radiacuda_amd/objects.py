@@ -1,0 +1,313 @@
+"""Host object model behind the ``rad`` drop-in API.
+
+Field sources (blocks, polyhedra, arc and block currents), containers, space
+transforms with multiplicities, and materials; plus the flattening that turns
+an object tree into the flat arrays the C ABI takes.
+
+The flattening follows Radia's own conventions:
+  * a transform list is kept per object with the most recently applied one
+    first (radg3d.h:871-875); a container's transforms are outside its
+    members' (radintrc.h:295-300, radintrc.cpp:526-553);
+  * a transform of multiplicity m > 1 contributes its powers 0..m-1, one of
+    multiplicity 1 is always applied (radg3d.cpp:57-115);
+  * copy 0 of an element (all symmetry powers zero) is its main transform,
+    which defines the collocation point and the row transform of the
+    interaction matrix (radintrc.cpp:557-573, 852);
+  * elements with a material are relaxed, every other source is external
+    (radintrc.cpp:248-270).
+"""
+import itertools
+import math
+
+import numpy as np
+
+# ---------------------------------------------------------------- transforms
+
+
+class Trf:
+    """Affine map x -> M x + V acting on field vectors as h -> s M h (gmtrans.h:30-85)."""
+
+    __slots__ = ("M", "V", "s")
+
+    def __init__(self, M=None, V=None, s=1.0):
+        self.M = np.eye(3) if M is None else np.asarray(M, dtype=np.float64).reshape(3, 3)
+        self.V = np.zeros(3) if V is None else np.asarray(V, dtype=np.float64).reshape(3)
+        self.s = float(s)
+
+    def compose(self, inner):
+        """self o inner (TrProduct in gmtrans.h)."""
+        return Trf(self.M @ inner.M, self.M @ inner.V + self.V, self.s * inner.s)
+
+    def power(self, k):
+        t = Trf()
+        for _ in range(k):
+            t = t.compose(self)
+        return t
+
+    @staticmethod
+    def translation(v):
+        return Trf(V=v)
+
+    @staticmethod
+    def rotation(p, axis, phi):
+        """Right-handed rotation by phi about the axis through p (gmtrans.cpp:16-46)."""
+        a = np.asarray(axis, dtype=np.float64)
+        a = a / np.linalg.norm(a)
+        c, s = math.cos(phi), math.sin(phi)
+        K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+        R = c * np.eye(3) + s * K + (1 - c) * np.outer(a, a)
+        p = np.asarray(p, dtype=np.float64)
+        return Trf(R, (np.eye(3) - R) @ p)
+
+    @staticmethod
+    def plane_symmetry(p, n):
+        n = np.asarray(n, dtype=np.float64)
+        n = n / np.linalg.norm(n)
+        M = np.eye(3) - 2.0 * np.outer(n, n)
+        return Trf(M, 2.0 * np.dot(np.asarray(p, dtype=np.float64), n) * n)
+
+    @staticmethod
+    def field_inversion():
+        return Trf(s=-1.0)
+
+
+# ---------------------------------------------------------------- objects
+
+
+class Obj:
+    kind = "obj"
+
+    def __init__(self):
+        self.trfs = []  # [(Trf, mult)], most recently applied first
+
+    def add_trf(self, trf, mult):
+        self.trfs.insert(0, (trf, int(mult)))
+
+
+class RecMag(Obj):
+    """Rectangular block with uniform magnetization or current density."""
+
+    kind = "recmag"
+
+    def __init__(self, center, dims, M=(0, 0, 0), J=(0, 0, 0)):
+        super().__init__()
+        self.center = np.asarray(center, dtype=np.float64).reshape(3)
+        self.dims = np.asarray(dims, dtype=np.float64).reshape(3)
+        self.M = np.asarray(M, dtype=np.float64).reshape(3).copy()
+        self.J = np.asarray(J, dtype=np.float64).reshape(3).copy()
+        self.mat = None
+
+    @property
+    def is_current(self):
+        return bool(np.any(self.J != 0))
+
+
+class Polyhedron(Obj):
+    kind = "poly"
+
+    def __init__(self, verts, faces, M=(0, 0, 0)):
+        super().__init__()
+        self.verts = np.asarray(verts, dtype=np.float64).reshape(-1, 3)
+        self.faces = [list(map(int, f)) for f in faces]  # 0-based
+        self.M = np.asarray(M, dtype=np.float64).reshape(3).copy()
+        self.mat = None
+        if len(self.faces) < 4 or len(self.verts) < 4:
+            raise ValueError("a polyhedron needs at least 4 vertices and 4 faces")
+        for f in self.faces:
+            if len(f) < 3 or min(f) < 0 or max(f) >= len(self.verts):
+                raise ValueError("face vertex index out of range")
+
+
+class ArcCur(Obj):
+    """Arc of rectangular section carrying azimuthal current density (radarccu.h)."""
+
+    kind = "arccur"
+
+    def __init__(self, center, rmin, rmax, phimin, phimax, height, nseg, j):
+        super().__init__()
+        self.center = np.asarray(center, dtype=np.float64).reshape(3)
+        self.rmin, self.rmax = float(rmin), float(rmax)
+        self.phimin, self.phimax = float(phimin), float(phimax)
+        self.height, self.nseg, self.j = float(height), int(nseg), float(j)
+
+
+class Background(Obj):
+    kind = "bckg"
+
+    def __init__(self, B):
+        super().__init__()
+        self.B = np.asarray(B, dtype=np.float64).reshape(3)
+
+
+class Container(Obj):
+    kind = "cnt"
+
+    def __init__(self, members):
+        super().__init__()
+        self.members = list(members)
+
+
+# ---------------------------------------------------------------- materials
+
+MAT_LIN_ISO, MAT_LIN_ANISO, MAT_NONLIN_FRM, MAT_NONLIN_TAB = 0, 1, 2, 3
+
+
+class Material:
+    """A material law (radmater.h).  ``finish`` binds an anisotropic linear
+    material's easy axis to an element's magnetization (radmater.h:235-250)."""
+
+    def __init__(self, kind, **kw):
+        self.kind = kind
+        self.p = kw
+
+    def record(self, elem_M):
+        """Per-element material record (rh_material layout, 16 doubles)."""
+        r = np.zeros(16)
+        r[0] = self.kind
+        if self.kind == MAT_LIN_ISO:
+            r[1] = self.p["ksi"]
+            r[13:16] = self.p.get("mr", (0, 0, 0))
+        elif self.kind == MAT_LIN_ANISO:
+            kpar, kper = self.p["kpar"], self.p["kper"]
+            mr = self.p.get("mr_vec")
+            if mr is None:
+                m = np.asarray(elem_M, dtype=np.float64)
+                am = float(np.linalg.norm(m))
+                if am < 1e-10:
+                    raise RuntimeError("Radia::Error107: zero magnetization, cannot define "
+                                       "the easy axis of a linear anisotropic material")
+                mr = (self.p["mr"] / am) * m
+            mr = np.asarray(mr, dtype=np.float64)
+            amr = float(np.linalg.norm(mr))
+            L = mr / amr if amr > 0 else np.array([0.0, 0.0, 1.0])
+            K = kper * np.eye(3) + (kpar - kper) * np.outer(L, L)
+            r[4:13] = K.ravel()
+            r[13:16] = mr
+        elif self.kind == MAT_NONLIN_FRM:
+            ks, ms = self.p["ks"], self.p["ms"]
+            r[1] = len(ks)
+            r[2:2 + len(ms)] = ms
+            r[5:5 + len(ks)] = ks
+        else:
+            raise NotImplementedError("tabulated M(H) materials are not supported yet")
+        return r
+
+
+# ---------------------------------------------------------------- flattening
+
+
+def _expand(trf_lists):
+    """All copies of an element given its transform lists outer->inner.
+
+    Returns a list of Trf; copy 0 has every symmetry power at zero."""
+    factors = []
+    for lst in trf_lists:
+        for trf, m in lst:
+            if m == 1:
+                factors.append([trf])
+            else:
+                factors.append([trf.power(k) for k in range(m)])
+    copies = []
+    for combo in itertools.product(*factors) if factors else [()]:
+        t = Trf()
+        for f in combo:
+            t = t.compose(f)
+        copies.append(t)
+    return copies
+
+
+def walk(registry, key, trf_stack=()):
+    """Yield (leaf_obj, leaf_key, transform lists outer->inner) for every leaf."""
+    o = registry[key]
+    stack = trf_stack + (o.trfs,) if o.trfs else trf_stack
+    if isinstance(o, Container):
+        for k in o.members:
+            yield from walk(registry, k, stack)
+    else:
+        yield o, key, stack
+
+
+class Flat:
+    """Flat arrays of a model: relaxable elements and external sources."""
+
+
+def flatten(registry, key, relax_only=False):
+    """Split the tree under ``key`` into relaxable elements (with a material)
+    and external sources, and build the flat element arrays."""
+    rel, ext = [], []
+    for o, k, stack in walk(registry, key):
+        if isinstance(o, (RecMag, Polyhedron)) and not (isinstance(o, RecMag) and o.is_current):
+            if o.mat is not None or not relax_only:
+                (rel if o.mat is not None else ext).append((o, k, stack))
+            else:
+                ext.append((o, k, stack))
+        else:
+            ext.append((o, k, stack))
+    return rel, ext
+
+
+def pack_elements(items):
+    """Flat element arrays (rh_model_desc layout) for magnetizable elements."""
+    n = len(items)
+    typ = np.zeros(n, np.int32)
+    rec_c = np.zeros((n, 3))
+    rec_d = np.zeros((n, 3))
+    vert_off = np.zeros(n + 1, np.int64)
+    face_off = np.zeros(n + 1, np.int64)
+    copy_off = np.zeros(n + 1, np.int64)
+    verts, face_len, face_idx = [], [], []
+    cM, cV, cS = [], [], []
+    M = np.zeros((n, 3))
+    for e, (o, _k, stack) in enumerate(items):
+        M[e] = o.M
+        if isinstance(o, RecMag):
+            typ[e] = 0
+            rec_c[e] = o.center
+            rec_d[e] = o.dims
+            vert_off[e + 1] = vert_off[e]
+            face_off[e + 1] = face_off[e]
+        else:
+            typ[e] = 1
+            verts.append(o.verts)
+            vert_off[e + 1] = vert_off[e] + len(o.verts)
+            face_off[e + 1] = face_off[e] + len(o.faces)
+            for f in o.faces:
+                face_len.append(len(f))
+                face_idx.extend(f)
+        cps = _expand(stack)
+        copy_off[e + 1] = copy_off[e] + len(cps)
+        for t in cps:
+            cM.append(t.M)
+            cV.append(t.V)
+            cS.append(t.s)
+    return dict(
+        type=typ, rec_c=rec_c, rec_d=rec_d, vert_off=vert_off,
+        verts=np.concatenate(verts) if verts else np.zeros((0, 3)),
+        face_off=face_off, face_len=np.asarray(face_len, np.int32),
+        face_idx=np.asarray(face_idx, np.int32), copy_off=copy_off,
+        cM=np.asarray(cM, dtype=np.float64).reshape(-1, 3, 3),
+        cV=np.asarray(cV, dtype=np.float64).reshape(-1, 3),
+        cS=np.asarray(cS, dtype=np.float64), M=M)
+
+
+def pack_currents(items):
+    """Flat arrays for current sources, each expanded over its copies.
+
+    Kind 0: block current (centre, dims, J); kind 1: arc current.
+    Every record is in its own frame with one copy transform."""
+    recs = []
+    for o, _k, stack in items:
+        for t in _expand(stack):
+            if isinstance(o, RecMag) and o.is_current:
+                p = np.concatenate([o.center, o.dims, o.J, np.zeros(3)])
+                recs.append((0, p, t))
+            elif isinstance(o, ArcCur):
+                p = np.array([o.center[0], o.center[1], o.center[2], o.rmin, o.rmax, o.phimin,
+                              o.phimax, o.height, o.j, float(o.nseg), 0, 0])
+                recs.append((1, p, t))
+    kind = np.array([r[0] for r in recs], np.int32)
+    par = np.array([r[1] for r in recs], dtype=np.float64).reshape(-1, 12)
+    tM = np.array([r[2].M for r in recs], dtype=np.float64).reshape(-1, 3, 3)
+    tV = np.array([r[2].V for r in recs], dtype=np.float64).reshape(-1, 3)
+    tS = np.array([r[2].s for r in recs], dtype=np.float64)
+    return dict(kind=kind, par=par, cM=tM, cV=tV, cS=tS)

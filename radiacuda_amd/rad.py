@@ -1,0 +1,554 @@
+"""Drop-in subset of Radia's Python API (``import radiacuda_amd.rad as rad``).
+
+Function names, argument order, defaults and return shapes follow the
+reference's Python module (cpp/src/clients/python/radpy.cpp, method table at
+:3341-3434).  Geometry, materials and transforms live in the host object model
+(objects.py); assembly, relaxation and field evaluation run in the HIP
+library through its C ABI (include/radhip.h) -- there is no CPU fallback.
+"""
+import math
+
+import numpy as np
+
+from . import objects as O
+from . import _native as N
+
+_reg = {}
+_next = [1]
+_gen = [0]             # structure generation (invalidates cached device models)
+_rand = ["on"]         # FldLenRndSw state
+_last_fld = ["none"]
+_last_asm = ["none"]
+_fallback = ["cpu"]
+
+
+def _bump():
+    _gen[0] += 1
+
+
+def _add(o):
+    k = _next[0]
+    _next[0] += 1
+    _reg[k] = o
+    _bump()
+    return k
+
+
+def _obj(k):
+    try:
+        return _reg[int(k)]
+    except (KeyError, TypeError, ValueError):
+        raise RuntimeError("Radia::Error003: no object with index %r" % (k,))
+
+
+def _vec3(v, name="vector"):
+    a = np.asarray(v, dtype=np.float64).ravel()
+    if a.size != 3:
+        raise RuntimeError("%s must have 3 components" % name)
+    return a
+
+
+# ------------------------------------------------------------------ objects
+def ObjRecMag(P, L, M=(0, 0, 0)):
+    """Rectangular block with centre P, dimensions L, magnetization M (radpy.cpp:144)."""
+    return _add(O.RecMag(_vec3(P), _vec3(L), _vec3(M)))
+
+
+def ObjRecCur(P, L, J):
+    """Current-carrying rectangular block (radpy ObjRecCur)."""
+    return _add(O.RecMag(_vec3(P), _vec3(L), (0, 0, 0), _vec3(J)))
+
+
+def ObjPolyhdr(V, F, M=(0, 0, 0), *args, **kw):
+    """Convex polyhedron from vertices V and 1-based faces F (radpy.cpp:230-357)."""
+    faces = [[int(i) - 1 for i in f] for f in F]
+    return _add(O.Polyhedron(np.asarray(V, dtype=np.float64), faces, _vec3(M)))
+
+
+def ObjMltExtPgn(slices, M=(0, 0, 0)):
+    """Polyhedra between consecutive planar slices [[[x,y],...], z] (radapl1.cpp:859-880)."""
+    M = _vec3(M)
+    ids = []
+    for (pa, za), (pb, zb) in zip(slices[:-1], slices[1:]):
+        pa = np.asarray(pa, dtype=np.float64)
+        pb = np.asarray(pb, dtype=np.float64)
+        if len(pa) != len(pb):
+            raise RuntimeError("slices must have equal vertex counts")
+        k = len(pa)
+        verts = [[x, y, za] for x, y in pa] + [[x, y, zb] for x, y in pb]
+        faces = [list(range(k)), list(range(k, 2 * k))]
+        for i in range(k):
+            j = (i + 1) % k
+            faces.append([i, j, k + j, k + i])
+        o = O.Polyhedron(np.asarray(verts), faces, M)
+        ids.append(_add(o))
+    return ids[0] if len(ids) == 1 else ObjCnt(ids)
+
+
+def ObjArcCur(P, R, Phi, h, nseg, j, man_auto="man", a="z"):
+    """Arc current of rectangular section (radapl1.cpp:1664-1710)."""
+    if str(man_auto).lower() != "man":
+        raise NotImplementedError("only the 'man' (fixed-segment) arc integration is supported")
+    o = O.ArcCur(_vec3(P), R[0], R[1], Phi[0], Phi[1], h, nseg, j)
+    k = _add(o)
+    _orient_along(k, P, a)
+    return k
+
+
+def ObjRaceTrk(P, R, L, h, nseg, j, man_auto="man", a="z"):
+    """Racetrack coil: four 90-degree arcs plus straight blocks (radapl1.cpp:1879-1970)."""
+    P = _vec3(P)
+    rmin, rmax = float(R[0]), float(R[1])
+    lx, ly = float(L[0]), float(L[1])
+    pi = 3.14159265358979324
+    arcs = []
+    spec = [((P[0] + 0.5 * lx, P[1] + 0.5 * ly), (0.0, 0.5 * pi)),
+            ((P[0] - 0.5 * lx, P[1] + 0.5 * ly), (0.5 * pi, pi)),
+            ((P[0] - 0.5 * lx, P[1] - 0.5 * ly), (pi, 1.5 * pi)),
+            ((P[0] + 0.5 * lx, P[1] - 0.5 * ly), (1.5 * pi, 1.999999999999 * pi))]
+    for (cx, cy), (p0, p1) in spec:
+        arcs.append(_add(O.ArcCur((cx, cy, P[2]), rmin, rmax, p0, p1, h, nseg, j)))
+    members = list(arcs)
+    if ly != 0.0:
+        eff = 0.5 * (lx + rmin + rmax)
+        d = (rmax - rmin, ly, h)
+        members.append(ObjRecCur((P[0] + eff, P[1], P[2]), d, (0.0, j, 0.0)))
+        members.append(ObjRecCur((P[0] - eff, P[1], P[2]), d, (0.0, -j, 0.0)))
+    if lx != 0.0:
+        eff = 0.5 * (ly + rmin + rmax)
+        d = (lx, rmax - rmin, h)
+        members.append(ObjRecCur((P[0], P[1] + eff, P[2]), d, (-j, 0.0, 0.0)))
+        members.append(ObjRecCur((P[0], P[1] - eff, P[2]), d, (j, 0.0, 0.0)))
+    k = ObjCnt(members)
+    _orient_along(k, P, a)
+    return k
+
+
+def _orient_along(k, P, a):
+    a = str(a).lower()
+    if a == "z":
+        return
+    # rotation taking the local z axis to the requested axis about P
+    if a == "x":
+        t = O.Trf.rotation(P, (0, 1, 0), 0.5 * math.pi)
+    elif a == "y":
+        t = O.Trf.rotation(P, (1, 0, 0), -0.5 * math.pi)
+    else:
+        raise RuntimeError("orientation must be 'x', 'y' or 'z'")
+    _obj(k).add_trf(t, 1)
+    _bump()
+
+
+def ObjBckg(B):
+    return _add(O.Background(_vec3(B)))
+
+
+def ObjCnt(members):
+    for k in members:
+        _obj(k)
+    return _add(O.Container([int(k) for k in members]))
+
+
+def ObjAddToCnt(cnt, members):
+    c = _obj(cnt)
+    if not isinstance(c, O.Container):
+        raise RuntimeError("Radia::Error004: not a container")
+    c.members.extend(int(k) for k in members)
+    _bump()
+    return cnt
+
+
+def ObjCntStuf(obj):
+    o = _obj(obj)
+    return list(o.members) if isinstance(o, O.Container) else [obj]
+
+
+def ObjCntSize(obj):
+    o = _obj(obj)
+    return len(o.members) if isinstance(o, O.Container) else 0
+
+
+def ObjDivMag(obj, k, *args):
+    """Subdivide a block into kx*ky*kz blocks ([k1,k2,k3] or [[k1,q1],...]); the
+    object becomes a container of the pieces (radrec.cpp:971-1080)."""
+    o = _obj(obj)
+    if isinstance(o, O.Container):
+        for m in list(o.members):
+            ObjDivMag(m, k, *args)
+        return obj
+    if not isinstance(o, O.RecMag):
+        raise NotImplementedError("ObjDivMag is supported for rectangular blocks only")
+    spec = []
+    for kk in k:
+        if isinstance(kk, (list, tuple)):
+            spec.append((int(round(kk[0])), float(kk[1])))
+        else:
+            spec.append((int(round(kk)), 1.0))
+    edges = []
+    for ax in range(3):
+        cnt, q = spec[ax]
+        L = o.dims[ax]
+        if cnt < 1:
+            raise RuntimeError("subdivision numbers must be >= 1")
+        if cnt == 1 or q == 1.0:
+            w = np.full(cnt, L / cnt)
+        else:
+            r = q ** (1.0 / (cnt - 1))
+            w = r ** np.arange(cnt)
+            w = w * (L / w.sum())
+        e = o.center[ax] - 0.5 * L + np.concatenate([[0.0], np.cumsum(w)])
+        edges.append(e)
+    kids = []
+    for ix in range(len(edges[0]) - 1):
+        for iy in range(len(edges[1]) - 1):
+            for iz in range(len(edges[2]) - 1):
+                lo = np.array([edges[0][ix], edges[1][iy], edges[2][iz]])
+                hi = np.array([edges[0][ix + 1], edges[1][iy + 1], edges[2][iz + 1]])
+                b = O.RecMag(0.5 * (lo + hi), hi - lo, o.M, o.J)
+                b.mat = o.mat
+                kids.append(_add(b))
+    c = O.Container(kids)
+    c.trfs = o.trfs
+    _reg[int(obj)] = c
+    _bump()
+    return obj
+
+
+def ObjM(obj):
+    """[[centre, M], ...] of the magnetizable elements (radpy ObjM)."""
+    out = []
+    for o, _k, _s in O.walk(_reg, int(obj)):
+        if isinstance(o, O.RecMag):
+            out.append([list(o.center), list(o.M)])
+        elif isinstance(o, O.Polyhedron):
+            out.append([list(o.verts.mean(0)), list(o.M)])
+    return out[0] if len(out) == 1 else out
+
+
+def ObjSetM(obj, M):
+    M = _vec3(M)
+    for o, _k, _s in O.walk(_reg, int(obj)):
+        if isinstance(o, (O.RecMag, O.Polyhedron)) and not getattr(o, "is_current", False):
+            o.M = M.copy()
+    return obj
+
+
+# ------------------------------------------------------------------ transforms
+def TrfTrsl(V):
+    return _add(O.Trf.translation(_vec3(V)))
+
+
+def TrfRot(P, V, phi):
+    return _add(O.Trf.rotation(_vec3(P), _vec3(V), float(phi)))
+
+
+def TrfPlSym(P, N):
+    return _add(O.Trf.plane_symmetry(_vec3(P), _vec3(N)))
+
+
+def TrfInv():
+    return _add(O.Trf.field_inversion())
+
+
+def TrfCmbL(orig, trf):
+    t = _obj(orig)
+    _reg[int(orig)] = _obj(trf).compose(t)
+    return orig
+
+
+def TrfCmbR(orig, trf):
+    t = _obj(orig)
+    _reg[int(orig)] = t.compose(_obj(trf))
+    return orig
+
+
+def _trf(k):
+    t = _obj(k)
+    if not isinstance(t, O.Trf):
+        raise RuntimeError("Radia::Error012: not a transformation")
+    return t
+
+
+def TrfOrnt(obj, trf):
+    _obj(obj).add_trf(_trf(trf), 1)
+    _bump()
+    return obj
+
+
+def TrfMlt(obj, trf, mlt):
+    _obj(obj).add_trf(_trf(trf), int(mlt))
+    _bump()
+    return obj
+
+
+def TrfZerPerp(obj, P, N):
+    """Plane symmetry, multiplicity 2 (radentry.cpp:1802-1840)."""
+    _obj(obj).add_trf(O.Trf.plane_symmetry(_vec3(P), _vec3(N)), 2)
+    _bump()
+    return obj
+
+
+def TrfZerPara(obj, P, N):
+    """Plane symmetry combined with field inversion, multiplicity 2 (radentry.cpp:1802-1840)."""
+    t = O.Trf.plane_symmetry(_vec3(P), _vec3(N))
+    t.s = -1.0
+    _obj(obj).add_trf(t, 2)
+    _bump()
+    return obj
+
+
+# ------------------------------------------------------------------ materials
+def MatLin(ksi, mr):
+    """MatLin([ksipar, ksiper], mr | [mrx, mry, mrz]) (radmater.h:164-303)."""
+    if np.isscalar(ksi):
+        kpar = kper = float(ksi)
+    else:
+        kpar, kper = float(ksi[0]), float(ksi[1])
+    if np.isscalar(mr):
+        return _add(O.Material(O.MAT_LIN_ANISO, kpar=kpar, kper=kper, mr=float(mr)))
+    return _add(O.Material(O.MAT_LIN_ANISO, kpar=kpar, kper=kper, mr_vec=_vec3(mr)))
+
+
+def MatSatIsoFrm(*pairs):
+    """MatSatIsoFrm([ksi1,ms1],[ksi2,ms2],[ksi3,ms3]) (radentry.cpp:792-813)."""
+    ks = [float(p[0]) for p in pairs]
+    ms = [float(p[1]) for p in pairs]
+    return _add(O.Material(O.MAT_NONLIN_FRM, ks=ks, ms=ms))
+
+
+def MatApl(obj, mat):
+    m = _obj(mat)
+    if not isinstance(m, O.Material):
+        raise RuntimeError("Radia::Error007: not a material")
+    for o, _k, _s in O.walk(_reg, int(obj)):
+        if isinstance(o, (O.RecMag, O.Polyhedron)) and not getattr(o, "is_current", False):
+            o.mat = m
+    _bump()
+    return obj
+
+
+def MatMvsH(obj, comp, H):
+    o = None
+    for o, _k, _s in O.walk(_reg, int(obj)):
+        if getattr(o, "mat", None) is not None:
+            break
+    if o is None or o.mat is None:
+        raise RuntimeError("object has no material")
+    rec = o.mat.record(o.M)
+    M = N.mat_M(rec, _vec3(H))
+    return _component(M, comp or "m")
+
+
+# ------------------------------------------------------------------ relaxation
+class _Interaction:
+    pass
+
+
+def _ext_field_at(ext_items, pts):
+    """H of the external sources at lab points (element, currents, background)."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    H = np.zeros_like(pts)
+    mags = [it for it in ext_items if isinstance(it[0], (O.RecMag, O.Polyhedron))
+            and not getattr(it[0], "is_current", False)]
+    curs = [it for it in ext_items if isinstance(it[0], (O.ArcCur,)) or
+            (isinstance(it[0], O.RecMag) and it[0].is_current)]
+    if mags:
+        packed = O.pack_elements(mags)
+        with N.Model(packed) as m:
+            _b, h = m.field(packed["M"], pts, want_b=False)
+        H += h
+    if curs:
+        H += N.current_field(O.pack_currents(curs), pts)
+    for o, _k, stack in ext_items:
+        if isinstance(o, O.Background):
+            for t in O._expand(stack):
+                H += t.s * (t.M @ o.B)
+    return H
+
+
+def RlxPre(obj, srcobj=0, use_gpu=True):
+    """Builds the interaction structure of obj (radentry.cpp:923-930): device
+    model, device-resident interaction matrix and the external field."""
+    rel, ext = O.flatten(_reg, int(obj))
+    if srcobj:
+        _r2, e2 = O.flatten(_reg, int(srcobj))
+        ext = ext + [(o, k, s) for (o, k, s) in _r2 + e2]
+    if not rel:
+        raise RuntimeError("Radia::Error011: no relaxable elements (apply a material)")
+    it = _Interaction()
+    it.items = rel
+    it.packed = O.pack_elements(rel)
+    it.model = N.Model(it.packed)
+    it.matrix = it.model.assemble(N.RH_MIXED)
+    _last_asm[0] = "gpu"
+    X = it.model.colloc()
+    Hlab = _ext_field_at(ext, X) if ext else np.zeros_like(X)
+    # into element frames: s * Minv of the main copy (radintrc.cpp:1231)
+    co = it.packed["copy_off"]
+    hext = np.zeros_like(Hlab)
+    for e in range(len(rel)):
+        c = co[e]
+        hext[e] = it.packed["cS"][c] * np.linalg.solve(it.packed["cM"][c], Hlab[e])
+    it.hext = hext
+    it.mats = np.stack([o.mat.record(o.M) for (o, _k, _s) in rel])
+    it.H = np.zeros_like(hext)
+    return _add(it)
+
+
+def _opt(opts, name, default):
+    for s in opts:
+        for part in str(s).split(","):
+            if "->" in part:
+                k, v = part.split("->", 1)
+                if k.strip().lower() == name.lower():
+                    return v.strip()
+    return default
+
+
+def RlxAuto(intrc, prec, maxiter, meth=4, *opts):
+    """Relaxes to prec (T) within maxiter iterations; returns
+    [misfit, max|M|, max|H|, iterations] (radentry.h:746-761).  Every method
+    number runs the device solver (adaptive under-relaxed block Jacobi)."""
+    it = _obj(intrc)
+    if not isinstance(it, _Interaction):
+        raise RuntimeError("Radia::Error014: not an interaction structure")
+    zero_m = _opt(opts, "ZeroM", "True").lower() not in ("false", "0", "no")
+    omega = float(_opt(opts, "omega", "0"))
+    M0 = np.stack([o.M for (o, _k, _s) in it.items])
+    res, M, H = it.matrix.relax(it.mats, it.hext, M0, prec, maxiter, keep_m=not zero_m,
+                                omega0=omega)
+    for e, (o, _k, _s) in enumerate(it.items):
+        o.M = M[e].copy()
+    it.H = H
+    it.last = res
+    return [res.misfit, res.max_m, res.max_h, res.iters]
+
+
+def Solve(obj, prec, maxiter, meth=4):
+    """RlxPre + RlxAuto (radentry.cpp:1842-1855)."""
+    k = RlxPre(obj)
+    try:
+        return RlxAuto(k, prec, maxiter, meth)
+    finally:
+        UtiDel(k)
+
+
+# ------------------------------------------------------------------ field
+_cache = {}
+
+
+def _field_model(obj):
+    key = int(obj)
+    ent = _cache.get(key)
+    if ent is not None and ent[0] == _gen[0]:
+        return ent[1]
+    items = list(O.walk(_reg, key))
+    mags = [it for it in items if isinstance(it[0], (O.RecMag, O.Polyhedron))
+            and not getattr(it[0], "is_current", False)]
+    curs = [it for it in items if isinstance(it[0], O.ArcCur) or
+            (isinstance(it[0], O.RecMag) and it[0].is_current)]
+    bck = [it for it in items if isinstance(it[0], O.Background)]
+    packed = O.pack_elements(mags) if mags else None
+    model = N.Model(packed) if mags else None
+    cur = O.pack_currents(curs) if curs else None
+    ent = (_gen[0], (mags, packed, model, cur, bck))
+    if key in _cache and _cache[key][1][2] is not None:
+        _cache[key][1][2].close()
+    _cache[key] = ent
+    return ent[1]
+
+
+def _component(v, comp):
+    comp = comp.lower()
+    if len(comp) == 1:
+        return list(map(float, v))
+    idx = {"x": 0, "y": 1, "z": 2}
+    out = [float(v[idx[c]]) for c in comp[1:]]
+    return out[0] if len(out) == 1 else out
+
+
+def Fld(obj, comp, P, use_gpu=True, precision="double"):
+    """Field component(s) comp ('b','bx','h','hz',...) at one point or a list
+    of points (radpy.cpp:2389-2456)."""
+    comp = str(comp).lower()
+    if comp[0] not in "bh":
+        raise NotImplementedError("only B and H components are supported")
+    pts = np.asarray(P, dtype=np.float64)
+    single = pts.ndim == 1
+    pts = np.ascontiguousarray(pts.reshape(-1, 3))
+    mags, packed, model, cur, bck = _field_model(obj)
+    B = np.zeros_like(pts)
+    H = np.zeros_like(pts)
+    if model is not None:
+        M = np.stack([o.M for (o, _k, _s) in mags])
+        prec = N.RH_MIXED if str(precision).lower() == "single" else N.RH_FP64
+        b, h = model.field(M, pts, precision=prec)
+        B += b
+        H += h
+    if cur is not None:
+        c = N.current_field(cur, pts)
+        B += c
+        H += c
+    for o, _k, stack in bck:
+        for t in O._expand(stack):
+            v = t.s * (t.M @ o.B)
+            B += v
+            H += v
+    _last_fld[0] = "gpu"
+    src = B if comp[0] == "b" else H
+    res = [_component(v, comp) for v in src]
+    return res[0] if single else res
+
+
+# ------------------------------------------------------------------ utilities
+def FldLenRndSw(onoff):
+    """Length randomization switch.  The device formulas need no random
+    jitter (edge/plane limits are evaluated in cancellation-free forms), so
+    this only records the state."""
+    s = str(onoff).lower()
+    if s not in ("on", "off"):
+        raise RuntimeError("Radia::Error043")
+    _rand[0] = s
+    return 1 if s == "on" else 0
+
+
+def UtiDel(k):
+    o = _reg.pop(int(k), None)
+    if isinstance(o, _Interaction):
+        o.matrix.close()
+        o.model.close()
+    _bump()
+    return 0
+
+
+def UtiDelAll():
+    for k in list(_reg):
+        UtiDel(k)
+    for ent in _cache.values():
+        if ent[1][2] is not None:
+            ent[1][2].close()
+    _cache.clear()
+    _next[0] = 1
+    _bump()
+    return 0
+
+
+def UtiFldLastBackend():
+    return _last_fld[0]
+
+
+def UtiAsmLastBackend():
+    return _last_asm[0]
+
+
+def UtiGpuFallback(mode=None):
+    if mode is not None:
+        m = str(mode).lower()
+        if m not in ("cpu", "gpu_streaming", "break"):
+            raise RuntimeError("UtiGpuFallback: mode must be 'cpu', 'gpu_streaming' or 'break'")
+        _fallback[0] = m
+    return _fallback[0]
+
+
+def UtiVer():
+    return "radiacuda_amd 0.1 (gfx950)"

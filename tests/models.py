@@ -1,0 +1,94 @@
+"""Seeded model builders shared by the tests (Radia API calls only)."""
+import numpy as np
+
+from radiacuda_amd import rad
+from radiacuda_amd import objects as O
+
+TET_FACES = [[1, 2, 3], [1, 2, 4], [1, 3, 4], [2, 3, 4]]
+# Kuhn split of a unit cube (corner index = 4i + 2j + k) into 6 tetrahedra
+KUHN = [[0, 1, 3, 7], [0, 1, 5, 7], [0, 2, 3, 7], [0, 2, 6, 7], [0, 4, 5, 7], [0, 4, 6, 7]]
+HEX_FACES = [[1, 2, 4, 3], [5, 6, 8, 7], [1, 2, 6, 5], [3, 4, 8, 7], [1, 3, 7, 5], [2, 4, 8, 6]]
+
+
+def tet_mesh(shape, h, origin=(0, 0, 0), jitter=0.1, seed=0, M=None):
+    """Structured hexahedral grid, interior vertices jittered by +-jitter*h,
+    each cell split into 6 tetrahedra.  Returns the list of object ids."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    g = np.stack(np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1),
+                             indexing="ij"), -1).astype(float) * h + np.asarray(origin, float)
+    inner = np.zeros(g.shape[:3], bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    g[inner] += rng.uniform(-jitter * h, jitter * h, (inner.sum(), 3))
+    ids = []
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                c = np.array([g[i + a, j + b, k + d] for a in (0, 1) for b in (0, 1) for d in (0, 1)])
+                for t in KUHN:
+                    m = rng.uniform(-1, 1, 3) if M is None else M
+                    ids.append(rad.ObjPolyhdr(c[t].tolist(), TET_FACES, list(m)))
+    return ids
+
+
+def hex_block(center, dims, M, rot=None):
+    c = np.array([[x, y, z] for x in (-.5, .5) for y in (-.5, .5) for z in (-.5, .5)]) * dims
+    if rot is not None:
+        c = c @ np.asarray(rot).T
+    c = c + np.asarray(center, float)
+    return rad.ObjPolyhdr(c.tolist(), HEX_FACES, list(M))
+
+
+def mixed_model(seed=1, sym=True):
+    """Tets + blocks + a hexahedral polyhedron, with symmetries and a rotation."""
+    rng = np.random.default_rng(seed)
+    ids = tet_mesh((2, 1, 1), 4.0, origin=(1, 2, 3), jitter=0.08, seed=seed)
+    ids.append(rad.ObjRecMag([3, 11, 4], [2, 3, 4], list(rng.uniform(-1, 1, 3))))
+    ids.append(rad.ObjRecMag([-5, 4, 6], [3, 2, 2], list(rng.uniform(-1, 1, 3))))
+    ids.append(hex_block([-6, 1, 2], [3, 3, 2], rng.uniform(-1, 1, 3)))
+    a = rad.ObjCnt(ids[:6])
+    b = rad.ObjCnt(ids[6:])
+    if sym:
+        rad.TrfZerPara(a, [0, 0, 0], [0, 0, 1])
+        rad.TrfOrnt(b, rad.TrfRot([1, 1, 1], [0.2, 0.3, 1.0], 0.4))
+        rad.TrfZerPerp(b, [0, -1, 0], [0, 1, 0])
+    return rad.ObjCnt([a, b])
+
+
+def packed(obj, relax_only=False):
+    """Flat element arrays of every magnetizable element under obj."""
+    items = [it for it in O.walk(rad._reg, obj)
+             if isinstance(it[0], (O.RecMag, O.Polyhedron)) and not getattr(it[0], "is_current", False)]
+    return O.pack_elements(items), items
+
+
+def oracle_assemble(P):
+    """Oracle matrix with randomization off (FldLenRndSw('off')).  Blocks whose
+    collocation point lies exactly on a face plane of the source element are
+    0/0 in that mode (Radia's own behaviour); those blocks are recomputed with
+    Radia's default length randomization (1e-9 mm jitter).  Returns (Q, mask)
+    with mask True for the blocks that needed the jitter."""
+    from oracle import oracle as OR
+    OR.set_randomization(False)
+    om = OR.prepare_model(P)
+    Q = OR.assemble(om)
+    n = len(P["type"])
+    bad = ~np.isfinite(Q).reshape(n, 3, n, 3).all(axis=(1, 3))
+    if bad.any():
+        OR.set_randomization(True)
+        Qr = OR.assemble(OR.prepare_model(P))
+        OR.set_randomization(False)
+        for i, j in np.argwhere(bad):
+            Q[3 * i:3 * i + 3, 3 * j:3 * j + 3] = Qr[3 * i:3 * i + 3, 3 * j:3 * j + 3]
+    return Q, bad
+
+
+def block_rel(A, B, n, mask=None, tol_masked=None):
+    """Max over 3x3 blocks of |A-B|/max|B| (per block); blocks in mask are
+    returned separately."""
+    d = np.abs(A - B).reshape(n, 3, n, 3).max(axis=(1, 3))
+    s = np.abs(B).reshape(n, 3, n, 3).max(axis=(1, 3))
+    r = d / np.maximum(s, 1e-300)
+    if mask is None:
+        return r.max()
+    return (r[~mask].max() if (~mask).any() else 0.0), (r[mask].max() if mask.any() else 0.0)

@@ -1,0 +1,177 @@
+"""GPU parity of the three paths against the CPU oracle (FP64 unless stated).
+
+Tolerances: FP64 field and interaction blocks <= 1e-10 relative to the
+block / field scale; FP32 matrix storage <= 2e-7 relative per block;
+relaxed magnetization within a small multiple of the requested precision.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as OR
+from radiacuda_amd import _native as N
+from radiacuda_amd import objects as O
+from radiacuda_amd import rad
+from tests import models
+
+pytestmark = pytest.mark.gpu
+
+
+def _pts_off_planes(rng, n, lo, hi):
+    # random points; a random point lies on an element plane with probability 0
+    return rng.uniform(lo, hi, (n, 3))
+
+
+def _block_rel(A, B, n):
+    d = np.abs(A - B).reshape(n, 3, n, 3).max(axis=(1, 3))
+    s = np.abs(B).reshape(n, 3, n, 3).max(axis=(1, 3))
+    return (d / np.maximum(s, 1e-300)).max()
+
+
+@pytest.fixture
+def oracle_off():
+    OR.set_randomization(False)
+
+
+def test_field_fp64_matches_oracle(oracle_off):
+    print("case: field fp64 mixed model", flush=True)
+    top = models.mixed_model(seed=1)
+    P, _items = models.packed(top)
+    rng = np.random.default_rng(5)
+    pts = _pts_off_planes(rng, 700, -12, 16)
+    with N.Model(P) as m:
+        B, H = m.field(P["M"], pts)
+        om = OR.prepare_model(P)
+        Bo, Ho = OR.field(om, P["M"], pts)
+        assert np.abs(B - Bo).max() <= 1e-10 * np.abs(Bo).max()
+        assert np.abs(H - Ho).max() <= 1e-10 * np.abs(Ho).max()
+        # inside elements B = H + M (collocation points of each element)
+        X = OR.colloc(om)
+        B2, _ = m.field(P["M"], X)
+        Bo2, _ = OR.field(om, P["M"], X)
+        assert np.abs(B2 - Bo2).max() <= 1e-10 * np.abs(Bo2).max()
+
+
+def test_field_small_point_count_split(oracle_off):
+    """Few points and many elements exercise the element-split reduction."""
+    print("case: field split", flush=True)
+    ids = models.tet_mesh((4, 4, 2), 3.0, jitter=0.1, seed=3)
+    top = rad.ObjCnt(ids)
+    P, _ = models.packed(top)
+    rng = np.random.default_rng(1)
+    pts = rng.uniform(-3, 15, (37, 3))
+    with N.Model(P) as m:
+        B, _H = m.field(P["M"], pts)
+    Bo, _ = OR.field(OR.prepare_model(P), P["M"], pts)
+    assert np.abs(B - Bo).max() <= 1e-10 * np.abs(Bo).max()
+
+
+def test_assembly_fp64_matches_oracle(oracle_off):
+    print("case: assembly fp64", flush=True)
+    top = models.mixed_model(seed=2)
+    P, _ = models.packed(top)
+    n = len(P["type"])
+    with N.Model(P) as m:
+        q = m.assemble(N.RH_FP64)
+        Q = q.export()
+        q.close()
+    Qo, jit = models.oracle_assemble(P)
+    clean, jittered = models.block_rel(Q, Qo, n, jit)
+    assert np.isfinite(Q).all()
+    assert clean <= 1e-10
+    assert jittered <= 1e-6
+
+
+def test_assembly_fp32_storage(oracle_off):
+    print("case: assembly fp32 storage", flush=True)
+    ids = models.tet_mesh((3, 2, 2), 5.0, jitter=0.1, seed=4)
+    ids.append(rad.ObjRecMag([-6, 3, 2], [3, 4, 5], [0, 0, 1]))
+    top = rad.ObjCnt(ids)
+    P, _ = models.packed(top)
+    n = len(P["type"])
+    with N.Model(P) as m:
+        q = m.assemble(N.RH_MIXED)
+        Q = q.export()
+        q.close()
+    Qo, jit = models.oracle_assemble(P)
+    clean, jittered = models.block_rel(Q, Qo, n, jit)
+    assert clean <= 2e-7 and jittered <= 1e-6
+    # diagonal blocks are kept in FP64
+    for i in range(n):
+        blk = slice(3 * i, 3 * i + 3)
+        assert np.abs(Q[blk, blk] - Qo[blk, blk]).max() <= 1e-12 * np.abs(Qo[blk, blk]).max()
+
+
+def test_relax_linear_matches_direct_solve(oracle_off):
+    """Linear materials: the relaxed M is the solution of (I - chi Q) M = chi H + M_r."""
+    print("case: relax linear", flush=True)
+    ids = models.tet_mesh((3, 2, 2), 4.0, jitter=0.1, seed=6, M=[0, 0, 1.0])
+    iron = rad.ObjCnt(ids)
+    rad.MatApl(iron, rad.MatLin([0.06, 0.17], 1.2))
+    coil = rad.ObjRecCur([6, 4, -12], [20, 20, 6], [0, 0, 0])
+    rad.ObjRecCur([6, 4, 14], [8, 8, 4], [2.0, -1.0, 0.0])
+    src = rad.ObjRecCur([6, -9, 4], [6, 4, 10], [0.0, 0.0, 3.0])
+    top = rad.ObjCnt([iron, src])
+    P, items = models.packed(iron)
+    mats = np.stack([o.mat.record(o.M) for (o, _k, _s) in items])
+    k = rad.RlxPre(top)
+    it = rad._reg[k]
+    res = rad.RlxAuto(k, 1e-11, 2000)
+    M = np.stack([o.M for (o, _k, _s) in items])
+    Qo = OR.assemble(OR.prepare_model(P))
+    n = len(items)
+    A = np.eye(3 * n)
+    rhs = np.zeros(3 * n)
+    for e in range(n):
+        K = mats[e, 4:13].reshape(3, 3)
+        A[3 * e:3 * e + 3, :] -= K @ Qo[3 * e:3 * e + 3, :]
+        rhs[3 * e:3 * e + 3] = K @ it.hext[e] + mats[e, 13:16]
+    Mref = np.linalg.solve(A, rhs).reshape(n, 3)
+    assert res[0] <= 1e-11
+    assert np.abs(M - Mref).max() <= 1e-9 * np.abs(Mref).max()
+    del coil
+
+
+def test_relax_nonlinear_matches_gauss_seidel(oracle_off):
+    """Nonlinear iron driven by a coil: the device solver and the oracle's
+    method-4 Gauss-Seidel reach the same magnetization."""
+    print("case: relax nonlinear", flush=True)
+    ids = models.tet_mesh((3, 3, 2), 4.0, jitter=0.1, seed=7, M=[0, 0, 0])
+    iron = rad.ObjCnt(ids)
+    rad.MatApl(iron, rad.MatSatIsoFrm([2118, 1.362], [63.06, 0.2605], [17.138, 0.4917]))
+    coil = rad.ObjRecCur([6, 6, 13], [30, 30, 4], [0, 0, 0])
+    rad.ObjRecCur([6, 6, -6], [10, 10, 4], [1.0, 0, 0])
+    src = rad.ObjRecCur([-8, 6, 4], [4, 12, 8], [0, 0, 2.0])
+    top = rad.ObjCnt([iron, src])
+    P, items = models.packed(iron)
+    mats = np.stack([o.mat.record(o.M) for (o, _k, _s) in items])
+    k = rad.RlxPre(top)
+    it = rad._reg[k]
+    res = rad.RlxAuto(k, 1e-9, 3000)
+    print("gpu relax", res, flush=True)
+    M = np.stack([o.M for (o, _k, _s) in items])
+    Qo = OR.assemble(OR.prepare_model(P))
+    Mo, _H, iters, stats = OR.relax_m4(Qo.astype(np.float32), mats, it.hext, 1e-10, 20000)
+    print("oracle m4", iters, stats, flush=True)
+    scale = np.abs(Mo).max()
+    assert res[0] <= 1e-9
+    assert np.abs(M - Mo).max() <= 1e-5 * scale
+    del coil
+
+
+def test_current_sources_match_oracle(oracle_off):
+    print("case: currents", flush=True)
+    a = rad.ObjArcCur([1, 2, 3], [5, 8], [0.2, 1.9], 3, 12, 2.5, "man", "z")
+    b = rad.ObjRecCur([4, -2, 1], [2, 3, 4], [1.5, -2.0, 0.7])
+    top = rad.ObjCnt([a, b])
+    rad.TrfOrnt(top, rad.TrfRot([0, 0, 0], [1, 1, 0], 0.3))
+    rng = np.random.default_rng(2)
+    pts = rng.uniform(-12, 12, (300, 3))
+    B = np.array(rad.Fld(top, "b", pts.tolist()))
+    t = O.Trf.rotation([0, 0, 0], [1, 1, 0], 0.3)
+    Bo = np.zeros_like(pts)
+    for i, p in enumerate(pts):
+        xl = np.linalg.solve(t.M, p - t.V)
+        b1, _ = OR.arccur_BH(xl, [1, 2, 3], 5, 8, 0.2, 1.9, 3, 2.5, 12)
+        b2, _ = OR.reccur_BH(xl, [4, -2, 1], [2, 3, 4], [1.5, -2.0, 0.7])
+        Bo[i] = t.M @ (b1 + b2)
+    assert np.abs(B - Bo).max() <= 1e-10 * np.abs(Bo).max()

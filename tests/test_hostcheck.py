@@ -1,0 +1,150 @@
+"""CPU build of the exact kernel math (rh_math.hpp / rh_eval.hpp / rh_current.hpp,
+compiled for the host into _rhcheck.so) against the oracle: validates the
+formulation and the packing without a GPU."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as OR
+from radiacuda_amd import _abi
+from radiacuda_amd import objects as O
+from radiacuda_amd import rad
+from tests import models
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(os.path.dirname(HERE), "radiacuda_amd", "_rhcheck.so")
+dp = ctypes.POINTER(ctypes.c_double)
+
+
+@pytest.fixture(scope="module")
+def L():
+    if not os.path.exists(LIB):
+        from radiacuda_amd import build
+        build.build()
+    return ctypes.CDLL(LIB)
+
+
+@pytest.fixture(autouse=True)
+def _rand_off():
+    OR.set_randomization(False)
+
+
+def _assemble(L, P, fp32=0):
+    desc, keep = _abi.model_desc(P)
+    n = len(P["type"])
+    Q = np.zeros((3 * n, 3 * n))
+    rc = L.rhc_assemble(ctypes.byref(desc), ctypes.c_int64(0), ctypes.c_int64(n), ctypes.c_int32(fp32),
+                        Q.ctypes.data_as(dp))
+    assert rc == 0
+    return Q
+
+
+def _field(L, P, M, pts, fp32=0):
+    desc, keep = _abi.model_desc(P)
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    B = np.zeros_like(pts)
+    H = np.zeros_like(pts)
+    rc = L.rhc_field(ctypes.byref(desc), M.ctypes.data_as(dp), pts.ctypes.data_as(dp),
+                     ctypes.c_int64(len(pts)), ctypes.c_int32(fp32), B.ctypes.data_as(dp),
+                     H.ctypes.data_as(dp))
+    assert rc == 0
+    return B, H
+
+
+def _block_rel(A, B, n):
+    d = np.abs(A - B).reshape(n, 3, n, 3).max(axis=(1, 3))
+    s = np.abs(B).reshape(n, 3, n, 3).max(axis=(1, 3))
+    return (d / np.maximum(s, 1e-300)).max()
+
+
+def test_assembly_formulation_matches_oracle(L):
+    top = models.mixed_model(seed=3)
+    P, _ = models.packed(top)
+    n = len(P["type"])
+    Q = _assemble(L, P)
+    Qo, jit = models.oracle_assemble(P)
+    clean, jittered = models.block_rel(Q, Qo, n, jit)
+    assert np.isfinite(Q).all()
+    assert clean <= 1e-10
+    assert jittered <= 1e-6
+
+
+def test_field_formulation_matches_oracle(L):
+    top = models.mixed_model(seed=4)
+    P, _ = models.packed(top)
+    rng = np.random.default_rng(9)
+    pts = rng.uniform(-12, 16, (300, 3))
+    B, H = _field(L, P, P["M"], pts)
+    Bo, Ho = OR.field(OR.prepare_model(P), P["M"], pts)
+    assert np.abs(B - Bo).max() <= 1e-11 * np.abs(Bo).max()
+    assert np.abs(H - Ho).max() <= 1e-11 * np.abs(Ho).max()
+
+
+def test_fp32_field_error_is_bounded(L):
+    """FP32 evaluation of the integrals (the mixed path): stated error vs FP64."""
+    ids = models.tet_mesh((3, 3, 2), 4.0, jitter=0.1, seed=5)
+    top = rad.ObjCnt(ids)
+    P, _ = models.packed(top)
+    rng = np.random.default_rng(2)
+    pts = rng.uniform(-4, 16, (200, 3))
+    B64, _ = _field(L, P, P["M"], pts, 0)
+    B32, _ = _field(L, P, P["M"], pts, 1)
+    err = np.abs(B32 - B64).max() / np.abs(B64).max()
+    assert err < 5e-4  # reference's FP32 face-kernel bound (tests/test_gpu_fld_fp32.py:72)
+
+
+def test_current_sources_match_oracle(L):
+    rot = O.Trf.rotation([1, 1, 1], [0.2, 0.3, 1], 0.4)
+    cur = [(O.RecMag([1, 2, 3], [2, 3, 4], (0, 0, 0), (1.5, -2, 0.7)), 0, ()),
+           (O.ArcCur([0, 0, 1], 5, 8, 0.2, 1.7, 3, 10, 2.5), 1, ([(rot, 1)],))]
+    pc = O.pack_currents(cur)
+    ns = len(pc["kind"])
+    cp = np.zeros((ns, 24))
+    for s in range(ns):
+        cp[s, :9] = np.linalg.inv(pc["cM"][s]).ravel()
+        cp[s, 9:12] = pc["cV"][s]
+        cp[s, 12:21] = (pc["cS"][s] * pc["cM"][s]).ravel()
+    rng = np.random.default_rng(4)
+    pts = np.ascontiguousarray(rng.uniform(-10, 10, (100, 3)))
+    B = np.zeros_like(pts)
+    kind = np.ascontiguousarray(pc["kind"])
+    par = np.ascontiguousarray(pc["par"])
+    L.rhc_current(ctypes.c_int64(ns), kind.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                  par.ctypes.data_as(dp), cp.ctypes.data_as(dp), pts.ctypes.data_as(dp),
+                  ctypes.c_int64(len(pts)), B.ctypes.data_as(dp))
+    Bo = np.zeros_like(pts)
+    for i, p in enumerate(pts):
+        b, _ = OR.reccur_BH(p, [1, 2, 3], [2, 3, 4], (1.5, -2, 0.7))
+        Bo[i] += b
+        xl = np.linalg.solve(rot.M, p - rot.V)
+        b, _ = OR.arccur_BH(xl, [0, 0, 1], 5, 8, 0.2, 1.7, 3, 2.5, 10)
+        Bo[i] += rot.M @ b
+    assert np.abs(B - Bo).max() <= 1e-12 * np.abs(Bo).max()
+
+
+def test_symmetric_model_equals_explicit_mirror(L):
+    """A TrfZerPara symmetry equals adding the mirrored elements explicitly with
+    inverted, mirrored magnetization (radentry.cpp:1802-1840)."""
+    rng = np.random.default_rng(8)
+    ids = models.tet_mesh((1, 1, 1), 3.0, origin=(1, 1, 2), jitter=0.1, seed=8)
+    a = rad.ObjCnt(ids)
+    rad.TrfZerPara(a, [0, 0, 0], [0, 0, 1])
+    P, items = models.packed(a)
+    rad.UtiDelAll()
+    ids2 = models.tet_mesh((1, 1, 1), 3.0, origin=(1, 1, 2), jitter=0.1, seed=8)
+    mir = []
+    S = np.diag([1.0, 1.0, -1.0])
+    for k in ids2:
+        o = rad._reg[k]
+        v = o.verts @ S.T
+        mir.append(rad.ObjPolyhdr(v.tolist(), [[i + 1 for i in f] for f in o.faces],
+                                  list(-(S @ o.M))))
+    b = rad.ObjCnt(ids2 + mir)
+    P2, _ = models.packed(b)
+    pts = rng.uniform(-6, 6, (100, 3))
+    B1, _ = _field(L, P, P["M"], pts)
+    B2, _ = _field(L, P2, P2["M"], pts)
+    assert np.abs(B1 - B2).max() <= 1e-12 * np.abs(B2).max()
