@@ -92,3 +92,70 @@ def block_rel(A, B, n, mask=None, tol_masked=None):
     if mask is None:
         return r.max()
     return (r[~mask].max() if (~mask).any() else 0.0), (r[mask].max() if mask.any() else 0.0)
+
+
+def oracle_field(P, M, pts):
+    """Oracle B, H with randomization off; points where that is 0/0 (exactly on
+    a face plane) are recomputed with Radia's default length randomization.
+    Returns (B, H, mask of recomputed points)."""
+    from oracle import oracle as OR
+    OR.set_randomization(False)
+    om = OR.prepare_model(P)
+    B, H = OR.field(om, M, pts)
+    bad = ~(np.isfinite(B).all(axis=1) & np.isfinite(H).all(axis=1))
+    if bad.any():
+        OR.set_randomization(True)
+        Br, Hr = OR.field(OR.prepare_model(P), M, np.asarray(pts)[bad])
+        OR.set_randomization(False)
+        B[bad] = Br
+        H[bad] = Hr
+    return B, H, bad
+
+
+def mat_M_and_D(rec, H):
+    """M(H) and dM/dH of a material record (kinds 0 linear iso, 1 linear
+    aniso, 2 tanh-sum formula), FP64, for the test-side reference solver."""
+    kind = int(rec[0])
+    mr = rec[13:16]
+    if kind == 0:
+        return rec[1] * H + mr, rec[1] * np.eye(3)
+    if kind == 1:
+        K = rec[4:13].reshape(3, 3)
+        return K @ H + mr, K
+    h = float(np.linalg.norm(H))
+    ms, ks = rec[2:2 + int(rec[1])], rec[5:5 + int(rec[1])]
+    f = sum(m * np.tanh(k * h / m) for m, k in zip(ms, ks) if m != 0)
+    def sech2(x):
+        e = np.exp(-2.0 * abs(x))
+        return 4.0 * e / (1.0 + e) ** 2
+    fp = sum(k * sech2(k * h / m) for m, k in zip(ms, ks) if m != 0)
+    if h == 0:
+        return mr.copy(), fp * np.eye(3)
+    u = H / h
+    chi = f / h
+    return chi * H + mr, chi * np.eye(3) + (fp - chi) * np.outer(u, u)
+
+
+def fp64_fixed_point(Q, mats, hext, M0=None, tol=1e-14, max_iter=100):
+    """Newton polish of M = M_mat(Q M + hext) in FP64 with the exact Jacobian
+    (dense, small n), started from M0 (e.g. the oracle's Gauss-Seidel result):
+    the reference the device solver is checked against on the very matrix the
+    device stores."""
+    n = len(mats)
+    if M0 is None:
+        M = np.array([r[13:16] for r in mats], dtype=np.float64).ravel()
+    else:
+        M = np.asarray(M0, dtype=np.float64).ravel().copy()
+    for _ in range(max_iter):
+        H = (Q @ M).reshape(n, 3) + hext
+        F = np.zeros(3 * n)
+        D = np.zeros((3 * n, 3 * n))
+        for e in range(n):
+            m, d = mat_M_and_D(mats[e], H[e])
+            F[3 * e:3 * e + 3] = M[3 * e:3 * e + 3] - m
+            D[3 * e:3 * e + 3, 3 * e:3 * e + 3] = d
+        if np.abs(F).max() <= tol:
+            break
+        J = np.eye(3 * n) - D @ Q
+        M = M - np.linalg.solve(J, F)
+    return M.reshape(n, 3)
