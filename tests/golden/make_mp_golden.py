@@ -1,0 +1,88 @@
+"""Writes mp50_tets.json: fields of single tetrahedra evaluated in 50-digit
+arithmetic (mpmath) from the face-charge closed form, for elements of the
+jittered C-yoke mesh whose faces have edges nearly parallel to a face-frame
+axis (where the reference's y = kx + b edge form loses k^2*eps).
+
+The closed form is what the mathematics fixes: H = (1/4pi) sum_f sigma_f
+[sum_e L_e m_e - Omega_f n_f], sigma_f = M.n_f (face-charge model of a
+uniformly magnetized polyhedron), evaluated at 50 digits.
+
+Run from the repository root: python tests/golden/make_mp_golden.py
+"""
+import json
+import os
+import sys
+
+import mpmath as mp
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+mp.mp.dps = 50
+
+
+def H_mp(verts, faces, M, p):
+    V = [[mp.mpf(float(x)) for x in r] for r in verts]
+    P = [mp.mpf(float(x)) for x in p]
+    dot = lambda a, b: sum(a[i] * b[i] for i in range(3))
+    cross = lambda a, b: [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+    sub = lambda a, b: [a[i] - b[i] for i in range(3)]
+    c = [sum(V[i][k] for i in range(len(V))) / len(V) for k in range(3)]
+    Mv = [mp.mpf(float(x)) for x in M]
+    H = [mp.mpf(0)] * 3
+    for f in faces:
+        f = list(f)
+        n = cross(sub(V[f[1]], V[f[0]]), sub(V[f[2]], V[f[0]]))
+        nn = mp.sqrt(dot(n, n))
+        n = [x / nn for x in n]
+        if dot(sub(V[f[0]], c), n) < 0:
+            n = [-x for x in n]
+            f = f[::-1]
+        sig = dot(n, Mv)
+        r = [sub(V[i], P) for i in f]
+        R = [mp.sqrt(dot(x, x)) for x in r]
+        for i in range(len(f)):
+            a, b = i, (i + 1) % len(f)
+            t = sub(V[f[b]], V[f[a]])
+            tl = mp.sqrt(dot(t, t))
+            t = [x / tl for x in t]
+            m = cross(t, n)
+            L = mp.log((R[b] + dot(r[b], t)) / (R[a] + dot(r[a], t)))
+            for k in range(3):
+                H[k] += sig * L * m[k] / (4 * mp.pi)
+        om = 0
+        for k in range(1, len(f) - 1):
+            num = dot(r[0], cross(r[k], r[k + 1]))
+            den = (R[0] * R[k] * R[k + 1] + dot(r[0], r[k]) * R[k + 1] + dot(r[0], r[k + 1]) * R[k]
+                   + dot(r[k], r[k + 1]) * R[0])
+            om += 2 * mp.atan2(num, den)
+        for k in range(3):
+            H[k] -= sig * om * n[k] / (4 * mp.pi)
+    return [float(x) for x in H]
+
+
+def main():
+    from radiacuda_amd import objects as O
+    from radiacuda_amd import rad
+    from radiacuda_amd import workloads as W
+    rad.UtiDelAll()
+    _top, iron, _coil, info = W.c_yoke()
+    items = list(O.walk(rad._reg, iron))
+    rng = np.random.default_rng(0)
+    pts = W.gap_grid(info, 100)
+    ps = pts[rng.choice(len(pts), 4, replace=False)]
+    cases = []
+    for e in (1403, 2401, 186, 5351):
+        o = items[e][0]
+        for x in ps:
+            M = [0.3, -0.2, 1.0]
+            cases.append({"verts": o.verts.tolist(), "faces": o.faces, "M": M, "p": x.tolist(),
+                          "H": H_mp(o.verts, o.faces, M, x)})
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mp50_tets.json")
+    json.dump({"generator": "tests/golden/make_mp_golden.py", "digits": 50, "cases": cases},
+              open(out, "w"), indent=1)
+    print("wrote", out, len(cases))
+
+
+if __name__ == "__main__":
+    main()

@@ -163,7 +163,8 @@ def test_relax_nonlinear_matches_fixed_point(oracle_off):
     Mn = models.fp64_fixed_point(Qs, mats, it.hext, M0=Mo)
     scale = np.abs(Mn).max()
     assert res[0] <= 1e-11
-    assert np.abs(M - Mn).max() <= 1e-9 * scale
+    # residual <= 1e-11 T times the conditioning of chi~2000 iron
+    assert np.abs(M - Mn).max() <= 1e-7 * scale
     # FP32 dot products move the Gauss-Seidel fixed point by ~1.3e-4 here
     assert np.abs(M - Mo).max() <= 5e-4 * scale
 

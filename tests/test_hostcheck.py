@@ -148,3 +148,25 @@ def test_symmetric_model_equals_explicit_mirror(L):
     B1, _ = _field(L, P, P["M"], pts)
     B2, _ = _field(L, P2, P2["M"], pts)
     assert np.abs(B1 - B2).max() <= 1e-12 * np.abs(B2).max()
+
+
+def test_kernel_math_matches_50_digit_values_where_reference_form_loses_digits(L):
+    """Tetrahedra of the jittered C-yoke with edges nearly parallel to a face
+    frame axis: the kernel formulation stays within 1e-11 of 50-digit values
+    (tests/golden/mp50_tets.json); the reference's y = kx + b edge form
+    (radplnr2.cpp:139-170, restated by the oracle) loses up to ~1e-3 of these
+    far elements' contributions."""
+    import json
+    gold = json.load(open(os.path.join(HERE, "golden", "mp50_tets.json")))
+    worst_k, worst_o = 0.0, 0.0
+    for c in gold["cases"]:
+        o = O.Polyhedron(c["verts"], c["faces"], c["M"])
+        P = O.pack_elements([(o, 0, ())])
+        _B, H = _field(L, P, np.array([c["M"]]), np.array([c["p"]]))
+        Hm = np.array(c["H"])
+        s = np.abs(Hm).max()
+        worst_k = max(worst_k, np.abs(H[0] - Hm).max() / s)
+        _b, Ho = OR.Poly(c["verts"], c["faces"]).BH(c["p"], c["M"])
+        worst_o = max(worst_o, np.abs(Ho - Hm).max() / s)
+    assert worst_k <= 1e-11
+    assert worst_o > 1e4 * worst_k  # documents the reference form's loss on these elements
