@@ -118,6 +118,7 @@ struct rh_model {
     double* d_copies = nullptr;
     double* d_colloc = nullptr;
     double* d_rowt = nullptr;
+    int32_t* d_kidx = nullptr;
     // growable workspace for the field entry points
     double* d_work = nullptr;
     size_t work_cap = 0;
@@ -156,6 +157,7 @@ void free_model(rh_model* m)
     dfree(m->d_copies);
     dfree(m->d_colloc);
     dfree(m->d_rowt);
+    dfree(m->d_kidx);
     dfree(m->d_work);
     delete m;
 }
@@ -221,6 +223,7 @@ rh_status rh_model_create(const rh_model_desc* desc, int device, void* stream, r
     if (e == hipSuccess) e = upload(&m->d_copies, P.copies, s);
     if (e == hipSuccess) e = upload(&m->d_colloc, P.colloc, s);
     if (e == hipSuccess) e = upload(&m->d_rowt, P.rowt, s);
+    if (e == hipSuccess) e = upload(&m->d_kidx, P.kind_idx, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         std::string msg = std::string("model upload: ") + hipGetErrorString(e);
@@ -236,6 +239,8 @@ rh_status rh_model_create(const rh_model_desc* desc, int device, void* stream, r
     d.copies = m->d_copies;
     d.colloc = m->d_colloc;
     d.rowt = m->d_rowt;
+    d.kidx = m->d_kidx;
+    for (int k = 0; k < 4; k++) d.koff[k] = P.kind_off[k];
     d.sym = P.sym;
     d.rowt_identity = P.rowt_identity;
     *out = m;
@@ -640,15 +645,15 @@ rh_status rh_relax(rh_matrix* q, const double* mats, const double* hext, double*
 static rh_status field_dev_impl(rh_model* m, const double* d_M, const double* d_pts, int64_t np,
                                 double* d_B, double* d_H, bool fp32)
 {
-    const int64_t splits = rh::field_splits(np, m->host.n);
+    const int64_t slices = rh::field_slices(m->dev, np);
     double* work = nullptr;
-    if (splits > 1) {
-        RH_HIP(grow(m, size_t(6 * np * splits)));
+    if (slices > 1) {
+        RH_HIP(grow(m, size_t(6 * np * slices)));
         work = m->d_work;
     }
     Timer t("field", m->stream);
     t.begin();
-    RH_HIP(rh::launch_field(m->dev, d_M, d_pts, np, d_B, d_H, work, splits, fp32, m->stream));
+    RH_HIP(rh::launch_field(m->dev, d_M, d_pts, np, d_B, d_H, work, fp32, m->stream));
     t.end();
     return RH_OK;
 }

@@ -34,15 +34,26 @@ RH_HD void mat3_vec(const double* m, const double v[3], double o[3])
     o[2] = m[6] * v[0] + m[7] * v[1] + m[8] * v[2];
 }
 
+// Element evaluation specialised on the kind slot (0 block, 1 tetrahedron,
+// 2 general polyhedron, -1 = read the kind at run time).
+template <class R, int KIND>
+RH_HD bool eval_k(int kind, const double* __restrict__ rec, const double p[3], double q6[6])
+{
+    if (KIND == 0) return eval_rec<R>(rec, p, q6);
+    if (KIND == 1) return eval_tet<R>(rec, p, q6);
+    if (KIND == 2) return eval_gen<R>(rec, p, q6);
+    return eval_elem<R>(kind, rec, p, q6);
+}
+
 // Sum over copies of F_c Q_j(x'), without the row transform.
-template <class R, bool SYM>
+template <class R, bool SYM, int KIND = -1>
 RH_HD void block_Q(const ModelView& mv, int64_t j, const double x[3], double q9[9])
 {
-    const int kind = mv.kind[j];
+    const int kind = (KIND < 0) ? mv.kind[j] : 0;
     const double* rec = mv.data + mv.off[j];
     if (!SYM) {
         double q6[6];
-        eval_elem<R>(kind, rec, x, q6);
+        eval_k<R, KIND>(kind, rec, x, q6);
         q6_to_q9(q6, q9);
         return;
     }
@@ -51,7 +62,7 @@ RH_HD void block_Q(const ModelView& mv, int64_t j, const double x[3], double q9[
         const double* cp = mv.copies + 24 * c;
         double xl[3], q6[6], ql[9];
         apply_inv(cp, x, xl);
-        eval_elem<R>(kind, rec, xl, q6);
+        eval_k<R, KIND>(kind, rec, xl, q6);
         q6_to_q9(q6, ql);
         const double* F = cp + 12;
         for (int a = 0; a < 3; a++)
@@ -60,18 +71,20 @@ RH_HD void block_Q(const ModelView& mv, int64_t j, const double x[3], double q9[
     }
 }
 
-// Adds the B and H of elements [j0, j1) at x to B, H.
-template <class R, bool SYM>
-RH_HD void field_add(const ModelView& mv, const double* M, int64_t j0, int64_t j1, const double x[3],
-                     double B[3], double H[3])
+// Adds the B and H of the elements idx[k0..k1) (all of kind slot KIND, or of
+// any kind when KIND < 0; idx == nullptr means the identity list) at x.
+template <class R, bool SYM, int KIND = -1>
+RH_HD void field_add(const ModelView& mv, const double* M, const int32_t* idx, int64_t k0,
+                     int64_t k1, const double x[3], double B[3], double H[3])
 {
-    for (int64_t j = j0; j < j1; j++) {
-        const int kind = mv.kind[j];
+    for (int64_t k = k0; k < k1; k++) {
+        const int64_t j = idx ? int64_t(idx[k]) : k;
+        const int kind = (KIND < 0) ? mv.kind[j] : 0;
         const double* rec = mv.data + mv.off[j];
         const double m[3] = {M[3 * j], M[3 * j + 1], M[3 * j + 2]};
         if (!SYM) {
             double q6[6];
-            bool in = eval_elem<R>(kind, rec, x, q6);
+            bool in = eval_k<R, KIND>(kind, rec, x, q6);
             double h0 = q6[0] * m[0] + q6[3] * m[1] + q6[4] * m[2];
             double h1 = q6[3] * m[0] + q6[1] * m[1] + q6[5] * m[2];
             double h2 = q6[4] * m[0] + q6[5] * m[1] + q6[2] * m[2];
@@ -85,7 +98,7 @@ RH_HD void field_add(const ModelView& mv, const double* M, int64_t j0, int64_t j
             const double* cp = mv.copies + 24 * c;
             double xl[3], q6[6];
             apply_inv(cp, x, xl);
-            bool in = eval_elem<R>(kind, rec, xl, q6);
+            bool in = eval_k<R, KIND>(kind, rec, xl, q6);
             double hl[3] = {q6[0] * m[0] + q6[3] * m[1] + q6[4] * m[2],
                             q6[3] * m[0] + q6[1] * m[1] + q6[5] * m[2],
                             q6[4] * m[0] + q6[5] * m[1] + q6[2] * m[2]};

@@ -144,7 +144,12 @@ void pack_tet(std::vector<Vec> V, std::vector<double>& d)
         d.push_back(len);
         for (int k = 0; k < 6; k++) d.push_back(w6[k]);
     }
-    for (int f = 0; f < 4; f++) append(d, n[f]);
+    for (int f = 0; f < 4; f++) {
+        const int* fv = kTetFace[f];
+        Vec c = crs(sub(V[fv[1]], V[fv[0]]), sub(V[fv[2]], V[fv[0]]));
+        append(d, n[f]);
+        d.push_back(nrm(c));  // twice the face area
+    }
 }
 
 void pack_gen(const std::vector<Vec>& V, const std::vector<Face>& F, std::vector<double>& d)
@@ -291,6 +296,14 @@ rh_status pack_model(const rh_model_desc* d, PackedModel* out, std::string* err)
             P.counts[7] += int64_t(F.size());
         }
     }
+    // element lists per kind slot: 0 blocks, 1 tetrahedra, 2 general polyhedra
+    for (int slot = 0; slot < 3; slot++) {
+        const int32_t want = (slot == 0) ? EK_REC : (slot == 1 ? EK_TET : EK_GEN);
+        P.kind_off[slot] = int64_t(P.kind_idx.size());
+        for (int64_t e = 0; e < n; e++)
+            if (P.kind[e] == want) P.kind_idx.push_back(int32_t(e));
+    }
+    P.kind_off[3] = int64_t(P.kind_idx.size());
     // copies
     const int64_t C = d->copy_off[n];
     P.counts[0] = n;

@@ -22,6 +22,8 @@ struct PackedModel {
     std::vector<double> copies;     // [kCopySize * C]
     std::vector<double> colloc;     // [3n] collocation points, lab frame
     std::vector<double> rowt;       // [9n] row transforms s*Minv of the main copy
+    std::vector<int32_t> kind_idx;  // element indices grouped by kind slot (block, tet, general)
+    int64_t kind_off[4] = {0, 0, 0, 0};
     bool sym = false;               // some element has a non-identity or extra copy
     bool rowt_identity = true;      // every row transform is the identity
     int64_t counts[8] = {0};        // see rh_model_counts

@@ -55,8 +55,8 @@ int rhc_field(const rh_model_desc* d, const double* M, const double* pts, int64_
     rh::ModelView mv = view(P);
     for (int64_t p = 0; p < np; p++) {
         double b[3] = {0, 0, 0}, h[3] = {0, 0, 0};
-        if (fp32) rh::field_add<float, true>(mv, M, 0, P.n, pts + 3 * p, b, h);
-        else rh::field_add<double, true>(mv, M, 0, P.n, pts + 3 * p, b, h);
+        if (fp32) rh::field_add<float, true>(mv, M, nullptr, 0, P.n, pts + 3 * p, b, h);
+        else rh::field_add<double, true>(mv, M, nullptr, 0, P.n, pts + 3 * p, b, h);
         for (int k = 0; k < 3; k++) {
             if (B) B[3 * p + k] = b[k];
             if (H) H[3 * p + k] = h[k];

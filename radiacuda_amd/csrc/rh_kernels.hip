@@ -31,13 +31,15 @@ template <class T>
 __device__ __forceinline__ void store_out(T* p, double v) { *p = T(v); }
 
 // ---------------------------------------------------------------- assembly
-template <class R, bool SYM, bool ROWT, class OUT>
+// Block columns are the elements kidx[k0 .. k1) of one kind slot.
+template <class R, bool SYM, bool ROWT, class OUT, int KIND>
 __global__ __launch_bounds__(64) void k_assemble(DevModel m, OUT* __restrict__ Q, int64_t ld,
                                                  double* __restrict__ qdiag, int64_t jt)
 {
     const int64_t i = int64_t(blockIdx.x) * 64 + threadIdx.x;
-    const int64_t j0 = int64_t(blockIdx.y) * jt;
-    const int64_t j1 = (j0 + jt < m.n) ? (j0 + jt) : m.n;
+    const int64_t kend = m.koff[KIND + 1];
+    const int64_t k0 = m.koff[KIND] + int64_t(blockIdx.y) * jt;
+    const int64_t k1 = (k0 + jt < kend) ? (k0 + jt) : kend;
     if (i >= m.n) return;
     const ModelView mv = view(m);
     const double x[3] = {m.colloc[3 * i], m.colloc[3 * i + 1], m.colloc[3 * i + 2]};
@@ -45,9 +47,10 @@ __global__ __launch_bounds__(64) void k_assemble(DevModel m, OUT* __restrict__ Q
     if (ROWT)
         for (int k = 0; k < 9; k++) rt[k] = m.rowt[9 * i + k];
     OUT* row = Q + (3 * i) * ld;
-    for (int64_t j = j0; j < j1; j++) {
+    for (int64_t k = k0; k < k1; k++) {
+        const int64_t j = m.kidx[k];
         double q[9];
-        block_Q<R, SYM>(mv, j, x, q);
+        block_Q<R, SYM, KIND>(mv, j, x, q);
         if (ROWT) {
             double t[9];
             for (int a = 0; a < 3; a++)
@@ -70,29 +73,48 @@ __global__ __launch_bounds__(64) void k_assemble(DevModel m, OUT* __restrict__ Q
 }
 
 template <bool SYM, bool ROWT, class OUT>
-hipError_t asm_dispatch(const DevModel& m, OUT* Q, int64_t ld, double* qdiag, dim3 grid, int64_t jt,
-                        hipStream_t s)
+hipError_t asm_dispatch(const DevModel& m, OUT* Q, int64_t ld, double* qdiag, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt);
-    return hipGetLastError();
+    const int64_t gx = (m.n + 63) / 64;
+    for (int slot = 0; slot < 3; slot++) {
+        const int64_t cnt = m.koff[slot + 1] - m.koff[slot];
+        if (cnt == 0) continue;
+        // enough workgroups to fill 256 CUs several times over
+        const int64_t gy_target = std::max<int64_t>(1, (8192 + gx - 1) / gx);
+        int64_t jt = std::max<int64_t>(4, (cnt + gy_target - 1) / gy_target);
+        jt = std::min<int64_t>(jt, 256);
+        const dim3 grid{unsigned(gx), unsigned((cnt + jt - 1) / jt), 1u};
+        if (slot == 0)
+            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 0>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt);
+        else if (slot == 1)
+            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 1>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt);
+        else
+            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 2>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------- field
-template <class R, bool SYM>
+// Sums the elements kidx[koff[KIND] + y*js ...] of one kind slot; slice
+// `slice0 + blockIdx.y` of the partial buffer (or B/H directly).
+template <class R, bool SYM, int KIND>
 __global__ __launch_bounds__(256) void k_field(DevModel m, const double* __restrict__ M,
                                                const double* __restrict__ pts, int64_t np,
-                                               int64_t js, double* __restrict__ B,
+                                               int64_t js, int64_t slice0, double* __restrict__ B,
                                                double* __restrict__ H, double* __restrict__ part)
 {
     const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (p >= np) return;
-    const int64_t j0 = int64_t(blockIdx.y) * js;
-    const int64_t j1 = (j0 + js < m.n) ? (j0 + js) : m.n;
+    const int64_t kend = m.koff[KIND + 1];
+    const int64_t k0 = m.koff[KIND] + int64_t(blockIdx.y) * js;
+    const int64_t k1 = (k0 + js < kend) ? (k0 + js) : kend;
     const double x[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
     double b[3] = {0, 0, 0}, h[3] = {0, 0, 0};
-    field_add<R, SYM>(view(m), M, j0, j1, x, b, h);
+    field_add<R, SYM, KIND>(view(m), M, m.kidx, k0, k1, x, b, h);
     if (part) {
-        double* o = part + (int64_t(blockIdx.y) * np + p) * 6;
+        double* o = part + ((slice0 + int64_t(blockIdx.y)) * np + p) * 6;
         for (int k = 0; k < 3; k++) {
             o[k] = b[k];
             o[3 + k] = h[k];
@@ -437,54 +459,85 @@ hipError_t launch_assemble(const DevModel& m, void* Q, int elem_bytes, int64_t l
                            hipStream_t s)
 {
     if (m.n == 0) return hipSuccess;
-    const int64_t gx = (m.n + 63) / 64;
-    // enough workgroups to fill 256 CUs several times over
-    int64_t gy_target = std::max<int64_t>(1, (8192 + gx - 1) / gx);
-    int64_t jt = std::max<int64_t>(4, (m.n + gy_target - 1) / gy_target);
-    jt = std::min<int64_t>(jt, 256);
-    const int64_t gy = (m.n + jt - 1) / jt;
-    const dim3 grid{unsigned(gx), unsigned(gy), 1u};
     if (elem_bytes == 4) {
         float* q = static_cast<float*>(Q);
-        if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, grid, jt, s);
-        if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, grid, jt, s);
-        return asm_dispatch<true, true>(m, q, ld, qdiag, grid, jt, s);
+        if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, s);
+        if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, s);
+        return asm_dispatch<true, true>(m, q, ld, qdiag, s);
     }
     double* q = static_cast<double*>(Q);
-    if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, grid, jt, s);
-    if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, grid, jt, s);
-    return asm_dispatch<true, true>(m, q, ld, qdiag, grid, jt, s);
+    if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, s);
+    if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, s);
+    return asm_dispatch<true, true>(m, q, ld, qdiag, s);
 }
 
-int64_t field_splits(int64_t np, int64_t n)
+namespace {
+// element splits for one kind slot: enough workgroups to fill the chip
+int64_t slot_splits(int64_t np, int64_t cnt)
 {
+    if (cnt == 0) return 0;
     const int64_t blocks = (np + 255) / 256;
-    if (blocks >= 2048 || n < 64) return 1;
-    int64_t s = (2048 + blocks - 1) / blocks;
-    s = std::min<int64_t>(s, std::max<int64_t>(1, n / 16));
-    return std::max<int64_t>(1, s);
+    if (blocks >= 2048 || cnt < 64) return 1;
+    int64_t sp = (2048 + blocks - 1) / blocks;
+    sp = std::min<int64_t>(sp, std::max<int64_t>(1, cnt / 16));
+    return std::max<int64_t>(1, sp);
+}
+
+template <class R, bool SYM, int KIND>
+void field_slot(const DevModel& m, const double* M, const double* pts, int64_t np, int64_t splits,
+                int64_t slice0, double* B, double* H, double* part, hipStream_t s)
+{
+    const int64_t cnt = m.koff[KIND + 1] - m.koff[KIND];
+    const int64_t js = (cnt + splits - 1) / splits;
+    const dim3 grid{unsigned((np + 255) / 256), unsigned(splits), 1u};
+    hipLaunchKernelGGL((k_field<R, SYM, KIND>), grid, dim3(256), 0, s, m, M, pts, np, js, slice0, B,
+                       H, part);
+}
+
+template <class R, bool SYM>
+hipError_t field_all(const DevModel& m, const double* M, const double* pts, int64_t np, double* B,
+                     double* H, double* work, hipStream_t s)
+{
+    int64_t sp[3], total = 0;
+    for (int slot = 0; slot < 3; slot++) {
+        sp[slot] = slot_splits(np, m.koff[slot + 1] - m.koff[slot]);
+        total += sp[slot];
+    }
+    double* part = (total > 1) ? work : nullptr;
+    int64_t slice0 = 0;
+    if (sp[0]) field_slot<R, SYM, 0>(m, M, pts, np, sp[0], slice0, B, H, part, s);
+    slice0 += sp[0];
+    if (sp[1]) field_slot<R, SYM, 1>(m, M, pts, np, sp[1], slice0, B, H, part, s);
+    slice0 += sp[1];
+    if (sp[2]) field_slot<R, SYM, 2>(m, M, pts, np, sp[2], slice0, B, H, part, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || total <= 1) return e;
+    hipLaunchKernelGGL(k_field_reduce, dim3(unsigned((np + 255) / 256)), dim3(256), 0, s, np, total,
+                       part, B, H);
+    return hipGetLastError();
+}
+}  // namespace
+
+int64_t field_slices(const DevModel& m, int64_t np)
+{
+    int64_t total = 0;
+    for (int slot = 0; slot < 3; slot++) total += slot_splits(np, m.koff[slot + 1] - m.koff[slot]);
+    return std::max<int64_t>(1, total);
 }
 
 hipError_t launch_field(const DevModel& m, const double* M, const double* pts, int64_t np,
-                        double* B, double* H, double* work, int64_t splits, bool fp32,
-                        hipStream_t s)
+                        double* B, double* H, double* work, bool fp32, hipStream_t s)
 {
     if (np == 0) return hipSuccess;
-    const int64_t gx = (np + 255) / 256;
-    const int64_t js = (m.n + splits - 1) / splits;
-    const dim3 grid{unsigned(gx), unsigned(splits), 1u};
-    double* part = (splits > 1) ? work : nullptr;
-    if (fp32) {
-        if (m.sym) hipLaunchKernelGGL((k_field<float, true>), grid, dim3(256), 0, s, m, M, pts, np, js, B, H, part);
-        else hipLaunchKernelGGL((k_field<float, false>), grid, dim3(256), 0, s, m, M, pts, np, js, B, H, part);
-    } else {
-        if (m.sym) hipLaunchKernelGGL((k_field<double, true>), grid, dim3(256), 0, s, m, M, pts, np, js, B, H, part);
-        else hipLaunchKernelGGL((k_field<double, false>), grid, dim3(256), 0, s, m, M, pts, np, js, B, H, part);
+    if (m.n == 0) {
+        if (B) (void)hipMemsetAsync(B, 0, size_t(3 * np) * sizeof(double), s);
+        if (H) (void)hipMemsetAsync(H, 0, size_t(3 * np) * sizeof(double), s);
+        return hipGetLastError();
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || splits == 1) return e;
-    hipLaunchKernelGGL(k_field_reduce, dim3(unsigned(gx)), dim3(256), 0, s, np, splits, part, B, H);
-    return hipGetLastError();
+    if (fp32) return m.sym ? field_all<float, true>(m, M, pts, np, B, H, work, s)
+                           : field_all<float, false>(m, M, pts, np, B, H, work, s);
+    return m.sym ? field_all<double, true>(m, M, pts, np, B, H, work, s)
+                 : field_all<double, false>(m, M, pts, np, B, H, work, s);
 }
 
 hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t ld, const double* x,

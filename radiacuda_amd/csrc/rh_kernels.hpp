@@ -16,6 +16,8 @@ struct DevModel {
     const double* copies = nullptr;
     const double* colloc = nullptr;  // [3n] lab
     const double* rowt = nullptr;    // [9n]
+    const int32_t* kidx = nullptr;   // element indices grouped by kind slot
+    int64_t koff[4] = {0, 0, 0, 0};  // slot ranges in kidx (block, tet, general)
     bool sym = false;
     bool rowt_identity = true;
 };
@@ -26,12 +28,12 @@ struct DevModel {
 hipError_t launch_assemble(const DevModel& m, void* Q, int elem_bytes, int64_t ld, double* qdiag,
                            hipStream_t s);
 
-// Number of element splits the field kernel uses for np points (>= 1).
-int64_t field_splits(int64_t np, int64_t n);
-// B, H (either may be null) at np points; work holds 6*np*splits doubles when splits > 1.
+// Number of partial-sum slices the field kernels use for np points (>= 1);
+// the work buffer needs 6 * np * slices doubles when slices > 1.
+int64_t field_slices(const DevModel& m, int64_t np);
+// B, H (either may be null) at np points.
 hipError_t launch_field(const DevModel& m, const double* M, const double* pts, int64_t np,
-                        double* B, double* H, double* work, int64_t splits, bool fp32,
-                        hipStream_t s);
+                        double* B, double* H, double* work, bool fp32, hipStream_t s);
 
 // y = Q x (+ add), Q: nrows x ld (ld multiple of 256, padded columns zero,
 // x padded with zeros to ld).

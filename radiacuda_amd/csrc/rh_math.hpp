@@ -50,7 +50,8 @@ enum ElemKind : int32_t { EK_REC = 0, EK_TET = 1, EK_GEN = 3 };
 // Record sizes (doubles).
 constexpr int kRecSize = 6;             // centre(3), half-dims(3)
 constexpr int kTetEdge = 10;            // t(3), len, W sym (xx yy zz xy xz yz)
-constexpr int kTetSize = 12 + 6 * kTetEdge + 12;  // verts, edges, face normals = 84
+constexpr int kTetFaceRec = 4;         // n(3), 2*area
+constexpr int kTetSize = 12 + 6 * kTetEdge + 4 * kTetFaceRec;  // verts, edges, faces = 88
 constexpr int kGenHdr = 4;              // nv, ne, nf, nfv
 constexpr int kGenEdge = 12;            // a, b, t(3), len, W sym(6)
 constexpr int kGenFace = 5;             // n(3), fv begin, fv count
@@ -78,6 +79,8 @@ RH_HD V3<R> cross(const V3<R>& a, const V3<R>& b)
 
 RH_HD double rh_log(double x) { return log(x); }
 RH_HD float rh_log(float x) { return logf(x); }
+RH_HD double rh_log1p(double x) { return log1p(x); }
+RH_HD float rh_log1p(float x) { return log1pf(x); }
 RH_HD double rh_sqrt(double x) { return sqrt(x); }
 RH_HD float rh_sqrt(float x) { return sqrtf(x); }
 RH_HD double rh_atan2(double y, double x) { return atan2(y, x); }
@@ -106,6 +109,28 @@ RH_HD void edge_ratio(const V3<R>& ra, R Ra, const V3<R>& rb, R Rb, const V3<R>&
     bool good = sm >= R(0);
     num = (Rn + sn) * (good ? R(1) : (Rm - sm));
     den = good ? (Rm + sm) : d2;
+}
+
+// The same edge integral as ln(1 + delta) with delta >= 0 formed without
+// cancellation: num - den = len * (1 + |s_a + s_b| / (R_a + R_b)) in either
+// orientation, and 1/den = (R_m - s_m)/d^2 when s_m < 0.  Used by the FP32
+// evaluation, where a ratio near 1 would otherwise carry eps32 absolute error.
+template <class R>
+RH_HD R edge_log1p(const V3<R>& ra, R Ra, const V3<R>& rb, R Rb, const V3<R>& t, R len)
+{
+    R sa = dot(ra, t), sb = dot(rb, t);
+    R ssum = sa + sb;
+    bool pos = ssum >= R(0);
+    R Rm = pos ? Ra : Rb, sm = pos ? sa : -sb;
+    R g = len * (R(1) + fabs(ssum) / (Ra + Rb));
+    R delta;
+    if (sm >= R(0)) {
+        delta = g / (Rm + sm);
+    } else {
+        V3<R> c = cross(ra, t);
+        delta = g * (Rm - sm) / dot(c, c);
+    }
+    return rh_log1p(delta);
 }
 
 // Solid-angle factor of triangle (r1, r2, r3) as a complex number whose
@@ -257,9 +282,14 @@ RH_HD bool eval_tet(const double* __restrict__ rec, const double p[3], double q6
         const double* E = ed + kTetEdge * e;
         const int a = kTetEdgeA[e], b = kTetEdgeB[e];
         V3<R> t = v3(R(E[0]), R(E[1]), R(E[2]));
-        R num, den;
-        edge_ratio(r[a], Rv[a], r[b], Rv[b], t, num, den);
-        double L = double(rh_log(num / den));
+        double L;
+        if (sizeof(R) == 4) {
+            L = double(edge_log1p(r[a], Rv[a], r[b], Rv[b], t, R(E[3])));
+        } else {
+            R num, den;
+            edge_ratio(r[a], Rv[a], r[b], Rv[b], t, num, den);
+            L = double(rh_log(num / den));
+        }
 #pragma unroll
         for (int k = 0; k < 6; k++) acc[k] += L * E[4 + k];
     }
@@ -268,10 +298,16 @@ RH_HD bool eval_tet(const double* __restrict__ rec, const double p[3], double q6
 #pragma unroll
     for (int f = 0; f < 4; f++) {
         const int a = kTetFace[f][0], b = kTetFace[f][1], c = kTetFace[f][2];
-        R re, im;
-        tri_factor(r[a], Rv[a], r[b], Rv[b], r[c], Rv[c], re, im);
-        double om = 2.0 * double(rh_atan2(im, re));
-        const double nx = fn[3 * f], ny = fn[3 * f + 1], nz = fn[3 * f + 2];
+        const double* F = fn + kTetFaceRec * f;
+        const V3<R> n = v3(R(F[0]), R(F[1]), R(F[2]));
+        // r_a . n is the signed distance of the face plane from the point; the
+        // triple product r_a.(r_b x r_c) equals 2*area*(r_a . n) exactly
+        const R hd = dot(r[a], n);
+        const R im = R(F[3]) * hd;
+        const R re = Rv[a] * Rv[b] * Rv[c] + dot(r[a], r[b]) * Rv[c] + dot(r[a], r[c]) * Rv[b] +
+                     dot(r[b], r[c]) * Rv[a];
+        const double om = 2.0 * double(rh_atan2(im, re));
+        const double nx = F[0], ny = F[1], nz = F[2];
         acc[0] -= om * nx * nx;
         acc[1] -= om * ny * ny;
         acc[2] -= om * nz * nz;
@@ -279,8 +315,7 @@ RH_HD bool eval_tet(const double* __restrict__ rec, const double p[3], double q6
         acc[4] -= om * nx * nz;
         acc[5] -= om * ny * nz;
         // strictly on the inner side of every face plane
-        double h = (rec[3 * a] - p[0]) * nx + (rec[3 * a + 1] - p[1]) * ny + (rec[3 * a + 2] - p[2]) * nz;
-        inside = inside && (h > 0.0);
+        inside = inside && (hd > R(0));
     }
 #pragma unroll
     for (int k = 0; k < 6; k++) q6[k] = kInv4Pi * acc[k];
@@ -303,9 +338,15 @@ RH_HD bool eval_gen(const double* __restrict__ rec, const double p[3], double q6
         const int a = int(E[0]), b = int(E[1]);
         V3<R> ra = rel<R>(vt + 3 * a, p), rb = rel<R>(vt + 3 * b, p);
         R Ra = rh_sqrt(dot(ra, ra)), Rb = rh_sqrt(dot(rb, rb));
-        R num, den;
-        edge_ratio(ra, Ra, rb, Rb, v3(R(E[2]), R(E[3]), R(E[4])), num, den);
-        double L = double(rh_log(num / den));
+        const V3<R> t = v3(R(E[2]), R(E[3]), R(E[4]));
+        double L;
+        if (sizeof(R) == 4) {
+            L = double(edge_log1p(ra, Ra, rb, Rb, t, R(E[5])));
+        } else {
+            R num, den;
+            edge_ratio(ra, Ra, rb, Rb, t, num, den);
+            L = double(rh_log(num / den));
+        }
         for (int k = 0; k < 6; k++) acc[k] += L * E[6 + k];
     }
     bool inside = true;
