@@ -49,6 +49,11 @@ def main():
     ap.add_argument("--max-iter", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=48, help="rows/points in the CPU sample")
+    ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
+                    help="time one precision only (profiler runs); prints no JSON line")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="after the steps, copy a MIB-sized device buffer twice (a kernel of "
+                         "known HBM traffic, for calibrating profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -108,6 +113,8 @@ def main():
 
     results = {}
     for label, prec in (("fp64", N.RH_FP64), ("mixed", N.RH_MIXED)):
+        if args.only and label != args.only:
+            continue
         log("== %s: allocating and assembling the matrix" % label)
         Q = model.assemble(prec)
         opts = _abi.RelaxOpts(args.prec, args.max_iter, 0, 0.0, 9, 0)
@@ -148,7 +155,7 @@ def main():
             log("step %d done" % s)
         torch.cuda.synchronize()
         wall = time.perf_counter() - tstart
-        kt = {f: N.kernel_time(f) for f in ("asm", "field", "matvec")}
+        kt = {f: N.kernel_time(f) for f in ("asm", "field", "matvec", "refine")}
         N.set_timing(False)
         K = args.steps
         results[label] = dict(
@@ -156,6 +163,7 @@ def main():
             asm_ms=1e3 * phase["asm"] / K, hext_ms=1e3 * phase["hext"] / K,
             solve_s=phase["solve"] / K, fld_ms=1e3 * phase["fld"] / K,
             iters=res.iters, misfit=res.misfit, converged=bool(res.converged),
+            refined=res.refined,
             kernel_ms={f: kt[f][0] for f in kt}, kernel_counts={f: kt[f][1] for f in kt},
             B=d_B.cpu().numpy(), M=d_M.cpu().numpy())
         Q.close()
@@ -163,6 +171,16 @@ def main():
         log("%s: %.1f ms/step (asm %.2f ms, solve %.3f s in %d it, fld %.1f ms)"
             % (label, r["ms_per_step"], r["asm_ms"], r["solve_s"], r["iters"], r["fld_ms"]))
 
+    if args.calib_copy > 0:
+        nel = args.calib_copy * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d MiB read + %d MiB written, twice" % (args.calib_copy, args.calib_copy))
+    if args.only:
+        return 0
     r64, rmx = results["fp64"], results["mixed"]
     scale = np.abs(r64["B"]).max()
     err_mixed = float(np.abs(rmx["B"] - r64["B"]).max() / scale)
@@ -237,6 +255,8 @@ def main():
         "mixed": {"asm_qblocks_per_s": n * n / (rmx["asm_ms"] * 1e-3),
                   "asm_kernel_ms": rmx["kernel_ms"]["asm"], "solve_s": rmx["solve_s"],
                   "solve_iters": rmx["iters"], "solve_misfit": rmx["misfit"],
+                  "fp64_refinement_passes": rmx["refined"],
+                  "refine_field_ms": rmx["kernel_ms"]["refine"],
                   "matvec_kernel_ms": rmx["kernel_ms"]["matvec"],
                   "matvec_GBps": (4.0 * n3 * n3 / (rmx["kernel_ms"]["matvec"] * 1e-3) / 1e9
                                   if rmx["kernel_ms"]["matvec"] > 0 else None),

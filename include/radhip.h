@@ -118,7 +118,11 @@ typedef struct {
     int32_t keep_m;    /* 1: start from M_inout ('ZeroM->False'), 0: from M_r  */
     double omega0;     /* mixing factor of the iteration (0 -> 0.5)            */
     int32_t method;    /* 9 (or 0): Anderson-accelerated block Jacobi          */
-    int32_t reserved;  /* must be 0                                            */
+    int32_t refine;    /* RH_MIXED matrix only: FP64 refinement passes after   *
+                        * convergence (0 -> 1, -1 -> none); each pass takes the *
+                        * FP64 field at the collocation points, corrects the   *
+                        * external field by its defect against the stored       *
+                        * matrix and resumes the iteration                     */
 } rh_relax_opts;
 
 typedef struct {
@@ -128,6 +132,7 @@ typedef struct {
     int32_t iters;     /* iterations performed                      */
     int32_t matvecs;   /* matrix passes (each streams the matrix)   */
     int32_t converged; /* 1 if misfit <= prec                       */
+    int32_t refined;   /* FP64 refinement passes performed          */
 } rh_relax_result;
 
 /* Relaxes the magnetization to M = M_mat(H_ext + Q*M).  mats: [16n] host

@@ -181,12 +181,15 @@ class Matrix:
         check(lib().rh_matrix_export(self.h, row0, nrows, _p(out)))
         return out
 
-    def relax(self, mats, hext, M0, prec, max_iter, keep_m=False, omega0=0.0):
+    def relax(self, mats, hext, M0, prec, max_iter, keep_m=False, omega0=0.0, refine=0):
+        """refine: FP64 refinement passes of a mixed-precision matrix
+        (0 = library default of one pass, -1 = none)."""
         mats = _c(mats).reshape(-1)
         hext = _c(hext).reshape(-1)
         M = _c(M0).reshape(-1).copy()
         H = np.zeros_like(M)
-        opts = _abi.RelaxOpts(float(prec), int(max_iter), 1 if keep_m else 0, float(omega0), 9, 0)
+        opts = _abi.RelaxOpts(float(prec), int(max_iter), 1 if keep_m else 0, float(omega0), 9,
+                              int(refine))
         res = _abi.RelaxResult()
         check(lib().rh_relax(self.h, _p(mats), _p(hext), _p(M), _p(H), ctypes.byref(opts),
                              ctypes.byref(res)))

@@ -13,6 +13,7 @@
 #include "../../include/radhip.h"
 #include "rh_geom.hpp"
 #include "rh_kernels.hpp"
+#include "rh_math.hpp"
 
 namespace {
 
@@ -119,6 +120,13 @@ struct rh_model {
     double* d_colloc = nullptr;
     double* d_rowt = nullptr;
     int32_t* d_kidx = nullptr;
+    double* d_charged = nullptr;  // field-only tetrahedron records, rebuilt per field call
+    // adjoint geometry, uploaded on the first adjoint call
+    double* d_adj = nullptr;
+    int64_t* d_adj_off = nullptr;
+    int64_t* d_vert_off = nullptr;
+    int64_t* d_slices = nullptr;
+    int64_t n_slices = 0;
     // growable workspace for the field entry points
     double* d_work = nullptr;
     size_t work_cap = 0;
@@ -158,6 +166,11 @@ void free_model(rh_model* m)
     dfree(m->d_colloc);
     dfree(m->d_rowt);
     dfree(m->d_kidx);
+    dfree(m->d_charged);
+    dfree(m->d_adj);
+    dfree(m->d_adj_off);
+    dfree(m->d_vert_off);
+    dfree(m->d_slices);
     dfree(m->d_work);
     delete m;
 }
@@ -224,6 +237,8 @@ rh_status rh_model_create(const rh_model_desc* desc, int device, void* stream, r
     if (e == hipSuccess) e = upload(&m->d_colloc, P.colloc, s);
     if (e == hipSuccess) e = upload(&m->d_rowt, P.rowt, s);
     if (e == hipSuccess) e = upload(&m->d_kidx, P.kind_idx, s);
+    if (e == hipSuccess)
+        e = dalloc(&m->d_charged, size_t(rh::kChargedTet * (P.kind_off[2] - P.kind_off[1])));
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         std::string msg = std::string("model upload: ") + hipGetErrorString(e);
@@ -436,6 +451,171 @@ bool small_solve(int m, std::vector<double> G, std::vector<double> r, std::vecto
 
 }  // namespace
 
+static rh_status field_dev_impl(rh_model* m, const double* d_M, const double* d_pts, int64_t np,
+                                double* d_B, double* d_H, bool fp32, const char* family = "field");
+
+namespace {
+// Anderson-accelerated block-Jacobi iterations from the iterate in w.x until
+// misfit <= prec or max_iter; writes the final magnetization to d_M and the
+// element fields to d_H.
+// Transposed (adjoint) iteration of a linear-material system: the matrix pass
+// becomes y = offdiag(Q)^T K^T lam and the update the transposed local solve.
+struct AdjointPass {
+    double* mu = nullptr;    // [n3] K^T lam
+    double* tpart = nullptr; // [chunks * ld] transposed-pass partials
+    int64_t chunks = 0;
+};
+
+rh_status relax_run(rh_matrix* q, const double* d_mats, const double* d_hext, double* d_M,
+                    double* d_H, const rh_relax_opts* opts, RelaxWork& w, double* hbuf,
+                    int64_t nb, int64_t db, rh_relax_result* res,
+                    const AdjointPass* adj = nullptr, bool keep_if_converged = false)
+{
+    rh_model* m = q->m;
+    hipStream_t s = m->stream;
+    const int64_t n = m->host.n, n3 = q->n3;
+    const double prec = opts->prec;
+    const int max_iter = std::max(1, int(opts->max_iter));
+    double beta = (opts->omega0 > 0.0) ? opts->omega0 : 0.5;
+    std::vector<double> G(size_t(kHist * kHist), 0.0);
+    int hist = 0, head = 0;  // ring of the last `hist` differences, newest at head-1
+    double best = INFINITY;
+    Timer tm("matvec", s);
+    int it = 0;
+    double misfit = INFINITY, maxM = 0.0, maxH = 0.0;
+    while (true) {
+        it++;
+        tm.begin();
+        if (adj) {
+            RH_HIP(rh::launch_adj_scale(n, d_mats, nullptr, w.x, adj->mu, s));
+            RH_HIP(rh::launch_matvec_t(q->d_Q, q->elem_bytes, n3, q->ld, adj->mu, adj->tpart,
+                                       adj->chunks, w.y, s));
+        } else {
+            RH_HIP(rh::launch_matvec(q->d_Q, q->elem_bytes, n3, q->ld, w.x, nullptr, w.y, s));
+        }
+        tm.end();
+        if (adj)
+            RH_HIP(rh::launch_adj_update(n, d_mats, d_hext, q->d_qdiag, w.y, w.x, w.F, w.part, s));
+        else
+            RH_HIP(rh::launch_relax_update(n, d_mats, d_hext, q->d_qdiag, w.y, w.x, w.F, d_H, w.part, s));
+        // tentatively extend the history with the newest differences
+        if (it > 1) {
+            const double* v1[2] = {w.F, w.Fp};
+            const double c1[2] = {1.0, -1.0};
+            RH_HIP(rh::launch_lincomb(2, v1, c1, n3, w.dF[head], s));
+            const double* v2[2] = {w.x, w.xp};
+            RH_HIP(rh::launch_lincomb(2, v2, c1, n3, w.dX[head], s));
+            head = (head + 1) % kHist;
+            hist = std::min(hist + 1, kHist);
+        }
+        RH_HIP(hipMemcpyAsync(w.Fp, w.F, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        RH_HIP(hipMemcpyAsync(w.xp, w.x, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        int slot[kHist];
+        for (int k = 0; k < hist; k++) slot[k] = (head - hist + k + kHist) % kHist;
+        int nd = 0;
+        if (hist > 0) {
+            // dots: newest dF with every dF, and every dF with F
+            const double* a[17];
+            const double* b[17];
+            const int newest = slot[hist - 1];
+            for (int k = 0; k < hist; k++) {
+                a[nd] = w.dF[newest];
+                b[nd] = w.dF[slot[k]];
+                nd++;
+            }
+            for (int k = 0; k < hist; k++) {
+                a[nd] = w.dF[slot[k]];
+                b[nd] = w.F;
+                nd++;
+            }
+            RH_HIP(rh::launch_dots(nd, a, b, n3, w.dots, s));
+            RH_HIP(hipMemcpyAsync(hbuf + 3 * nb, w.dots, size_t(nd * db) * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+        }
+        RH_HIP(hipMemcpyAsync(hbuf, w.part, size_t(3 * nb) * sizeof(double), hipMemcpyDeviceToHost, s));
+        RH_HIP(hipStreamSynchronize(s));
+        double f2 = 0.0;
+        maxM = 1e-17;
+        maxH = 1e-17;
+        for (int64_t bb = 0; bb < nb; bb++) {
+            f2 += hbuf[3 * bb];
+            maxM = std::max(maxM, hbuf[3 * bb + 1]);
+            maxH = std::max(maxH, hbuf[3 * bb + 2]);
+        }
+        misfit = std::sqrt(f2 / double(n));
+        if (!std::isfinite(misfit)) return fail(RH_ERR_NOTCONV, "relaxation produced a non-finite residual");
+        if (misfit <= prec && it == 1 && keep_if_converged) break;  // d_M already holds x
+        if (misfit <= prec || it >= max_iter) {
+            // final magnetization: the last proposal x + F
+            const double* vv[2] = {w.x, w.F};
+            const double cc[2] = {1.0, 1.0};
+            RH_HIP(rh::launch_lincomb(2, vv, cc, n3, d_M, s));
+            break;
+        }
+        // safeguard: restart the history when the residual grows a lot
+        bool plain = (hist == 0);
+        if (misfit > 1e4 * best) {
+            hist = 0;
+            head = 0;
+            beta = std::max(0.05, 0.5 * beta);
+            plain = true;
+        }
+        best = std::min(best, misfit);
+        std::vector<double> gam;
+        if (!plain) {
+            std::vector<double> dv(size_t(nd), 0.0);
+            for (int d = 0; d < nd; d++)
+                for (int64_t bb = 0; bb < db; bb++) dv[size_t(d)] += hbuf[3 * nb + d * db + bb];
+            const int newest = slot[hist - 1];
+            for (int k = 0; k < hist; k++) {
+                G[size_t(newest * kHist + slot[k])] = dv[size_t(k)];
+                G[size_t(slot[k] * kHist + newest)] = dv[size_t(k)];
+            }
+            std::vector<double> Gm(static_cast<size_t>(hist) * hist);
+            std::vector<double> r(static_cast<size_t>(hist));
+            for (int i = 0; i < hist; i++) {
+                r[size_t(i)] = dv[size_t(hist + i)];
+                for (int j = 0; j < hist; j++) Gm[size_t(i * hist + j)] = G[size_t(slot[i] * kHist + slot[j])];
+            }
+            if (!small_solve(hist, Gm, r, gam)) {
+                hist = 0;
+                head = 0;
+                plain = true;
+            }
+        }
+        if (plain) {
+            const double* v[2] = {w.x, w.F};
+            const double c[2] = {1.0, beta};
+            RH_HIP(rh::launch_lincomb(2, v, c, n3, w.x, s));
+            continue;
+        }
+        // x <- x + beta F - sum_k gam_k (dX_k + beta dF_k)
+        const double* v[34];
+        double c[34];
+        int nv = 0;
+        v[nv] = w.x;
+        c[nv++] = 1.0;
+        v[nv] = w.F;
+        c[nv++] = beta;
+        for (int k = 0; k < hist; k++) {
+            v[nv] = w.dX[slot[k]];
+            c[nv++] = -gam[size_t(k)];
+            v[nv] = w.dF[slot[k]];
+            c[nv++] = -beta * gam[size_t(k)];
+        }
+        RH_HIP(rh::launch_lincomb(nv, v, c, n3, w.x, s));
+    }
+    RH_HIP(hipStreamSynchronize(s));
+    res->misfit = misfit;
+    res->max_m = maxM;
+    res->max_h = maxH;
+    res->iters += it;
+    res->matvecs += it;
+    res->converged = misfit <= prec ? 1 : 0;
+    return RH_OK;
+}
+}  // namespace
+
 rh_status rh_relax_dev(rh_matrix* q, const double* d_mats, const double* d_hext, double* d_M,
                        double* d_H, const rh_relax_opts* opts, rh_relax_result* res)
 {
@@ -473,134 +653,48 @@ rh_status rh_relax_dev(rh_matrix* q, const double* d_mats, const double* d_hext,
             RH_HIP(hipMemcpy2DAsync(w.x + k, 3 * sizeof(double), d_mats + 13 + k, 16 * sizeof(double),
                                     sizeof(double), size_t(n), hipMemcpyDeviceToDevice, s));
     }
-    const double prec = opts->prec;
-    const int max_iter = std::max(1, int(opts->max_iter));
-    double beta = (opts->omega0 > 0.0) ? opts->omega0 : 0.5;
-    std::vector<double> hp(size_t(3 * nb)), hd(size_t(17 * db));
-    std::vector<double> G(size_t(kHist * kHist), 0.0);
-    int hist = 0, head = 0;  // ring of the last `hist` differences, newest at head-1
-    double best = INFINITY;
-    Timer tm("matvec", s);
-    int it = 0;
-    double misfit = INFINITY, maxM = 0.0, maxH = 0.0;
-    bool done = false;
-    while (!done) {
-        it++;
-        tm.begin();
+    // one pinned read-back per iteration: residual partials, then dot partials
+    double* hbuf = nullptr;
+    RH_HIP(hipHostMalloc(reinterpret_cast<void**>(&hbuf), size_t(3 * nb + 17 * db) * sizeof(double)));
+    struct PinnedFree {
+        double* p;
+        ~PinnedFree() { (void)hipHostFree(p); }
+    } pinned_guard{hbuf};
+    rh_status st = relax_run(q, d_mats, d_hext, d_M, d_H, opts, w, hbuf, nb, db, res);
+    if (st != RH_OK) return st;
+    // FP64 refinement of a solve on the FP32-stored matrix: the defect between
+    // the FP64 field at the collocation points and the stored-matrix field
+    // becomes a correction to the external field; the iteration resumes from
+    // the current magnetization unless that already meets prec on the
+    // corrected system.
+    const int passes = (q->prec == RH_MIXED) ? (opts->refine == 0 ? 1 : std::max(0, int(opts->refine))) : 0;
+    if (passes == 0) return RH_OK;
+    double *d_H64 = nullptr, *d_hx = nullptr;
+    struct Free2 {
+        double** a;
+        double** b;
+        ~Free2()
+        {
+            dfree(*a);
+            dfree(*b);
+        }
+    } free2{&d_H64, &d_hx};
+    RH_HIP(dalloc(&d_H64, size_t(n3)));
+    RH_HIP(dalloc(&d_hx, size_t(n3)));
+    const rh_relax_opts o2 = *opts;
+    for (int pss = 0; pss < passes; pss++) {
+        st = field_dev_impl(m, d_M, m->d_colloc, n, nullptr, d_H64, false, "refine");
+        if (st != RH_OK) return st;
+        RH_HIP(hipMemcpyAsync(w.x, d_M, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
         RH_HIP(rh::launch_matvec(q->d_Q, q->elem_bytes, n3, q->ld, w.x, nullptr, w.y, s));
-        tm.end();
-        RH_HIP(rh::launch_relax_update(n, d_mats, d_hext, q->d_qdiag, w.y, w.x, w.F, d_H, w.part, s));
-        RH_HIP(hipMemcpyAsync(hp.data(), w.part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-        RH_HIP(hipStreamSynchronize(s));
-        double f2 = 0.0;
-        maxM = 1e-17;
-        maxH = 1e-17;
-        for (int64_t b = 0; b < nb; b++) {
-            f2 += hp[size_t(3 * b)];
-            maxM = std::max(maxM, hp[size_t(3 * b + 1)]);
-            maxH = std::max(maxH, hp[size_t(3 * b + 2)]);
-        }
-        misfit = std::sqrt(f2 / double(n));
-        if (!std::isfinite(misfit)) return fail(RH_ERR_NOTCONV, "relaxation produced a non-finite residual");
-        if (misfit <= prec || it >= max_iter) {
-            // final magnetization: the last proposal x + F
-            const double* vv[2] = {w.x, w.F};
-            const double cc[2] = {1.0, 1.0};
-            RH_HIP(rh::launch_lincomb(2, vv, cc, n3, d_M, s));
-            done = true;
-            break;
-        }
-        // safeguard: restart the history when the residual grows a lot
-        if (misfit > 1e4 * best) {
-            hist = 0;
-            head = 0;
-            beta = std::max(0.05, 0.5 * beta);
-        }
-        best = std::min(best, misfit);
-        if (it > 1) {
-            // new difference column at slot `head`
-            const double* v1[2] = {w.F, w.Fp};
-            const double c1[2] = {1.0, -1.0};
-            RH_HIP(rh::launch_lincomb(2, v1, c1, n3, w.dF[head], s));
-            const double* v2[2] = {w.x, w.xp};
-            RH_HIP(rh::launch_lincomb(2, v2, c1, n3, w.dX[head], s));
-            head = (head + 1) % kHist;
-            hist = std::min(hist + 1, kHist);
-        }
-        RH_HIP(hipMemcpyAsync(w.Fp, w.F, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
-        RH_HIP(hipMemcpyAsync(w.xp, w.x, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
-        if (hist == 0) {
-            const double* v[2] = {w.x, w.F};
-            const double c[2] = {1.0, beta};
-            RH_HIP(rh::launch_lincomb(2, v, c, n3, w.x, s));
-            continue;
-        }
-        // slots in age order: oldest..newest
-        int slot[kHist];
-        for (int k = 0; k < hist; k++) slot[k] = (head - hist + k + kHist) % kHist;
-        const int newest = slot[hist - 1];
-        // dots: newest dF with every dF, and every dF with F
-        const double* a[17];
-        const double* b[17];
-        int nd = 0;
-        for (int k = 0; k < hist; k++) {
-            a[nd] = w.dF[newest];
-            b[nd] = w.dF[slot[k]];
-            nd++;
-        }
-        for (int k = 0; k < hist; k++) {
-            a[nd] = w.dF[slot[k]];
-            b[nd] = w.F;
-            nd++;
-        }
-        RH_HIP(rh::launch_dots(nd, a, b, n3, w.dots, s));
-        RH_HIP(hipMemcpyAsync(hd.data(), w.dots, size_t(nd * db) * sizeof(double), hipMemcpyDeviceToHost, s));
-        RH_HIP(hipStreamSynchronize(s));
-        std::vector<double> dv(size_t(nd), 0.0);
-        for (int d = 0; d < nd; d++)
-            for (int64_t bb = 0; bb < db; bb++) dv[size_t(d)] += hd[size_t(d * db + bb)];
-        for (int k = 0; k < hist; k++) {
-            G[size_t(newest * kHist + slot[k])] = dv[size_t(k)];
-            G[size_t(slot[k] * kHist + newest)] = dv[size_t(k)];
-        }
-        std::vector<double> Gm(static_cast<size_t>(hist) * hist);
-        std::vector<double> r(static_cast<size_t>(hist));
-        std::vector<double> gam;
-        for (int i = 0; i < hist; i++) {
-            r[size_t(i)] = dv[size_t(hist + i)];
-            for (int j = 0; j < hist; j++) Gm[size_t(i * hist + j)] = G[size_t(slot[i] * kHist + slot[j])];
-        }
-        if (!small_solve(hist, Gm, r, gam)) {
-            hist = 0;
-            head = 0;
-            const double* v[2] = {w.x, w.F};
-            const double c[2] = {1.0, beta};
-            RH_HIP(rh::launch_lincomb(2, v, c, n3, w.x, s));
-            continue;
-        }
-        // x <- x + beta F - sum_k gam_k (dX_k + beta dF_k)
-        const double* v[34];
-        double c[34];
-        int nv = 0;
-        v[nv] = w.x;
-        c[nv++] = 1.0;
-        v[nv] = w.F;
-        c[nv++] = beta;
-        for (int k = 0; k < hist; k++) {
-            v[nv] = w.dX[slot[k]];
-            c[nv++] = -gam[size_t(k)];
-            v[nv] = w.dF[slot[k]];
-            c[nv++] = -beta * gam[size_t(k)];
-        }
-        RH_HIP(rh::launch_lincomb(nv, v, c, n3, w.x, s));
+        RH_HIP(rh::launch_refine_defect(n, d_H64, m->host.rowt_identity ? nullptr : m->d_rowt, w.y,
+                                        q->d_qdiag, d_M, d_hext, d_hx, s));
+        res->matvecs += 1;
+        // a magnetization that already meets prec on the corrected system is kept
+        st = relax_run(q, d_mats, d_hx, d_M, d_H, &o2, w, hbuf, nb, db, res, nullptr, true);
+        if (st != RH_OK) return st;
+        res->refined += 1;
     }
-    RH_HIP(hipStreamSynchronize(s));
-    res->misfit = misfit;
-    res->max_m = maxM;
-    res->max_h = maxH;
-    res->iters = it;
-    res->matvecs = it;
-    res->converged = misfit <= prec ? 1 : 0;
     return RH_OK;
 }
 
@@ -643,7 +737,7 @@ rh_status rh_relax(rh_matrix* q, const double* mats, const double* hext, double*
 
 // ---------------------------------------------------------------- field
 static rh_status field_dev_impl(rh_model* m, const double* d_M, const double* d_pts, int64_t np,
-                                double* d_B, double* d_H, bool fp32)
+                                double* d_B, double* d_H, bool fp32, const char* family)
 {
     const int64_t slices = rh::field_slices(m->dev, np);
     double* work = nullptr;
@@ -651,9 +745,9 @@ static rh_status field_dev_impl(rh_model* m, const double* d_M, const double* d_
         RH_HIP(grow(m, size_t(6 * np * slices)));
         work = m->d_work;
     }
-    Timer t("field", m->stream);
+    Timer t(family, m->stream);
     t.begin();
-    RH_HIP(rh::launch_field(m->dev, d_M, d_pts, np, d_B, d_H, work, fp32, m->stream));
+    RH_HIP(rh::launch_field(m->dev, d_M, d_pts, np, d_B, d_H, work, m->d_charged, fp32, m->stream));
     t.end();
     return RH_OK;
 }
@@ -711,6 +805,176 @@ rh_status rh_field(rh_model* m, const double* M, const double* pts, int64_t np, 
 }
 
 // ---------------------------------------------------------------- currents
+// ---------------------------------------------------------------- adjoint
+rh_status rh_model_vertex_count(const rh_model* m, int64_t* nvert)
+{
+    if (!m || !nvert) return fail(RH_ERR_ARG, "null argument");
+    *nvert = m->host.vert_off.empty() ? 0 : m->host.vert_off.back();
+    return RH_OK;
+}
+
+static rh_status adj_geometry(rh_model* m)
+{
+    if (m->d_adj) return RH_OK;
+    const rh::PackedModel& P = m->host;
+    std::string err;
+    std::vector<int64_t> sl = rh::adj_geom_slices(P.vert_off, &err);
+    if (sl.empty()) return fail(RH_ERR_UNSUPPORTED, err);
+    hipStream_t s = m->stream;
+    RH_HIP(upload(&m->d_adj, P.adj, s));
+    RH_HIP(upload(&m->d_adj_off, P.adj_off, s));
+    RH_HIP(upload(&m->d_vert_off, P.vert_off, s));
+    RH_HIP(upload(&m->d_slices, sl, s));
+    m->n_slices = int64_t(sl.size()) - 1;
+    RH_HIP(hipStreamSynchronize(s));
+    return RH_OK;
+}
+
+rh_status rh_field_adjoint_dev(rh_model* m, const double* d_M, const double* d_pts, int64_t np,
+                               const double* d_gB, int32_t field, double* d_gM, double* d_gV,
+                               double* d_gX)
+{
+    if (!m || np < 0 || (np > 0 && (!d_pts || !d_gB))) return fail(RH_ERR_ARG, "bad argument");
+    if ((d_gV || d_gX) && !d_M) return fail(RH_ERR_ARG, "vertex or point gradients need M");
+    if (field != 0 && field != 1) return fail(RH_ERR_ARG, "field must be 0 (B) or 1 (H)");
+    RH_HIP(hipSetDevice(m->device));
+    hipStream_t s = m->stream;
+    const rh::PackedModel& P = m->host;
+    const int64_t n = P.n;
+    const int64_t nvert = P.vert_off.empty() ? 0 : P.vert_off.back();
+    if ((d_gV || d_gX) && P.kind_off[1] > 0)
+        return fail(RH_ERR_UNSUPPORTED, "vertex and point gradients need a model of polyhedra only");
+    if (np == 0 || n == 0) {
+        if (d_gM) RH_HIP(hipMemsetAsync(d_gM, 0, size_t(3 * n) * sizeof(double), s));
+        if (d_gV) RH_HIP(hipMemsetAsync(d_gV, 0, size_t(3 * nvert) * sizeof(double), s));
+        if (d_gX) RH_HIP(hipMemsetAsync(d_gX, 0, size_t(3 * np) * sizeof(double), s));
+        RH_HIP(hipStreamSynchronize(s));
+        return RH_OK;
+    }
+    Timer t("adjoint", s);
+    if (d_gM) {
+        const int64_t groups = rh::adj_point_groups(np, rh::adj_m_slices(n));
+        RH_HIP(grow(m, size_t(groups * 3 * n)));
+        t.begin();
+        RH_HIP(rh::launch_adj_m(m->dev, d_pts, np, d_gB, field == 0, m->d_work, groups, d_gM, s));
+        t.end();
+    }
+    if (d_gV || d_gX) {
+        rh_status st = adj_geometry(m);
+        if (st != RH_OK) return st;
+        const int64_t groups = rh::adj_point_groups(np, m->n_slices);
+        const size_t nv_part = size_t(groups * 3 * nvert);
+        const size_t nx_part = d_gX ? size_t(m->n_slices * 3 * np) : 0;
+        RH_HIP(grow(m, nv_part + nx_part));
+        t.begin();
+        RH_HIP(rh::launch_adj_geom(m->dev, m->d_adj, m->d_adj_off, m->d_vert_off, m->d_slices,
+                                   m->n_slices, nvert, d_M, d_pts, np, d_gB, m->d_work,
+                                   m->d_work + nv_part, groups, d_gV, d_gX, s));
+        t.end();
+    }
+    RH_HIP(hipStreamSynchronize(s));
+    return RH_OK;
+}
+
+rh_status rh_field_adjoint(rh_model* m, const double* M, const double* pts, int64_t np,
+                           const double* gB, int32_t field, double* gM, double* gV, double* gX)
+{
+    if (!m || np < 0 || (np > 0 && (!pts || !gB))) return fail(RH_ERR_ARG, "bad argument");
+    RH_HIP(hipSetDevice(m->device));
+    hipStream_t s = m->stream;
+    const int64_t n = m->host.n;
+    const int64_t nvert = m->host.vert_off.empty() ? 0 : m->host.vert_off.back();
+    std::vector<double*> bufs;
+    auto cleanup = [&]() {
+        for (double* b : bufs) dfree(b);
+    };
+    // device copy of a host array (h may be null: allocation only)
+    auto dev = [&](const double* h, size_t cnt, double** d) -> hipError_t {
+        hipError_t e = dalloc(d, std::max<size_t>(cnt, 1));
+        if (e != hipSuccess) return e;
+        bufs.push_back(*d);
+        if (h && cnt) e = hipMemcpyAsync(*d, h, cnt * sizeof(double), hipMemcpyHostToDevice, s);
+        return e;
+    };
+    double *dM = nullptr, *dP = nullptr, *dG = nullptr, *dgM = nullptr, *dgV = nullptr, *dgX = nullptr;
+    hipError_t e = hipSuccess;
+    if (M) e = dev(M, size_t(3 * n), &dM);
+    if (e == hipSuccess) e = dev(pts, size_t(3 * np), &dP);
+    if (e == hipSuccess) e = dev(gB, size_t(3 * np), &dG);
+    if (e == hipSuccess && gM) e = dev(nullptr, size_t(3 * n), &dgM);
+    if (e == hipSuccess && gV) e = dev(nullptr, size_t(3 * nvert), &dgV);
+    if (e == hipSuccess && gX) e = dev(nullptr, size_t(3 * np), &dgX);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_HIP,
+                    std::string("adjoint upload: ") + hipGetErrorString(e));
+    }
+    rh_status st = rh_field_adjoint_dev(m, dM, dP, np, dG, field, dgM, dgV, dgX);
+    if (st == RH_OK) {
+        if (gM) e = hipMemcpy(gM, dgM, size_t(3 * n) * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && gV)
+            e = hipMemcpy(gV, dgV, size_t(3 * nvert) * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && gX)
+            e = hipMemcpy(gX, dgX, size_t(3 * np) * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = fail(RH_ERR_HIP, std::string("adjoint download: ") + hipGetErrorString(e));
+    }
+    cleanup();
+    return st;
+}
+
+rh_status rh_relax_adjoint_dev(rh_matrix* q, const double* d_mats, const double* d_rhs, double* d_lam,
+                               const rh_relax_opts* opts, rh_relax_result* res)
+{
+    if (!q || !d_mats || !d_rhs || !d_lam || !opts || !res) return fail(RH_ERR_ARG, "null argument");
+    rh_model* m = q->m;
+    RH_HIP(hipSetDevice(m->device));
+    hipStream_t s = m->stream;
+    const int64_t n = m->host.n, n3 = q->n3;
+    std::memset(res, 0, sizeof(*res));
+    if (n == 0) {
+        res->converged = 1;
+        return RH_OK;
+    }
+    RelaxWork w;
+    RH_HIP(dalloc(&w.x, size_t(q->ld)));
+    RH_HIP(dalloc(&w.y, size_t(n3)));
+    RH_HIP(dalloc(&w.F, size_t(n3)));
+    RH_HIP(dalloc(&w.Fp, size_t(n3)));
+    RH_HIP(dalloc(&w.xp, size_t(n3)));
+    for (int k = 0; k < kHist; k++) {
+        RH_HIP(dalloc(&w.dX[k], size_t(n3)));
+        RH_HIP(dalloc(&w.dF[k], size_t(n3)));
+    }
+    const int64_t nb = rh::relax_blocks(n);
+    const int64_t db = rh::dot_blocks(n3);
+    RH_HIP(dalloc(&w.part, size_t(3 * nb)));
+    RH_HIP(dalloc(&w.dots, size_t(17 * db)));
+    AdjointPass ap;
+    ap.chunks = rh::matvec_t_chunks(n3, q->ld);
+    struct Free2 {
+        double** a;
+        double** b;
+        ~Free2()
+        {
+            dfree(*a);
+            dfree(*b);
+        }
+    } free2{&ap.mu, &ap.tpart};
+    RH_HIP(dalloc(&ap.mu, size_t(n3)));
+    RH_HIP(dalloc(&ap.tpart, size_t(ap.chunks * q->ld)));
+    RH_HIP(hipMemsetAsync(w.x, 0, size_t(q->ld) * sizeof(double), s));
+    if (opts->keep_m)
+        RH_HIP(hipMemcpyAsync(w.x, d_lam, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    double* hbuf = nullptr;
+    RH_HIP(hipHostMalloc(reinterpret_cast<void**>(&hbuf), size_t(3 * nb + 17 * db) * sizeof(double)));
+    struct PinnedFree {
+        double* p;
+        ~PinnedFree() { (void)hipHostFree(p); }
+    } pinned_guard{hbuf};
+    return relax_run(q, d_mats, d_rhs, d_lam, w.y /* unused field output */, opts, w, hbuf, nb, db,
+                     res, &ap);
+}
+
 static rh_status pack_currents(int64_t nsrc, const double* tm, const double* tv, const double* ts,
                                std::vector<double>& cp)
 {

@@ -17,6 +17,10 @@ struct ModelView {
     const double* data;
     const int64_t* copy_off;
     const double* copies;  // kCopySize (24) doubles each: Minv, V, F, pad
+    // optional field-only tetrahedron records for the current magnetization,
+    // kChargedTet doubles each, indexed by position in the tetrahedron list
+    const double* charged = nullptr;
+    int64_t charged_base = 0;
 };
 
 RH_HD void apply_inv(const double* c, const double x[3], double o[3])
@@ -82,32 +86,46 @@ RH_HD void field_add(const ModelView& mv, const double* M, const int32_t* idx, i
         const int kind = (KIND < 0) ? mv.kind[j] : 0;
         const double* rec = mv.data + mv.off[j];
         const double m[3] = {M[3 * j], M[3 * j + 1], M[3 * j + 2]};
+        const bool charged = (KIND == 1) && mv.charged != nullptr;
+        const double* crec = charged ? mv.charged + kChargedTet * (k - mv.charged_base) : nullptr;
         if (!SYM) {
-            double q6[6];
-            bool in = eval_k<R, KIND>(kind, rec, x, q6);
-            double h0 = q6[0] * m[0] + q6[3] * m[1] + q6[4] * m[2];
-            double h1 = q6[3] * m[0] + q6[1] * m[1] + q6[5] * m[2];
-            double h2 = q6[4] * m[0] + q6[5] * m[1] + q6[2] * m[2];
-            H[0] += h0; H[1] += h1; H[2] += h2;
-            B[0] += h0 + (in ? m[0] : 0.0);
-            B[1] += h1 + (in ? m[1] : 0.0);
-            B[2] += h2 + (in ? m[2] : 0.0);
+            double hl[3];
+            bool in;
+            if (charged) {
+                in = field_tet<R>(crec, x, hl);
+            } else {
+                double q6[6];
+                in = eval_k<R, KIND>(kind, rec, x, q6);
+                hl[0] = q6[0] * m[0] + q6[3] * m[1] + q6[4] * m[2];
+                hl[1] = q6[3] * m[0] + q6[1] * m[1] + q6[5] * m[2];
+                hl[2] = q6[4] * m[0] + q6[5] * m[1] + q6[2] * m[2];
+            }
+            H[0] += hl[0]; H[1] += hl[1]; H[2] += hl[2];
+            B[0] += hl[0] + (in ? m[0] : 0.0);
+            B[1] += hl[1] + (in ? m[1] : 0.0);
+            B[2] += hl[2] + (in ? m[2] : 0.0);
             continue;
         }
         for (int64_t c = mv.copy_off[j]; c < mv.copy_off[j + 1]; c++) {
             const double* cp = mv.copies + 24 * c;
-            double xl[3], q6[6];
+            double xl[3], hl[3];
             apply_inv(cp, x, xl);
-            bool in = eval_k<R, KIND>(kind, rec, xl, q6);
-            double hl[3] = {q6[0] * m[0] + q6[3] * m[1] + q6[4] * m[2],
-                            q6[3] * m[0] + q6[1] * m[1] + q6[5] * m[2],
-                            q6[4] * m[0] + q6[5] * m[1] + q6[2] * m[2]};
+            bool in;
+            if (charged) {
+                in = field_tet<R>(crec, xl, hl);
+            } else {
+                double q6[6];
+                in = eval_k<R, KIND>(kind, rec, xl, q6);
+                hl[0] = q6[0] * m[0] + q6[3] * m[1] + q6[4] * m[2];
+                hl[1] = q6[3] * m[0] + q6[1] * m[1] + q6[5] * m[2];
+                hl[2] = q6[4] * m[0] + q6[5] * m[1] + q6[2] * m[2];
+            }
             double bl[3] = {hl[0] + (in ? m[0] : 0.0), hl[1] + (in ? m[1] : 0.0),
                             hl[2] + (in ? m[2] : 0.0)};
             double hg[3], bg[3];
             mat3_vec(cp + 12, hl, hg);
             mat3_vec(cp + 12, bl, bg);
-            for (int k = 0; k < 3; k++) { H[k] += hg[k]; B[k] += bg[k]; }
+            for (int q = 0; q < 3; q++) { H[q] += hg[q]; B[q] += bg[q]; }
         }
     }
 }

@@ -228,9 +228,15 @@ rh_status pack_model(const rh_model_desc* d, PackedModel* out, std::string* err)
     const int64_t nf_all = n > 0 ? d->face_off[n] : 0;
     std::vector<int64_t> fstart(nf_all + 1, 0);
     for (int64_t f = 0; f < nf_all; f++) fstart[f + 1] = fstart[f] + d->face_len[f];
+    P.adj_off.assign(size_t(n + 1), 0);
+    P.vert_off.assign(size_t(n + 1), 0);
     for (int64_t e = 0; e < n; e++) {
         P.off[e] = int64_t(P.data.size());
+        P.adj_off[e] = int64_t(P.adj.size());
+        P.vert_off[e + 1] = P.vert_off[e];
         if (d->elem_type[e] == 0) {
+            P.adj.push_back(0.0);
+            P.adj.push_back(0.0);
             P.kind[e] = EK_REC;
             const double* c = d->rec_center + 3 * e;
             const double* w = d->rec_dims + 3 * e;
@@ -279,6 +285,20 @@ rh_status pack_model(const rh_model_desc* d, PackedModel* out, std::string* err)
         }
         rh_status st = orient_faces(V, F, err);
         if (st != RH_OK) return st;
+        {
+            std::vector<double> tri;
+            for (const Face& f : F)
+                for (size_t k = 1; k + 1 < f.v.size(); k++) {
+                    tri.push_back(double(f.v[0]));
+                    tri.push_back(double(f.v[k]));
+                    tri.push_back(double(f.v[k + 1]));
+                }
+            P.adj.push_back(double(V.size()));
+            P.adj.push_back(double(tri.size() / 3));
+            for (const Vec& v : V) append(P.adj, v);
+            P.adj.insert(P.adj.end(), tri.begin(), tri.end());
+            P.vert_off[e + 1] = P.vert_off[e] + int64_t(V.size());
+        }
         bool tet = (V.size() == 4 && F.size() == 4);
         for (const Face& f : F) tet = tet && (f.v.size() == 3);
         if (tet) {
@@ -296,6 +316,7 @@ rh_status pack_model(const rh_model_desc* d, PackedModel* out, std::string* err)
             P.counts[7] += int64_t(F.size());
         }
     }
+    P.adj_off[n] = int64_t(P.adj.size());
     // element lists per kind slot: 0 blocks, 1 tetrahedra, 2 general polyhedra
     for (int slot = 0; slot < 3; slot++) {
         const int32_t want = (slot == 0) ? EK_REC : (slot == 1 ? EK_TET : EK_GEN);

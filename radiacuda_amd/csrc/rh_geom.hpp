@@ -27,6 +27,12 @@ struct PackedModel {
     bool sym = false;               // some element has a non-identity or extra copy
     bool rowt_identity = true;      // every row transform is the identity
     int64_t counts[8] = {0};        // see rh_model_counts
+    // adjoint geometry (rh_adjoint.hpp): per element nv, ntri, vertices in the
+    // caller's order, fan triangles of the outward-oriented faces; blocks get
+    // an empty record (nv = ntri = 0)
+    std::vector<int64_t> adj_off;   // [n+1]
+    std::vector<double> adj;
+    std::vector<int64_t> vert_off;  // [n+1] first vertex of each element in the caller's list
 };
 
 // Returns RH_OK or an error status with a message in *err.

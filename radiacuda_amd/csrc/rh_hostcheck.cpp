@@ -65,6 +65,44 @@ int rhc_field(const rh_model_desc* d, const double* M, const double* pts, int64_
     return RH_OK;
 }
 
+// Field through the per-kind path the kernels use (blocks, tetrahedra with
+// charged records, general polyhedra).
+int rhc_field_kinds(const rh_model_desc* d, const double* M, const double* pts, int64_t np,
+                    int32_t fp32, double* B, double* H)
+{
+    rh::PackedModel P;
+    rh_status st = rh::pack_model(d, &P, &g_err);
+    if (st != RH_OK) return st;
+    rh::ModelView mv = view(P);
+    const int64_t t0 = P.kind_off[1], t1 = P.kind_off[2];
+    std::vector<double> charged(size_t(rh::kChargedTet * (t1 - t0)) + 1);
+    for (int64_t k = t0; k < t1; k++) {
+        const int64_t j = P.kind_idx[k];
+        rh::charge_tet(P.data.data() + P.off[j], M + 3 * j, charged.data() + rh::kChargedTet * (k - t0));
+    }
+    mv.charged = charged.data();
+    mv.charged_base = t0;
+    const int32_t* idx = P.kind_idx.data();
+    for (int64_t p = 0; p < np; p++) {
+        double b[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+        const double* x = pts + 3 * p;
+        if (fp32) {
+            rh::field_add<float, true, 0>(mv, M, idx, P.kind_off[0], P.kind_off[1], x, b, h);
+            rh::field_add<float, true, 1>(mv, M, idx, P.kind_off[1], P.kind_off[2], x, b, h);
+            rh::field_add<float, true, 2>(mv, M, idx, P.kind_off[2], P.kind_off[3], x, b, h);
+        } else {
+            rh::field_add<double, true, 0>(mv, M, idx, P.kind_off[0], P.kind_off[1], x, b, h);
+            rh::field_add<double, true, 1>(mv, M, idx, P.kind_off[1], P.kind_off[2], x, b, h);
+            rh::field_add<double, true, 2>(mv, M, idx, P.kind_off[2], P.kind_off[3], x, b, h);
+        }
+        for (int k = 0; k < 3; k++) {
+            if (B) B[3 * p + k] = b[k];
+            if (H) H[3 * p + k] = h[k];
+        }
+    }
+    return RH_OK;
+}
+
 int rhc_colloc(const rh_model_desc* d, double* X)
 {
     rh::PackedModel P;

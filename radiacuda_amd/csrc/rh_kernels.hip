@@ -24,7 +24,18 @@ namespace {
 
 __host__ __device__ __forceinline__ ModelView view(const DevModel& m)
 {
-    return ModelView{m.kind, m.off, m.data, m.copy_off, m.copies};
+    return ModelView{m.kind, m.off, m.data, m.copy_off, m.copies, m.charged, m.koff[1]};
+}
+
+// Field-only tetrahedron records for magnetization M (one lane per tetrahedron).
+__global__ __launch_bounds__(256) void k_charge_tets(DevModel m, const double* __restrict__ M,
+                                                     double* __restrict__ out)
+{
+    const int64_t k = m.koff[1] + int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (k >= m.koff[2]) return;
+    const int64_t j = m.kidx[k];
+    const double mm[3] = {M[3 * j], M[3 * j + 1], M[3 * j + 2]};
+    charge_tet(m.data + m.off[j], mm, out + kChargedTet * (k - m.koff[1]));
 }
 
 template <class T>
@@ -113,6 +124,90 @@ __global__ __launch_bounds__(256) void k_field(DevModel m, const double* __restr
     const double x[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
     double b[3] = {0, 0, 0}, h[3] = {0, 0, 0};
     field_add<R, SYM, KIND>(view(m), M, m.kidx, k0, k1, x, b, h);
+    if (part) {
+        double* o = part + ((slice0 + int64_t(blockIdx.y)) * np + p) * 6;
+        for (int k = 0; k < 3; k++) {
+            o[k] = b[k];
+            o[3 + k] = h[k];
+        }
+    } else {
+        for (int k = 0; k < 3; k++) {
+            if (B) B[3 * p + k] = b[k];
+            if (H) H[3 * p + k] = h[k];
+        }
+    }
+}
+
+// Tetrahedron field sweep over charged records staged in LDS: the workgroup
+// copies kTetTile records (86 doubles each) with coalesced 16-byte loads, and
+// every lane then reads them at wave-uniform LDS addresses (broadcast, no bank
+// conflicts) while it evaluates its own point.  Staging keeps the record out of
+// scalar registers, whose pressure otherwise spills through VGPR lanes.
+constexpr int kTetTile = 32;
+// occupancy floor for the tetrahedron sweep (waves per SIMD): bounds the VGPRs
+// the scheduler may use for hoisted LDS reads
+#ifndef RH_TET_WAVES
+#define RH_TET_WAVES 3
+#endif
+
+template <class R, bool SYM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH_TET_WAVES))) void k_field_tet(DevModel m, const double* __restrict__ pts,
+                                                   int64_t np, int64_t js, int64_t slice0,
+                                                   double* __restrict__ B, double* __restrict__ H,
+                                                   double* __restrict__ part)
+{
+    __shared__ __align__(16) double tile[kTetTile * kChargedTet];
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const bool live = p < np;
+    const int64_t ntet = m.koff[2] - m.koff[1];
+    const int64_t t0 = int64_t(blockIdx.y) * js;
+    const int64_t t1 = (t0 + js < ntet) ? (t0 + js) : ntet;
+    double x[3] = {0.0, 0.0, 0.0};
+    if (live) {
+        x[0] = pts[3 * p];
+        x[1] = pts[3 * p + 1];
+        x[2] = pts[3 * p + 2];
+    }
+    double b[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+    for (int64_t base = t0; base < t1; base += kTetTile) {
+        const int cnt = int((t1 - base < kTetTile) ? (t1 - base) : kTetTile);
+        const double2* src = reinterpret_cast<const double2*>(m.charged + kChargedTet * base);
+        double2* dst = reinterpret_cast<double2*>(tile);
+        __syncthreads();  // the previous tile has been consumed
+        for (int i = threadIdx.x; i < cnt * (kChargedTet / 2); i += 256) dst[i] = src[i];
+        __syncthreads();
+        if (!live) continue;
+#pragma unroll 1
+        for (int q = 0; q < cnt; q++) {
+            const double* rec = tile + kChargedTet * q;
+            const double mm[3] = {rec[kChargedM], rec[kChargedM + 1], rec[kChargedM + 2]};
+            if (!SYM) {
+                double hl[3];
+                const bool in = field_tet<R>(rec, x, hl);
+                for (int k = 0; k < 3; k++) {
+                    h[k] += hl[k];
+                    b[k] += hl[k] + (in ? mm[k] : 0.0);
+                }
+                continue;
+            }
+            const int64_t j = m.kidx[m.koff[1] + base + q];
+            for (int64_t c = m.copy_off[j]; c < m.copy_off[j + 1]; c++) {
+                const double* cp = m.copies + 24 * c;
+                double xl[3], hl[3], hg[3], bg[3];
+                apply_inv(cp, x, xl);
+                const bool in = field_tet<R>(rec, xl, hl);
+                const double bl[3] = {hl[0] + (in ? mm[0] : 0.0), hl[1] + (in ? mm[1] : 0.0),
+                                      hl[2] + (in ? mm[2] : 0.0)};
+                mat3_vec(cp + 12, hl, hg);
+                mat3_vec(cp + 12, bl, bg);
+                for (int k = 0; k < 3; k++) {
+                    h[k] += hg[k];
+                    b[k] += bg[k];
+                }
+            }
+        }
+    }
+    if (!live) return;
     if (part) {
         double* o = part + ((slice0 + int64_t(blockIdx.y)) * np + p) * 6;
         for (int k = 0; k < 3; k++) {
@@ -350,6 +445,32 @@ __device__ void local_solve(const double* mat, const double* q, const double* he
     }
 }
 
+// FP64 defect of the stored-matrix field: out = hext + rowT*H64 - y - Qii M,
+// with H64 the FP64 field of all elements at the collocation points (lab
+// frame) and y the stored-matrix product (diagonal blocks excluded).
+__global__ __launch_bounds__(256) void k_refine_defect(int64_t n, const double* __restrict__ H64,
+                                                       const double* __restrict__ rowt,
+                                                       const double* __restrict__ y,
+                                                       const double* __restrict__ qdiag,
+                                                       const double* __restrict__ M,
+                                                       const double* __restrict__ hext,
+                                                       double* __restrict__ out)
+{
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    double h[3] = {H64[3 * i], H64[3 * i + 1], H64[3 * i + 2]};
+    if (rowt) {
+        const double* t = rowt + 9 * i;
+        const double g[3] = {h[0], h[1], h[2]};
+        for (int k = 0; k < 3; k++) h[k] = t[3 * k] * g[0] + t[3 * k + 1] * g[1] + t[3 * k + 2] * g[2];
+    }
+    const double* D = qdiag + 9 * i;
+    const double m[3] = {M[3 * i], M[3 * i + 1], M[3 * i + 2]};
+    for (int k = 0; k < 3; k++)
+        out[3 * i + k] = hext[3 * i + k] + (h[k] - y[3 * i + k]) -
+                         (D[3 * k] * m[0] + D[3 * k + 1] * m[1] + D[3 * k + 2] * m[2]);
+}
+
 __global__ __launch_bounds__(256) void k_relax_update(int64_t n, const double* __restrict__ mats,
                                                       const double* __restrict__ hext,
                                                       const double* __restrict__ qdiag,
@@ -401,24 +522,33 @@ struct DotPack {
     const double* b[17];
 };
 
+// All nd dot products in one pass: every lane keeps nd running sums, waves
+// reduce by shuffles, the workgroup writes one partial per dot.
 __global__ __launch_bounds__(256) void k_dots(int nd, DotPack pk, int64_t len,
                                               double* __restrict__ partials)
 {
-    __shared__ double red[256];
-    for (int d = 0; d < nd; d++) {
-        double acc = 0.0;
-        for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < len;
-             i += int64_t(gridDim.x) * 256)
-            acc += pk.a[d][i] * pk.b[d][i];
-        red[threadIdx.x] = acc;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) partials[d * gridDim.x + blockIdx.x] = red[0];
-        __syncthreads();
+    __shared__ double red[4][17];
+    double acc[17];
+#pragma unroll
+    for (int d = 0; d < 17; d++) acc[d] = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < len;
+         i += int64_t(gridDim.x) * 256) {
+#pragma unroll
+        for (int d = 0; d < 17; d++)
+            if (d < nd) acc[d] += pk.a[d][i] * pk.b[d][i];
     }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < 17; d++) {
+        if (d < nd) {
+            const double v = wave_sum(acc[d]);
+            if (lane == 0) red[wave][d] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nd)
+        partials[threadIdx.x * gridDim.x + blockIdx.x] =
+            red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
 }
 
 struct CombPack {
@@ -488,8 +618,14 @@ void field_slot(const DevModel& m, const double* M, const double* pts, int64_t n
                 int64_t slice0, double* B, double* H, double* part, hipStream_t s)
 {
     const int64_t cnt = m.koff[KIND + 1] - m.koff[KIND];
-    const int64_t js = (cnt + splits - 1) / splits;
+    int64_t js = (cnt + splits - 1) / splits;
     const dim3 grid{unsigned((np + 255) / 256), unsigned(splits), 1u};
+    if (KIND == 1 && m.charged) {
+        js = (js + kTetTile - 1) / kTetTile * kTetTile;
+        hipLaunchKernelGGL((k_field_tet<R, SYM>), grid, dim3(256), 0, s, m, pts, np, js, slice0, B,
+                           H, part);
+        return;
+    }
     hipLaunchKernelGGL((k_field<R, SYM, KIND>), grid, dim3(256), 0, s, m, M, pts, np, js, slice0, B,
                        H, part);
 }
@@ -525,14 +661,22 @@ int64_t field_slices(const DevModel& m, int64_t np)
     return std::max<int64_t>(1, total);
 }
 
-hipError_t launch_field(const DevModel& m, const double* M, const double* pts, int64_t np,
-                        double* B, double* H, double* work, bool fp32, hipStream_t s)
+hipError_t launch_field(const DevModel& m0, const double* M, const double* pts, int64_t np,
+                        double* B, double* H, double* work, double* charged, bool fp32,
+                        hipStream_t s)
 {
     if (np == 0) return hipSuccess;
-    if (m.n == 0) {
+    if (m0.n == 0) {
         if (B) (void)hipMemsetAsync(B, 0, size_t(3 * np) * sizeof(double), s);
         if (H) (void)hipMemsetAsync(H, 0, size_t(3 * np) * sizeof(double), s);
         return hipGetLastError();
+    }
+    DevModel m = m0;
+    const int64_t ntet = m.koff[2] - m.koff[1];
+    if (ntet > 0 && charged) {
+        hipLaunchKernelGGL(k_charge_tets, dim3(unsigned((ntet + 255) / 256)), dim3(256), 0, s, m, M,
+                           charged);
+        m.charged = charged;
     }
     if (fp32) return m.sym ? field_all<float, true>(m, M, pts, np, B, H, work, s)
                            : field_all<float, false>(m, M, pts, np, B, H, work, s);
@@ -566,7 +710,16 @@ hipError_t launch_relax_update(int64_t n, const double* mats, const double* hext
     return hipGetLastError();
 }
 
-int64_t dot_blocks(int64_t len) { return std::max<int64_t>(1, std::min<int64_t>(256, (len + 4095) / 4096)); }
+hipError_t launch_refine_defect(int64_t n, const double* H64, const double* rowt, const double* y,
+                                const double* qdiag, const double* M, const double* hext,
+                                double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_refine_defect, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, H64, rowt,
+                       y, qdiag, M, hext, out);
+    return hipGetLastError();
+}
+
+int64_t dot_blocks(int64_t len) { return std::max<int64_t>(1, std::min<int64_t>(128, (len + 511) / 512)); }
 
 hipError_t launch_dots(int nd, const double* const* a, const double* const* b, int64_t len,
                        double* partials, hipStream_t s)

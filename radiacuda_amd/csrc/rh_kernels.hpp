@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
+#include <vector>
 
 namespace rh {
 
@@ -18,6 +20,7 @@ struct DevModel {
     const double* rowt = nullptr;    // [9n]
     const int32_t* kidx = nullptr;   // element indices grouped by kind slot
     int64_t koff[4] = {0, 0, 0, 0};  // slot ranges in kidx (block, tet, general)
+    const double* charged = nullptr;  // field-only tetrahedron records (per call)
     bool sym = false;
     bool rowt_identity = true;
 };
@@ -31,9 +34,11 @@ hipError_t launch_assemble(const DevModel& m, void* Q, int elem_bytes, int64_t l
 // Number of partial-sum slices the field kernels use for np points (>= 1);
 // the work buffer needs 6 * np * slices doubles when slices > 1.
 int64_t field_slices(const DevModel& m, int64_t np);
-// B, H (either may be null) at np points.
+// B, H (either may be null) at np points.  charged: kChargedTet doubles per
+// tetrahedron, rebuilt from M before the sweep.
 hipError_t launch_field(const DevModel& m, const double* M, const double* pts, int64_t np,
-                        double* B, double* H, double* work, bool fp32, hipStream_t s);
+                        double* B, double* H, double* work, double* charged, bool fp32,
+                        hipStream_t s);
 
 // y = Q x (+ add), Q: nrows x ld (ld multiple of 256, padded columns zero,
 // x padded with zeros to ld).
@@ -48,6 +53,11 @@ hipError_t launch_relax_update(int64_t n, const double* mats, const double* hext
                                const double* qdiag, const double* y, const double* M, double* F,
                                double* Hout, double* partials, hipStream_t s);
 
+// out = hext + rowT*H64 - y - Qii*M per element (rowt may be null = identity).
+hipError_t launch_refine_defect(int64_t n, const double* H64, const double* rowt, const double* y,
+                                const double* qdiag, const double* M, const double* hext,
+                                double* out, hipStream_t s);
+
 // Batched dot products: out[k*nb + b] = partial of sum_i a_k[i]*b_k[i] over block b.
 int64_t dot_blocks(int64_t len);
 hipError_t launch_dots(int nd, const double* const* a, const double* const* b, int64_t len,
@@ -60,5 +70,38 @@ hipError_t launch_lincomb(int nv, const double* const* v, const double* c, int64
 // B += field of current sources (kind [ns], par [12 ns], copies [24 ns]) at np points.
 hipError_t launch_current(int64_t ns, const int32_t* kind, const double* par, const double* cp,
                           const double* pts, int64_t np, double* B, hipStream_t s);
+
+// ---------------------------------------------------------------- adjoint
+// Point groups (workgroups along the point axis, each striding over tiles of
+// 256 points) for `slices` element slices.
+int64_t adj_point_groups(int64_t np, int64_t slices);
+// Element slices of the dL/dM sweep.
+int64_t adj_m_slices(int64_t n);
+// Element slice bounds of the dL/dvertex sweep (<= 128 vertices per slice);
+// empty with *err set if an element has more vertices.
+std::vector<int64_t> adj_geom_slices(const std::vector<int64_t>& vert_off, std::string* err);
+// gM [3n] = sum_p sum_c (Q_j(x'_pc) + [addin and inside] I) F_c^T g_p.
+// part: groups * 3n doubles.
+hipError_t launch_adj_m(const DevModel& m, const double* pts, int64_t np, const double* g, bool addin,
+                        double* part, int64_t groups, double* gM, hipStream_t s);
+// gV [3 nvert] = dL/d(vertex), gX [3 np] = dL/d(point) (either may be null).
+// partv: groups * 3 nvert doubles; partx: nslices * 3 np doubles.
+hipError_t launch_adj_geom(const DevModel& m, const double* adj, const int64_t* adj_off,
+                           const int64_t* vert_off, const int64_t* sl, int64_t nslices,
+                           int64_t nvert, const double* M, const double* pts, int64_t np,
+                           const double* g, double* partv, double* partx, int64_t groups,
+                           double* gV, double* gX, hipStream_t s);
+// y = Q^T x over the stored rows (diagonal blocks stored as zeros);
+// part: chunks * ld doubles.
+int64_t matvec_t_chunks(int64_t nrows, int64_t ld);
+hipError_t launch_matvec_t(const void* Q, int elem_bytes, int64_t nrows, int64_t ld, const double* x,
+                           double* part, int64_t chunks, double* y, hipStream_t s);
+// mu_i = rowT_i^T K_i^T lam_i (rowt may be null).
+hipError_t launch_adj_scale(int64_t n, const double* mats, const double* rowt, const double* lam,
+                            double* mu, hipStream_t s);
+// Block-Jacobi update of (I - Q^T K^T) lam = rhs (linear materials).
+hipError_t launch_adj_update(int64_t n, const double* mats, const double* rhs, const double* qdiag,
+                             const double* y, const double* lam, double* F, double* partials,
+                             hipStream_t s);
 
 }  // namespace rh

@@ -79,12 +79,40 @@ RH_HD V3<R> cross(const V3<R>& a, const V3<R>& b)
 
 RH_HD double rh_log(double x) { return log(x); }
 RH_HD float rh_log(float x) { return logf(x); }
-RH_HD double rh_log1p(double x) { return log1p(x); }
-RH_HD float rh_log1p(float x) { return log1pf(x); }
 RH_HD double rh_sqrt(double x) { return sqrt(x); }
-RH_HD float rh_sqrt(float x) { return sqrtf(x); }
+RH_HD double rh_rcp(double x) { return 1.0 / x; }
 RH_HD double rh_atan2(double y, double x) { return atan2(y, x); }
 RH_HD float rh_atan2(float y, float x) { return atan2f(y, x); }
+// FP32 square root and reciprocal: the hardware instructions (about 1 ulp) on
+// the device instead of the correctly rounded library sequences.
+RH_HD float rh_sqrt(float x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+RH_HD float rh_rcp(float x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+// ln(1 + x) for x >= 0.  FP64: the library routine.  FP32: with u = fl(1 + x),
+// ln(u) * x / (u - 1) is accurate to a few ulp because u - 1 is exact and
+// ln(1 + y)/y varies slowly (Goldberg, "What every computer scientist should
+// know about floating-point arithmetic", 1991, Theorem 4).
+RH_HD double rh_log1p(double x) { return log1p(x); }
+RH_HD float rh_log1p(float x)
+{
+    const float u = 1.0f + x;
+    const float d = u - 1.0f;
+    const float r = rh_log(u) * (x * rh_rcp(d));
+    return (d == 0.0f) ? x : ((u < 3.0e38f) ? r : u);
+}
 
 // Vertex relative to the observation point, formed in FP64 then rounded to R.
 template <class R>
@@ -118,19 +146,16 @@ RH_HD void edge_ratio(const V3<R>& ra, R Ra, const V3<R>& rb, R Rb, const V3<R>&
 template <class R>
 RH_HD R edge_log1p(const V3<R>& ra, R Ra, const V3<R>& rb, R Rb, const V3<R>& t, R len)
 {
-    R sa = dot(ra, t), sb = dot(rb, t);
-    R ssum = sa + sb;
-    bool pos = ssum >= R(0);
-    R Rm = pos ? Ra : Rb, sm = pos ? sa : -sb;
-    R g = len * (R(1) + fabs(ssum) / (Ra + Rb));
-    R delta;
-    if (sm >= R(0)) {
-        delta = g / (Rm + sm);
-    } else {
-        V3<R> c = cross(ra, t);
-        delta = g * (Rm - sm) / dot(c, c);
-    }
-    return rh_log1p(delta);
+    const R sa = dot(ra, t), sb = dot(rb, t);
+    const R ssum = sa + sb;
+    const bool pos = ssum >= R(0);
+    const R Rm = pos ? Ra : Rb, sm = pos ? sa : -sb;
+    const R g = len * (R(1) + fabs(ssum) * rh_rcp(Ra + Rb));
+    const V3<R> c = cross(ra, t);
+    const bool good = sm >= R(0);
+    const R num = good ? g : g * (Rm - sm);
+    const R den = good ? (Rm + sm) : dot(c, c);
+    return rh_log1p(num * rh_rcp(den));
 }
 
 // Solid-angle factor of triangle (r1, r2, r3) as a complex number whose
@@ -294,19 +319,34 @@ RH_HD bool eval_tet(const double* __restrict__ rec, const double p[3], double q6
         for (int k = 0; k < 6; k++) acc[k] += L * E[4 + k];
     }
     const double* fn = rec + 12 + 6 * kTetEdge;
+    // signed plane distances (r_a . n_f) decide the inside test and the sign of
+    // every solid angle, so the two stay consistent
+    R hd[4];
     bool inside = true;
 #pragma unroll
     for (int f = 0; f < 4; f++) {
-        const int a = kTetFace[f][0], b = kTetFace[f][1], c = kTetFace[f][2];
         const double* F = fn + kTetFaceRec * f;
-        const V3<R> n = v3(R(F[0]), R(F[1]), R(F[2]));
-        // r_a . n is the signed distance of the face plane from the point; the
-        // triple product r_a.(r_b x r_c) equals 2*area*(r_a . n) exactly
-        const R hd = dot(r[a], n);
-        const R im = R(F[3]) * hd;
-        const R re = Rv[a] * Rv[b] * Rv[c] + dot(r[a], r[b]) * Rv[c] + dot(r[a], r[c]) * Rv[b] +
-                     dot(r[b], r[c]) * Rv[a];
-        const double om = 2.0 * double(rh_atan2(im, re));
+        hd[f] = dot(r[kTetFace[f][0]], v3(R(F[0]), R(F[1]), R(F[2])));
+        inside = inside && (hd[f] > R(0));
+    }
+    // the solid angles of a closed surface sum to 4 pi inside and 0 outside:
+    // three atan2 give the fourth
+    double om_sum = 0.0;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        const double* F = fn + kTetFaceRec * f;
+        double om;
+        if (f < 3) {
+            const int a = kTetFace[f][0], b = kTetFace[f][1], c = kTetFace[f][2];
+            // r_a.(r_b x r_c) = 2*area*(r_a . n_f) exactly
+            const R im = R(F[3]) * hd[f];
+            const R re = Rv[a] * Rv[b] * Rv[c] + dot(r[a], r[b]) * Rv[c] + dot(r[a], r[c]) * Rv[b] +
+                         dot(r[b], r[c]) * Rv[a];
+            om = 2.0 * double(rh_atan2(im, re));
+            om_sum += om;
+        } else {
+            om = (inside ? 4.0 * 3.14159265358979323846 : 0.0) - om_sum;
+        }
         const double nx = F[0], ny = F[1], nz = F[2];
         acc[0] -= om * nx * nx;
         acc[1] -= om * ny * ny;
@@ -314,11 +354,104 @@ RH_HD bool eval_tet(const double* __restrict__ rec, const double p[3], double q6
         acc[3] -= om * nx * ny;
         acc[4] -= om * nx * nz;
         acc[5] -= om * ny * nz;
-        // strictly on the inner side of every face plane
-        inside = inside && (hd > R(0));
     }
 #pragma unroll
     for (int k = 0; k < 6; k++) q6[k] = kInv4Pi * acc[k];
+    return inside;
+}
+
+// ---------------------------------------------------------------- charged tetrahedron
+// Field-only record for a given magnetization M (element frame), built per
+// field call: verts(12) | edges x6: t(3), len, w = W_e M (3) | faces x4: n(3),
+// 2*area, g = (n.M) n (3) | M(3), pad.  H = 1/(4pi) [sum L_e w_e - sum Omega_f g_f].
+// 86 doubles = 43 x 16 bytes.
+constexpr int kChargedM = 12 + 6 * 7 + 4 * 7;  // offset of M
+constexpr int kChargedTet = kChargedM + 4;     // 86
+
+RH_HD void charge_tet(const double* __restrict__ rec, const double M[3], double* __restrict__ out)
+{
+    for (int k = 0; k < 12; k++) out[k] = rec[k];
+    for (int e = 0; e < 6; e++) {
+        const double* E = rec + 12 + kTetEdge * e;
+        double* o = out + 12 + 7 * e;
+        const double* W = E + 4;  // xx yy zz xy xz yz
+        o[0] = E[0]; o[1] = E[1]; o[2] = E[2]; o[3] = E[3];
+        o[4] = W[0] * M[0] + W[3] * M[1] + W[4] * M[2];
+        o[5] = W[3] * M[0] + W[1] * M[1] + W[5] * M[2];
+        o[6] = W[4] * M[0] + W[5] * M[1] + W[2] * M[2];
+    }
+    for (int f = 0; f < 4; f++) {
+        const double* F = rec + 12 + 6 * kTetEdge + kTetFaceRec * f;
+        double* o = out + 12 + 6 * 7 + 7 * f;
+        const double sg = F[0] * M[0] + F[1] * M[1] + F[2] * M[2];
+        o[0] = F[0]; o[1] = F[1]; o[2] = F[2]; o[3] = F[3];
+        o[4] = sg * F[0]; o[5] = sg * F[1]; o[6] = sg * F[2];
+    }
+    out[kChargedM] = M[0];
+    out[kChargedM + 1] = M[1];
+    out[kChargedM + 2] = M[2];
+    out[kChargedM + 3] = 0.0;
+}
+
+template <class R>
+RH_HD bool field_tet(const double* __restrict__ rec, const double p[3], double H[3])
+{
+    V3<R> r[4];
+    R Rv[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        r[v] = rel<R>(rec + 3 * v, p);
+        Rv[v] = rh_sqrt(dot(r[v], r[v]));
+    }
+    double h0 = 0.0, h1 = 0.0, h2 = 0.0;
+#pragma unroll
+    for (int e = 0; e < 6; e++) {
+        const double* E = rec + 12 + 7 * e;
+        const int a = kTetEdgeA[e], b = kTetEdgeB[e];
+        const V3<R> t = v3(R(E[0]), R(E[1]), R(E[2]));
+        double L;
+        if (sizeof(R) == 4) {
+            L = double(edge_log1p(r[a], Rv[a], r[b], Rv[b], t, R(E[3])));
+        } else {
+            R num, den;
+            edge_ratio(r[a], Rv[a], r[b], Rv[b], t, num, den);
+            L = double(rh_log(num / den));
+        }
+        h0 += L * E[4];
+        h1 += L * E[5];
+        h2 += L * E[6];
+    }
+    const double* fn = rec + 12 + 6 * 7;
+    R hd[4];
+    bool inside = true;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        const double* F = fn + 7 * f;
+        hd[f] = dot(r[kTetFace[f][0]], v3(R(F[0]), R(F[1]), R(F[2])));
+        inside = inside && (hd[f] > R(0));
+    }
+    double om_sum = 0.0;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        const double* F = fn + 7 * f;
+        double om;
+        if (f < 3) {
+            const int a = kTetFace[f][0], b = kTetFace[f][1], c = kTetFace[f][2];
+            const R im = R(F[3]) * hd[f];
+            const R re = Rv[a] * Rv[b] * Rv[c] + dot(r[a], r[b]) * Rv[c] + dot(r[a], r[c]) * Rv[b] +
+                         dot(r[b], r[c]) * Rv[a];
+            om = 2.0 * double(rh_atan2(im, re));
+            om_sum += om;
+        } else {
+            om = (inside ? 4.0 * 3.14159265358979323846 : 0.0) - om_sum;
+        }
+        h0 -= om * F[4];
+        h1 -= om * F[5];
+        h2 -= om * F[6];
+    }
+    H[0] = kInv4Pi * h0;
+    H[1] = kInv4Pi * h1;
+    H[2] = kInv4Pi * h2;
     return inside;
 }
 

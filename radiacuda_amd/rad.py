@@ -408,15 +408,18 @@ def _opt(opts, name, default):
 def RlxAuto(intrc, prec, maxiter, meth=4, *opts):
     """Relaxes to prec (T) within maxiter iterations; returns
     [misfit, max|M|, max|H|, iterations] (radentry.h:746-761).  Every method
-    number runs the device solver (adaptive under-relaxed block Jacobi)."""
+    number runs the device solver (Anderson-accelerated block Jacobi on the
+    FP32-stored matrix, then one FP64 refinement pass; option 'Refine->k'
+    sets k passes, 'Refine->-1' none)."""
     it = _obj(intrc)
     if not isinstance(it, _Interaction):
         raise RuntimeError("Radia::Error014: not an interaction structure")
     zero_m = _opt(opts, "ZeroM", "True").lower() not in ("false", "0", "no")
     omega = float(_opt(opts, "omega", "0"))
+    refine = int(_opt(opts, "Refine", "0"))
     M0 = np.stack([o.M for (o, _k, _s) in it.items])
     res, M, H = it.matrix.relax(it.mats, it.hext, M0, prec, maxiter, keep_m=not zero_m,
-                                omega0=omega)
+                                omega0=omega, refine=refine)
     for e, (o, _k, _s) in enumerate(it.items):
         o.M = M[e].copy()
     it.H = H

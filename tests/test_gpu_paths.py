@@ -134,8 +134,21 @@ def test_relax_linear_matches_direct_solve(oracle_off):
         A[3 * e:3 * e + 3, :] -= K @ Qs[3 * e:3 * e + 3, :]
         rhs[3 * e:3 * e + 3] = K @ it.hext[e] + mats[e, 13:16]
     Mref = np.linalg.solve(A, rhs).reshape(n, 3)
+    # FP64 system (oracle matrix): what the refined solve converges to
+    A64 = np.eye(3 * n)
+    for e in range(n):
+        K = mats[e, 4:13].reshape(3, 3)
+        A64[3 * e:3 * e + 3, :] -= K @ Qo[3 * e:3 * e + 3, :]
+    M64 = np.linalg.solve(A64, rhs).reshape(n, 3)
     assert res[0] <= 1e-12
-    assert np.abs(M - Mref).max() <= 1e-11 * np.abs(Mref).max()
+    print("refined vs FP64 system %.3e, stored system %.3e"
+          % (np.abs(M - M64).max() / np.abs(M64).max(), np.abs(M - Mref).max() / np.abs(Mref).max()))
+    assert np.abs(M - M64).max() <= 1e-11 * np.abs(M64).max()
+    # without refinement the solve stops at the stored (FP32 off-diagonal) system
+    M0 = np.stack([o.M for (o, _k, _s) in items])
+    r2, M2, _H2 = it.matrix.relax(it.mats, it.hext, M0, 1e-12, 2000, refine=-1)
+    assert r2.refined == 0 and r2.misfit <= 1e-12
+    assert np.abs(M2 - Mref).max() <= 1e-11 * np.abs(Mref).max()
 
 
 def test_relax_nonlinear_matches_fixed_point(oracle_off):
@@ -160,13 +173,20 @@ def test_relax_nonlinear_matches_fixed_point(oracle_off):
     Qo, _jit = models.oracle_assemble(P)
     Mo, _H, iters, stats = OR.relax_m4(Qo.astype(np.float32), mats, it.hext, 1e-8, 20000)
     print("oracle m4", iters, stats, flush=True)
-    Mn = models.fp64_fixed_point(Qs, mats, it.hext, M0=Mo)
+    # the refined solve reaches the FP64 fixed point of the oracle's matrix
+    Mn = models.fp64_fixed_point(Qo, mats, it.hext, M0=Mo)
     scale = np.abs(Mn).max()
     assert res[0] <= 1e-11
+    print("refined vs FP64 fixed point %.3e" % (np.abs(M - Mn).max() / scale), flush=True)
     # residual <= 1e-11 T times the conditioning of chi~2000 iron
     assert np.abs(M - Mn).max() <= 1e-7 * scale
     # FP32 dot products move the Gauss-Seidel fixed point by ~1.3e-4 here
     assert np.abs(M - Mo).max() <= 5e-4 * scale
+    # unrefined: the fixed point of the stored matrix
+    Ms = models.fp64_fixed_point(Qs, mats, it.hext, M0=Mo)
+    r2, M2, _H2 = it.matrix.relax(it.mats, it.hext, np.zeros_like(M), 1e-11, 3000, refine=-1)
+    assert r2.misfit <= 1e-11
+    assert np.abs(M2 - Ms).max() <= 1e-7 * scale
 
 
 def test_current_sources_match_oracle(oracle_off):
