@@ -182,6 +182,46 @@ rh_status rh_current_field_dev(int device, void *stream, int64_t nsrc, const int
                                const double *par, const double *tm, const double *tv,
                                const double *ts, const double *d_pts, int64_t np, double *d_B_acc);
 
+/* ------------------------------------------------------------------------ */
+/* Adjoint (differentiation)                                                  */
+/* ------------------------------------------------------------------------ */
+/* No counterpart in the reference: the reference has no derivatives of its
+ * field or relaxation.  These give the reverse-mode products a gradient-based
+ * design loop needs (dL/dM, dL/d(vertex), dL/d(point)) and the adjoint solve
+ * of the linear-material relaxation, all in FP64. */
+
+/* Number of polyhedron vertices of the model: the description's vertices in
+ * its order (blocks have none); gV below holds 3 doubles per vertex. */
+rh_status rh_model_vertex_count(const rh_model *m, int64_t *nvert);
+
+/* Adjoint of rh_field for weights gB = dL/dB (field = 0) or dL/dH (field = 1)
+ * at the np points; device arrays of doubles, any output may be null:
+ *   gM [3n]      dL/dM, element frames
+ *   gV [3 nvert] dL/d(vertex coordinates), element frames (needs d_M)
+ *   gX [3 np]    dL/d(point coordinates), lab frame       (needs d_M)
+ * gV and gX need a model of polyhedra only (RH_ERR_UNSUPPORTED otherwise).
+ * Vertex derivatives treat each face as its fan of triangles, which equals
+ * the face for planar faces and stays a closed surface under displacement. */
+rh_status rh_field_adjoint_dev(rh_model *m, const double *d_M, const double *d_pts, int64_t np,
+                               const double *d_gB, int32_t field, double *d_gM, double *d_gV,
+                               double *d_gX);
+/* Same on host arrays (M may be null when only gM is requested). */
+rh_status rh_field_adjoint(rh_model *m, const double *M, const double *pts, int64_t np,
+                           const double *gB, int32_t field, double *gM, double *gV, double *gX);
+
+/* Adjoint of the linear-material relaxation (material kinds 0 and 1):
+ * solves (I - Q^T K^T) lam = rhs on the stored matrix, K the susceptibility
+ * tensor of each element, by Anderson-accelerated block Jacobi on the
+ * transposed system.  With M solving (I - K Q) M = K Hext + M_r and a loss
+ * L(M), rhs = dL/dM gives dL/dM_r = lam and dL/dHext = K^T lam.  d_mats
+ * [16n], d_rhs [3n], d_lam [3n] device doubles (d_lam is the start value
+ * when opts->keep_m, and the result). */
+rh_status rh_relax_adjoint_dev(rh_matrix *q, const double *d_mats, const double *d_rhs,
+                               double *d_lam, const rh_relax_opts *opts, rh_relax_result *res);
+/* Same on host arrays. */
+rh_status rh_relax_adjoint(rh_matrix *q, const double *mats, const double *rhs, double *lam,
+                           const rh_relax_opts *opts, rh_relax_result *res);
+
 /* Diagnostics */
 const char *rh_last_error(void);
 int32_t rh_device_count(void);

@@ -32,6 +32,14 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise RadHipError("libradhip.so is not built (run: python -m radiacuda_amd.build); "
                           "the HIP path has no CPU fallback")
+    try:
+        # One HIP runtime per process: PyTorch ships its own copy of the runtime
+        # (needed as libamdhip64.so); loading it after this library's (needed as
+        # libamdhip64.so.7) would start a second runtime that finds no device.
+        # Loaded first, torch's copy satisfies this library's dependency too.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     st = ctypes.c_int
     L.rh_last_error.restype = ctypes.c_char_p
@@ -56,13 +64,23 @@ def lib():
                                    ctypes.c_int64, _d]
     L.rh_current_field_dev.argtypes = [ctypes.c_int, _vp, ctypes.c_int64, _i32, _d, _d, _d, _d,
                                        _vp, ctypes.c_int64, _vp]
+    L.rh_model_vertex_count.argtypes = [_vp, _i64]
+    L.rh_field_adjoint.argtypes = [_vp, _d, _d, ctypes.c_int64, _d, ctypes.c_int32, _d, _d, _d]
+    L.rh_field_adjoint_dev.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, ctypes.c_int32, _vp,
+                                       _vp, _vp]
+    L.rh_relax_adjoint_dev.argtypes = [_vp, _vp, _vp, _vp, ctypes.POINTER(_abi.RelaxOpts),
+                                       ctypes.POINTER(_abi.RelaxResult)]
+    L.rh_relax_adjoint.argtypes = [_vp, _d, _d, _d, ctypes.POINTER(_abi.RelaxOpts),
+                                   ctypes.POINTER(_abi.RelaxResult)]
     L.rh_set_timing.argtypes = [ctypes.c_int32]
     L.rh_kernel_time.argtypes = [ctypes.c_char_p, _d, _i64]
     for name in ["rh_model_create", "rh_model_destroy", "rh_model_colloc", "rh_model_counts",
                  "rh_matrix_assemble", "rh_matrix_destroy", "rh_matrix_export",
                  "rh_matrix_device_ptr", "rh_matrix_reassemble", "rh_relax", "rh_relax_dev",
                  "rh_field", "rh_field_dev", "rh_current_field", "rh_current_field_dev",
-                 "rh_set_timing", "rh_kernel_time"]:
+                 "rh_set_timing", "rh_kernel_time", "rh_model_vertex_count",
+                 "rh_field_adjoint", "rh_field_adjoint_dev", "rh_relax_adjoint_dev",
+                 "rh_relax_adjoint"]:
         getattr(L, name).restype = st
     if L.rh_device_count() < 1:
         raise RadHipError("no HIP device visible; the HIP path has no CPU fallback")
@@ -84,8 +102,8 @@ def _c(a):
 
 
 def stream_handle():
-    """The current torch stream when the caller has already initialised torch's
-    GPU context, else the null stream (torch is never imported from here)."""
+    """The current torch stream when torch's GPU context is initialised, else
+    the null stream."""
     import sys
     torch = sys.modules.get("torch")
     if torch is not None:
@@ -141,6 +159,27 @@ class Model:
     def assemble(self, prec=RH_MIXED):
         return Matrix(self, prec)
 
+    def vertex_count(self):
+        nv = np.zeros(1, np.int64)
+        check(lib().rh_model_vertex_count(self.h, nv.ctypes.data_as(_i64)))
+        return int(nv[0])
+
+    def field_adjoint(self, M, pts, gB, field="b", want_m=True, want_v=False, want_x=False):
+        """Reverse-mode products of the field at pts for weights gB = dL/dB
+        (field 'b') or dL/dH ('h'): returns (gM [n,3], gV [nvert,3], gX [np,3]),
+        None where not requested."""
+        pts = _c(pts).reshape(-1, 3)
+        gB = _c(gB).reshape(-1, 3)
+        Mv = _c(M).reshape(-1) if M is not None else None
+        gM = np.zeros((self.n, 3)) if want_m else None
+        gV = np.zeros((self.vertex_count(), 3)) if want_v else None
+        gX = np.zeros_like(pts) if want_x else None
+        check(lib().rh_field_adjoint(self.h, _p(Mv) if Mv is not None else None, _p(pts), len(pts),
+                                     _p(gB), 0 if field == "b" else 1,
+                                     _p(gM) if want_m else None, _p(gV) if want_v else None,
+                                     _p(gX) if want_x else None))
+        return gM, gV, gX
+
     def field(self, M, pts, want_b=True, want_h=True, precision=RH_FP64):
         M = _c(M).reshape(-1)
         pts = _c(pts).reshape(-1, 3)
@@ -194,6 +233,17 @@ class Matrix:
         check(lib().rh_relax(self.h, _p(mats), _p(hext), _p(M), _p(H), ctypes.byref(opts),
                              ctypes.byref(res)))
         return res, M.reshape(-1, 3), H.reshape(-1, 3)
+
+    def relax_adjoint(self, mats, rhs, prec, max_iter, lam0=None):
+        """Solves (I - Q^T K^T) lam = rhs (linear materials); returns (res, lam)."""
+        mats = _c(mats).reshape(-1)
+        rhs = _c(rhs).reshape(-1)
+        lam = np.zeros_like(rhs) if lam0 is None else _c(lam0).reshape(-1).copy()
+        opts = _abi.RelaxOpts(float(prec), int(max_iter), 0 if lam0 is None else 1, 0.0, 9, 0)
+        res = _abi.RelaxResult()
+        check(lib().rh_relax_adjoint(self.h, _p(mats), _p(rhs), _p(lam), ctypes.byref(opts),
+                                     ctypes.byref(res)))
+        return res, lam.reshape(-1, 3)
 
 
 def current_field(cur, pts, device=0):

@@ -25,6 +25,16 @@ rh_status fail(rh_status st, const std::string& msg)
     return st;
 }
 
+// Makes `device` current and drops an error left pending on this thread by
+// earlier HIP calls outside this library, so that the post-launch checks
+// below report this library's launches only.
+hipError_t enter_device(int device)
+{
+    const hipError_t e = hipSetDevice(device);
+    (void)hipGetLastError();
+    return e;
+}
+
 #define RH_HIP(call)                                                                     \
     do {                                                                                 \
         hipError_t e_ = (call);                                                          \
@@ -133,7 +143,8 @@ struct rh_model {
 };
 
 struct rh_matrix {
-    rh_model* m = nullptr;
+    rh_model* m = nullptr;  // must outlive the matrix
+    int device = 0;
     rh_precision prec = RH_MIXED;
     int elem_bytes = 4;
     int64_t n3 = 0, ld = 0;
@@ -217,7 +228,7 @@ rh_status rh_model_create(const rh_model_desc* desc, int device, void* stream, r
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(RH_ERR_HIP, "no HIP device available");
     if (device < 0 || device >= ndev) return fail(RH_ERR_ARG, "device index out of range");
-    RH_HIP(hipSetDevice(device));
+    RH_HIP(enter_device(device));
     rh_model* m = new rh_model();
     m->device = device;
     m->stream = static_cast<hipStream_t>(stream);
@@ -286,7 +297,7 @@ rh_status rh_model_counts(const rh_model* m, int64_t* counts8)
 static rh_status assemble_into(rh_matrix* q)
 {
     rh_model* m = q->m;
-    RH_HIP(hipSetDevice(m->device));
+    RH_HIP(enter_device(m->device));
     Timer t("asm", m->stream);
     t.begin();
     RH_HIP(rh::launch_assemble(m->dev, q->d_Q, q->elem_bytes, q->ld, q->d_qdiag, m->stream));
@@ -300,9 +311,10 @@ rh_status rh_matrix_assemble(rh_model* m, rh_precision prec, rh_matrix** out)
     if (!m || !out) return fail(RH_ERR_ARG, "null argument");
     *out = nullptr;
     if (prec != RH_FP64 && prec != RH_MIXED) return fail(RH_ERR_ARG, "unknown precision");
-    RH_HIP(hipSetDevice(m->device));
+    RH_HIP(enter_device(m->device));
     rh_matrix* q = new rh_matrix();
     q->m = m;
+    q->device = m->device;
     q->prec = prec;
     q->elem_bytes = (prec == RH_FP64) ? 8 : 4;
     q->n3 = 3 * m->host.n;
@@ -338,7 +350,7 @@ rh_status rh_matrix_reassemble(rh_matrix* q)
 rh_status rh_matrix_destroy(rh_matrix* q)
 {
     if (!q) return RH_OK;
-    (void)hipSetDevice(q->m->device);
+    (void)hipSetDevice(q->device);
     dfree(q->d_Q);
     dfree(q->d_qdiag);
     delete q;
@@ -358,7 +370,7 @@ rh_status rh_matrix_export(const rh_matrix* q, int64_t row0, int64_t nrows, doub
     if (!q || !out) return fail(RH_ERR_ARG, "null argument");
     if (row0 < 0 || nrows < 0 || row0 + nrows > q->n3) return fail(RH_ERR_ARG, "row range out of bounds");
     rh_model* m = q->m;
-    RH_HIP(hipSetDevice(m->device));
+    RH_HIP(enter_device(m->device));
     const int64_t n3 = q->n3;
     std::vector<char> buf(size_t(q->ld) * size_t(q->elem_bytes));
     for (int64_t r = row0; r < row0 + nrows; r++) {
@@ -622,7 +634,7 @@ rh_status rh_relax_dev(rh_matrix* q, const double* d_mats, const double* d_hext,
     if (!q || !d_mats || !d_hext || !d_M || !d_H || !opts || !res) return fail(RH_ERR_ARG, "null argument");
     if (opts->method != 9 && opts->method != 0) return fail(RH_ERR_UNSUPPORTED, "relaxation method must be 9");
     rh_model* m = q->m;
-    RH_HIP(hipSetDevice(m->device));
+    RH_HIP(enter_device(m->device));
     hipStream_t s = m->stream;
     const int64_t n = m->host.n, n3 = q->n3;
     std::memset(res, 0, sizeof(*res));
@@ -703,7 +715,7 @@ rh_status rh_relax(rh_matrix* q, const double* mats, const double* hext, double*
 {
     if (!q || !mats || !hext || !M_inout || !H_out) return fail(RH_ERR_ARG, "null argument");
     rh_model* m = q->m;
-    RH_HIP(hipSetDevice(m->device));
+    RH_HIP(enter_device(m->device));
     const int64_t n = m->host.n;
     double *dm = nullptr, *dh = nullptr, *dM = nullptr, *dH = nullptr;
     hipStream_t s = m->stream;
@@ -758,7 +770,7 @@ rh_status rh_field_dev(rh_model* m, const double* d_M, const double* d_pts, int6
     if (!m || (!d_B && !d_H) || np < 0) return fail(RH_ERR_ARG, "bad argument");
     if (np > 0 && (!d_M || !d_pts)) return fail(RH_ERR_ARG, "null argument");
     if (prec != RH_FP64 && prec != RH_MIXED) return fail(RH_ERR_ARG, "unknown precision");
-    RH_HIP(hipSetDevice(m->device));
+    RH_HIP(enter_device(m->device));
     rh_status st = field_dev_impl(m, d_M, d_pts, np, d_B, d_H, prec == RH_MIXED);
     if (st != RH_OK) return st;
     RH_HIP(hipStreamSynchronize(m->stream));
@@ -770,7 +782,7 @@ rh_status rh_field(rh_model* m, const double* M, const double* pts, int64_t np, 
 {
     if (prec != RH_FP64 && prec != RH_MIXED) return fail(RH_ERR_ARG, "unknown precision");
     if (!m || !pts || np < 0 || (!B_out && !H_out)) return fail(RH_ERR_ARG, "bad argument");
-    RH_HIP(hipSetDevice(m->device));
+    RH_HIP(enter_device(m->device));
     const int64_t n = m->host.n;
     if (np == 0) return RH_OK;
     double *dM = nullptr, *dP = nullptr, *dB = nullptr, *dH = nullptr;
@@ -837,7 +849,7 @@ rh_status rh_field_adjoint_dev(rh_model* m, const double* d_M, const double* d_p
     if (!m || np < 0 || (np > 0 && (!d_pts || !d_gB))) return fail(RH_ERR_ARG, "bad argument");
     if ((d_gV || d_gX) && !d_M) return fail(RH_ERR_ARG, "vertex or point gradients need M");
     if (field != 0 && field != 1) return fail(RH_ERR_ARG, "field must be 0 (B) or 1 (H)");
-    RH_HIP(hipSetDevice(m->device));
+    RH_HIP(enter_device(m->device));
     hipStream_t s = m->stream;
     const rh::PackedModel& P = m->host;
     const int64_t n = P.n;
@@ -880,7 +892,7 @@ rh_status rh_field_adjoint(rh_model* m, const double* M, const double* pts, int6
                            const double* gB, int32_t field, double* gM, double* gV, double* gX)
 {
     if (!m || np < 0 || (np > 0 && (!pts || !gB))) return fail(RH_ERR_ARG, "bad argument");
-    RH_HIP(hipSetDevice(m->device));
+    RH_HIP(enter_device(m->device));
     hipStream_t s = m->stream;
     const int64_t n = m->host.n;
     const int64_t nvert = m->host.vert_off.empty() ? 0 : m->host.vert_off.back();
@@ -927,13 +939,23 @@ rh_status rh_relax_adjoint_dev(rh_matrix* q, const double* d_mats, const double*
 {
     if (!q || !d_mats || !d_rhs || !d_lam || !opts || !res) return fail(RH_ERR_ARG, "null argument");
     rh_model* m = q->m;
-    RH_HIP(hipSetDevice(m->device));
+    RH_HIP(enter_device(m->device));
     hipStream_t s = m->stream;
     const int64_t n = m->host.n, n3 = q->n3;
     std::memset(res, 0, sizeof(*res));
     if (n == 0) {
         res->converged = 1;
         return RH_OK;
+    }
+    {
+        // linear materials only: the transposed local solve needs K
+        std::vector<double> kinds(static_cast<size_t>(n), 0.0);
+        RH_HIP(hipMemcpy2DAsync(kinds.data(), sizeof(double), d_mats, 16 * sizeof(double),
+                                sizeof(double), size_t(n), hipMemcpyDeviceToHost, s));
+        RH_HIP(hipStreamSynchronize(s));
+        for (double k : kinds)
+            if (k != 0.0 && k != 1.0)
+                return fail(RH_ERR_UNSUPPORTED, "relaxation adjoint needs linear materials (kinds 0, 1)");
     }
     RelaxWork w;
     RH_HIP(dalloc(&w.x, size_t(q->ld)));
@@ -975,6 +997,40 @@ rh_status rh_relax_adjoint_dev(rh_matrix* q, const double* d_mats, const double*
                      res, &ap);
 }
 
+rh_status rh_relax_adjoint(rh_matrix* q, const double* mats, const double* rhs, double* lam,
+                           const rh_relax_opts* opts, rh_relax_result* res)
+{
+    if (!q || !mats || !rhs || !lam || !opts || !res) return fail(RH_ERR_ARG, "null argument");
+    rh_model* m = q->m;
+    RH_HIP(enter_device(m->device));
+    const int64_t n = m->host.n;
+    hipStream_t s = m->stream;
+    double *dm = nullptr, *dr = nullptr, *dl = nullptr;
+    auto cleanup = [&]() {
+        dfree(dm);
+        dfree(dr);
+        dfree(dl);
+    };
+    hipError_t e = dalloc(&dm, size_t(16 * n) + 1);
+    if (e == hipSuccess) e = dalloc(&dr, size_t(3 * n) + 1);
+    if (e == hipSuccess) e = dalloc(&dl, size_t(3 * n) + 1);
+    if (e == hipSuccess) e = hipMemcpyAsync(dm, mats, size_t(16 * n) * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dr, rhs, size_t(3 * n) * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dl, lam, size_t(3 * n) * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(RH_ERR_HIP, std::string("adjoint relax upload: ") + hipGetErrorString(e));
+    }
+    rh_status st = rh_relax_adjoint_dev(q, dm, dr, dl, opts, res);
+    if (st == RH_OK) {
+        e = hipMemcpyAsync(lam, dl, size_t(3 * n) * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = fail(RH_ERR_HIP, std::string("adjoint relax download: ") + hipGetErrorString(e));
+    }
+    cleanup();
+    return st;
+}
+
 static rh_status pack_currents(int64_t nsrc, const double* tm, const double* tv, const double* ts,
                                std::vector<double>& cp)
 {
@@ -1004,7 +1060,7 @@ rh_status rh_current_field_dev(int device, void* stream, int64_t nsrc, const int
     std::vector<double> cp;
     rh_status st = pack_currents(nsrc, tm, tv, ts, cp);
     if (st != RH_OK) return st;
-    RH_HIP(hipSetDevice(device));
+    RH_HIP(enter_device(device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     int32_t* dk = nullptr;
     double *dp = nullptr, *dc = nullptr;
@@ -1032,7 +1088,7 @@ rh_status rh_current_field(int device, void* stream, int64_t nsrc, const int32_t
 {
     if (np == 0 || nsrc == 0) return RH_OK;
     if (!pts || !B_acc) return fail(RH_ERR_ARG, "null argument");
-    RH_HIP(hipSetDevice(device));
+    RH_HIP(enter_device(device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     double *dP = nullptr, *dB = nullptr;
     hipError_t e = dalloc(&dP, size_t(3 * np));

@@ -95,9 +95,10 @@ rh_status orient_faces(const std::vector<Vec>& V, std::vector<Face>& F, std::str
             std::reverse(f.v.begin(), f.v.end());
         }
         f.n = n;
-        // convexity (Radia::Error106): every vertex on the inner side
+        // convexity (Radia::Error106): every vertex on the inner side, to the
+        // planarity tolerance
         for (const Vec& p : V) {
-            if (dt(sub(p, V[f.v[0]]), n) > tol + 1e-7 * size) {
+            if (dt(sub(p, V[f.v[0]]), n) > tol + 1e-6 * size) {
                 *err = "polyhedron is not convex";
                 return RH_ERR_GEOMETRY;
             }
@@ -309,6 +310,32 @@ rh_status pack_model(const rh_model_desc* d, PackedModel* out, std::string* err)
             P.counts[7] += 4;
         } else {
             P.kind[e] = EK_GEN;
+            // A face that is planar only to the input tolerance is replaced by its
+            // fan of triangles (each with its own normal): the field is then that
+            // of the closed triangulated surface, which is smooth in the vertex
+            // positions and equals the face form when the face is planar.
+            {
+                double size = 0;
+                for (const Vec& p : V) size = std::max(size, nrm(sub(p, V[0])));
+                std::vector<Face> F2;
+                for (const Face& f : F) {
+                    bool flat = true;
+                    for (size_t i = 1; i < f.v.size(); i++)
+                        flat = flat && std::fabs(dt(sub(V[f.v[i]], V[f.v[0]]), f.n)) <= 1e-12 * size;
+                    if (flat || f.v.size() == 3) {
+                        F2.push_back(f);
+                        continue;
+                    }
+                    for (size_t k = 1; k + 1 < f.v.size(); k++) {
+                        Face t;
+                        t.v = {f.v[0], f.v[k], f.v[k + 1]};
+                        Vec c = crs(sub(V[t.v[1]], V[t.v[0]]), sub(V[t.v[2]], V[t.v[0]]));
+                        t.n = scl(c, 1.0 / nrm(c));
+                        F2.push_back(t);
+                    }
+                }
+                F.swap(F2);
+            }
             size_t before = P.data.size();
             pack_gen(V, F, P.data);
             P.counts[4]++;

@@ -4,6 +4,7 @@
 #include <string>
 #include <vector>
 
+#include "rh_adjoint.hpp"
 #include "rh_current.hpp"
 #include "rh_eval.hpp"
 #include "rh_geom.hpp"
@@ -122,6 +123,56 @@ int rhc_current(int64_t ns, const int32_t* kind, const double* par, const double
         for (int k = 0; k < 3; k++) B[3 * p + k] = b[k];
     }
     return 0;
+}
+
+// H at np points from the triangulated adjoint geometry (value only, FP64,
+// no symmetry copies): the function whose derivatives the adjoint kernels take.
+int rhc_adj_field(const rh_model_desc* d, const double* M, const double* pts, int64_t np, double* H)
+{
+    rh::PackedModel P;
+    rh_status st = rh::pack_model(d, &P, &g_err);
+    if (st != RH_OK) return st;
+    for (int64_t p = 0; p < np; p++) {
+        double h[3] = {0, 0, 0};
+        for (int64_t j = 0; j < P.n; j++) {
+            double hj[3];
+            rh::adj_field_value(P.adj.data() + P.adj_off[j], pts + 3 * p, M + 3 * j, hj);
+            for (int k = 0; k < 3; k++) h[k] += rh::kInv4Pi * hj[k];
+        }
+        for (int k = 0; k < 3; k++) H[3 * p + k] = h[k];
+    }
+    return RH_OK;
+}
+
+// Host version of the vertex and point gradients for weights g (dL/dH), no
+// symmetry copies: gV [3 nvert], gX [3 np].
+int rhc_adj_grad(const rh_model_desc* d, const double* M, const double* pts, int64_t np,
+                 const double* g, double* gV, double* gX)
+{
+    rh::PackedModel P;
+    rh_status st = rh::pack_model(d, &P, &g_err);
+    if (st != RH_OK) return st;
+    const int64_t nvert = P.vert_off.back();
+    for (int64_t k = 0; k < 3 * nvert; k++) gV[k] = 0.0;
+    for (int64_t p = 0; p < np; p++) {
+        const double gl[3] = {rh::kInv4Pi * g[3 * p], rh::kInv4Pi * g[3 * p + 1], rh::kInv4Pi * g[3 * p + 2]};
+        double gx[3] = {0, 0, 0};
+        for (int64_t j = 0; j < P.n; j++) {
+            const double* rec = P.adj.data() + P.adj_off[j];
+            for (int t = 0; t < rh::adj_ntri(rec); t++)
+                for (int k = 0; k < 3; k++) {
+                    double o[3];
+                    int vid = 0;
+                    rh::adj_tri_corner_grad(rec, t, k, pts + 3 * p, M + 3 * j, gl, o, &vid);
+                    for (int a = 0; a < 3; a++) {
+                        gV[3 * (P.vert_off[j] + vid) + a] += o[a];
+                        gx[a] -= o[a];
+                    }
+                }
+        }
+        for (int a = 0; a < 3; a++) gX[3 * p + a] = gx[a];
+    }
+    return RH_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,158 @@
+"""The benchmark configurations other than the timed one, at full size, on the
+GPU: parity against the oracle where it finishes in seconds (sampled rows and
+points) and size-independent properties elsewhere.
+
+* iron cube: 512 blocks (ObjDivMag 8x8x8), MatSatIsoFrm steel, four-bar
+  ObjRecCur loop, B on 10,000 points of a tilted line through the centre.
+* large yoke: 40,000 tetrahedra, 120,000 x 120,000 matrix, FP32 storage
+  (57.6 GB) with FP64 refinement and FP64 storage (115 GB), prec 1e-5.
+* Halbach ring: 20,000 hexahedral polyhedra, MatLin(0.06/0.17, 1.2 T), B at
+  2,000,000 points in the bore, FP32 vs FP64 field kernels.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as OR
+from radiacuda_amd import _native as N
+from radiacuda_amd import objects as O
+from radiacuda_amd import rad
+from radiacuda_amd import workloads as W
+from tests import models
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _rand_off():
+    OR.set_randomization(False)
+
+
+def _ext_field(coil_ids, X):
+    items = list(O.walk(rad._reg, coil_ids))
+    return N.current_field(O.pack_currents(items), X)
+
+
+def test_iron_cube_full_size():
+    print("case: iron cube 512 blocks", flush=True)
+    top, iron, coil, pts = W.iron_cube()
+    P, items = models.packed(iron)
+    n = len(items)
+    mats = np.stack([o.mat.record(o.M) for (o, _k, _s) in items])
+    model = N.Model(P)
+    X = model.colloc()
+    hext = _ext_field(coil, X)
+    # FP64 matrix vs the oracle (every block)
+    Q = model.assemble(N.RH_FP64)
+    Qs = Q.export()
+    Qo, _jit = models.oracle_assemble(P)
+    assert models.block_rel(Qs, Qo, n) <= 1e-10
+    t = time.perf_counter()
+    res, M, _H = Q.relax(mats, hext, np.zeros((n, 3)), 1e-10, 2000)
+    t_solve = time.perf_counter() - t
+    assert res.converged
+    # FP64 fixed point of the oracle's system (Newton polish from the GPU result)
+    Mn = models.fp64_fixed_point(Qo, mats, hext, M0=M)
+    assert np.abs(M - Mn).max() <= 1e-7 * np.abs(Mn).max()
+    # field of the relaxed state at the 10,000 line points: elements + coil
+    t = time.perf_counter()
+    B, _Hf = model.field(M, pts)
+    t_fld = time.perf_counter() - t
+    Bo, _Ho, bad = models.oracle_field(P, M, pts)
+    assert not bad.any()
+    assert np.abs(B - Bo).max() <= 1e-10 * np.abs(Bo).max()
+    # drop-in API end to end (mixed storage + FP64 refinement) at the
+    # configuration's prec 1e-4, against the FP64-storage solve stopped at the
+    # same prec: the same iteration on a matrix that differs by FP32 rounding
+    r = rad.Solve(top, 1e-4, 1000)
+    Bapi = np.array([rad.Fld(top, "b", list(p)) for p in pts[::500]])
+    r4, M4, _H4 = Q.relax(mats, hext, np.zeros((n, 3)), 1e-4, 1000)
+    B4 = model.field(M4, pts[::500])[0] + _ext_field(coil, pts[::500])
+    Bconv = B[::500] + _ext_field(coil, pts[::500])
+    scale = np.abs(B4).max()
+    print("iron cube: solve %.3f s (%d it), field %.3f s; Solve(1e-4) %s; API vs FP64 at 1e-4 %.2e; "
+          "at 1e-4 vs converged (1e-10) %.2e"
+          % (t_solve, res.iters, t_fld, r, np.abs(Bapi - B4).max() / scale,
+             np.abs(B4 - Bconv).max() / scale), flush=True)
+    assert r[0] <= 1e-4
+    assert np.abs(Bapi - B4).max() <= 1e-3 * scale
+    Q.close()
+    model.close()
+
+
+def test_large_yoke_120k_matrix():
+    print("case: large yoke 40000 tets", flush=True)
+    top, iron, coil, info = W.c_yoke(ny=50)
+    P, items = models.packed(iron)
+    n = len(items)
+    assert n == 40000
+    mats = np.stack([o.mat.record(o.M) for (o, _k, _s) in items])
+    model = N.Model(P)
+    X = model.colloc()
+    hext = N.current_field(O.pack_currents(list(O.walk(rad._reg, coil))), X)
+    rows = [0, 12345, 39999]
+    out = {}
+    for prec in (N.RH_MIXED, N.RH_FP64):
+        t = time.perf_counter()
+        Q = model.assemble(prec)
+        t_asm = time.perf_counter() - t
+        R = np.concatenate([Q.export(3 * i, 3) for i in rows])
+        t = time.perf_counter()
+        res, M, _H = Q.relax(mats, hext, np.zeros((n, 3)), 1e-5, 1000)
+        t_solve = time.perf_counter() - t
+        print("prec %d: assembly %.3f s, solve %.3f s, %d iterations, misfit %.2e, refined %d"
+              % (prec, t_asm, t_solve, res.iters, res.misfit, res.refined), flush=True)
+        assert res.converged and res.misfit <= 1e-5
+        out[prec] = (R, M)
+        Q.close()
+    # sampled rows against the oracle
+    om = OR.prepare_model(P)
+    Ro = np.concatenate([OR.assemble(om, rows=(i, i + 1)) for i in rows])
+    R64, M64 = out[N.RH_FP64]
+    R32, M32 = out[N.RH_MIXED]
+    scale = np.abs(Ro).max()
+    assert np.abs(R64 - Ro).max() <= 1e-10 * scale
+    assert np.abs(R32 - Ro).max() <= 1e-6 * scale
+    # both solves reach the same state to within the stopping tolerance
+    dM = np.abs(M32 - M64).max() / np.abs(M64).max()
+    print("large yoke: refined mixed vs FP64 solve, max |dM|/max|M| = %.2e" % dM, flush=True)
+    assert dM <= 1e-3
+    # field at sampled gap points against the oracle
+    pts = W.gap_grid(info, 20)[::97]
+    B, _H = model.field(M64, pts)
+    Bo, _Ho, _bad = models.oracle_field(P, M64, pts)
+    assert np.abs(B - Bo).max() <= 1e-9 * np.abs(Bo).max()
+    model.close()
+
+
+def test_halbach_field_2m_points():
+    print("case: halbach 20000 hexahedra, 2M points", flush=True)
+    top, pts = W.halbach()
+    k = rad.RlxPre(top)
+    t = time.perf_counter()
+    r = rad.RlxAuto(k, 1e-8, 1000)
+    t_solve = time.perf_counter() - t
+    assert r[0] <= 1e-8
+    it = rad._reg[k]
+    items = it.items
+    P = it.packed
+    M = np.stack([o.M for (o, _k, _s) in items])
+    model = it.model
+    times = {}
+    Bs = {}
+    for prec in (N.RH_FP64, N.RH_MIXED):
+        t = time.perf_counter()
+        B, _H = model.field(M, pts, precision=prec)
+        times[prec] = time.perf_counter() - t
+        Bs[prec] = B
+    scale = np.abs(Bs[N.RH_FP64]).max()
+    err32 = np.abs(Bs[N.RH_MIXED] - Bs[N.RH_FP64]).max() / scale
+    print("halbach: solve %.3f s; field FP64 %.2f s, FP32 %.2f s (2M points); FP32 vs FP64 %.2e"
+          % (t_solve, times[N.RH_FP64], times[N.RH_MIXED], err32), flush=True)
+    assert err32 <= 1e-5
+    rng = np.random.default_rng(3)
+    samp = rng.choice(len(pts), 48, replace=False)
+    Bo, _Ho, _bad = models.oracle_field(P, M, pts[samp])
+    assert np.abs(Bs[N.RH_FP64][samp] - Bo).max() <= 1e-10 * np.abs(Bo).max()
+    rad.UtiDel(k)
