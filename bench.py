@@ -25,11 +25,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Q-blocks/s, rad.Solve s, Fld points/s: mixed vs FP64 GPU at stated error"
-# Declared algorithmic FLOPs per (observation point, tetrahedron) pair of the
-# integrals as written in radiacuda_amd/csrc/rh_math.hpp, counting + - * / sqrt
-# as 1, log as 20 and atan2 as 30 (4 sqrt, 6 log, 6 div, 4 atan2 per pair).
+# Declared algorithmic FLOPs per observation point, counting + - * / sqrt as 1,
+# log as 20 and atan2 as 30, from the integrals as written:
+# - per-element form (radiacuda_amd/csrc/rh_math.hpp), per (point, tetrahedron)
+#   pair: 4 sqrt, 6 log, 6 div, 4 atan2;
+# - mesh form (radiacuda_amd/csrc/rh_mesh.hpp), per (point, unique edge):
+#   2 sqrt, 1 div, 1 log; per (point, unique triangle): 3 sqrt, 1 atan2.
 FLOP_TET_FIELD = 821
 FLOP_TET_BLOCK = 794
+FLOP_MESH_EDGE = 74
+FLOP_MESH_TRI = 93
 PEAK_FP64_VALU = 78.6   # TFLOP/s, MI355X vector FP64
 PEAK_FP32_VALU = 157.3  # TFLOP/s, MI355X vector FP32
 PEAK_HBM = 8000.0       # GB/s
@@ -227,7 +232,14 @@ def main():
 
     # roofline of the dominant kernel (field evaluation, FP64 integrals)
     fld_ms = r64["kernel_ms"]["field"]
-    flops = float(npts) * n * FLOP_TET_FIELD
+    if model.field_form() == 2:
+        ne, nf, _ni = model.mesh_counts()
+        flops = float(npts) * (ne * FLOP_MESH_EDGE + nf * FLOP_MESH_TRI)
+        fld_kernel = ("k_field_mesh<double> (FP64 integrals over %d unique edges and %d unique "
+                      "triangles)" % (ne, nf))
+    else:
+        flops = float(npts) * n * FLOP_TET_FIELD
+        fld_kernel = "k_field_tet<double> (FP64 integrals)"
     achieved = flops / (fld_ms * 1e-3) / 1e12 if fld_ms > 0 else 0.0
     asm_ms = r64["kernel_ms"]["asm"]
     mv_ms = r64["kernel_ms"]["matvec"]
@@ -264,7 +276,7 @@ def main():
                   "fld_kernel_ms": rmx["kernel_ms"]["field"], "end_to_end_ms": rmx["ms_per_step"],
                   "speedup_vs_fp64": ms / rmx["ms_per_step"],
                   "B_err_vs_fp64": err_mixed, "M_err_vs_fp64": err_M},
-        "roofline": {"bound": "valu", "kernel": "k_field<double> (FP64 integrals)",
+        "roofline": {"bound": "valu", "kernel": fld_kernel,
                      "achieved": achieved, "peak": PEAK_FP64_VALU, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_VALU, "traffic": None,
                      "flops_per_launch": flops, "avg_launch_ms": fld_ms},

@@ -78,6 +78,19 @@ rh_status rh_model_colloc(const rh_model *m, double *colloc_out);
 /* Counts: [0] elements, [1] copies, [2] tetrahedra, [3] hexahedra,
  * [4] other polyhedra, [5] blocks, [6] edges, [7] faces. */
 rh_status rh_model_counts(const rh_model *m, int64_t *counts8);
+/* Field-sweep form of the polyhedra.  A model without transform copies whose
+ * polyhedra have at most 8 faces each is also held as a conforming mesh: the
+ * field sweep then evaluates each unique edge and face once with the net
+ * weight of the elements sharing it, instead of every element separately (the
+ * same sum, regrouped).  form 0: automatic (mesh when held), 1: per element.
+ * No counterpart in the reference, which evaluates every polygon of every
+ * element (cpp/src/core/radgpu_fld.cu:381-526). */
+rh_status rh_model_set_field_form(rh_model *m, int32_t form);
+/* The form field sweeps use: 1 per element, 2 mesh. */
+rh_status rh_model_field_form(const rh_model *m, int32_t *form);
+/* Mesh form sizes: [0] unique edges, [1] unique faces (triangles and
+ * quadrilaterals), [2] polyhedra; zeros when the model holds no mesh form. */
+rh_status rh_model_mesh_counts(const rh_model *m, int64_t *counts3);
 
 /* ------------------------------------------------------------------------ */
 /* Interaction matrix                                                         */

@@ -65,6 +65,9 @@ def lib():
     L.rh_current_field_dev.argtypes = [ctypes.c_int, _vp, ctypes.c_int64, _i32, _d, _d, _d, _d,
                                        _vp, ctypes.c_int64, _vp]
     L.rh_model_vertex_count.argtypes = [_vp, _i64]
+    L.rh_model_set_field_form.argtypes = [_vp, ctypes.c_int32]
+    L.rh_model_field_form.argtypes = [_vp, _i32]
+    L.rh_model_mesh_counts.argtypes = [_vp, _i64]
     L.rh_field_adjoint.argtypes = [_vp, _d, _d, ctypes.c_int64, _d, ctypes.c_int32, _d, _d, _d]
     L.rh_field_adjoint_dev.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, ctypes.c_int32, _vp,
                                        _vp, _vp]
@@ -80,7 +83,8 @@ def lib():
                  "rh_field", "rh_field_dev", "rh_current_field", "rh_current_field_dev",
                  "rh_set_timing", "rh_kernel_time", "rh_model_vertex_count",
                  "rh_field_adjoint", "rh_field_adjoint_dev", "rh_relax_adjoint_dev",
-                 "rh_relax_adjoint"]:
+                 "rh_relax_adjoint", "rh_model_set_field_form", "rh_model_field_form",
+                 "rh_model_mesh_counts"]:
         getattr(L, name).restype = st
     if L.rh_device_count() < 1:
         raise RadHipError("no HIP device visible; the HIP path has no CPU fallback")
@@ -158,6 +162,21 @@ class Model:
 
     def assemble(self, prec=RH_MIXED):
         return Matrix(self, prec)
+
+    def set_field_form(self, form):
+        """0: automatic (conforming-mesh form when held), 1: per element."""
+        check(lib().rh_model_set_field_form(self.h, int(form)))
+
+    def field_form(self):
+        f = np.zeros(1, np.int32)
+        check(lib().rh_model_field_form(self.h, f.ctypes.data_as(_i32)))
+        return int(f[0])
+
+    def mesh_counts(self):
+        """(unique edges, unique faces, polyhedra) of the mesh form, zeros if none."""
+        c = np.zeros(3, np.int64)
+        check(lib().rh_model_mesh_counts(self.h, c.ctypes.data_as(_i64)))
+        return tuple(int(v) for v in c)
 
     def vertex_count(self):
         nv = np.zeros(1, np.int64)

@@ -8,12 +8,14 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/radhip.h"
 #include "rh_geom.hpp"
 #include "rh_kernels.hpp"
 #include "rh_math.hpp"
+#include "rh_mesh.hpp"
 
 namespace {
 
@@ -137,6 +139,8 @@ struct rh_model {
     int64_t* d_vert_off = nullptr;
     int64_t* d_slices = nullptr;
     int64_t n_slices = 0;
+    // conforming-mesh field form
+    std::vector<void*> mesh_bufs;
     // growable workspace for the field entry points
     double* d_work = nullptr;
     size_t work_cap = 0;
@@ -182,6 +186,7 @@ void free_model(rh_model* m)
     dfree(m->d_adj_off);
     dfree(m->d_vert_off);
     dfree(m->d_slices);
+    for (void* b : m->mesh_bufs) dfree(b);
     dfree(m->d_work);
     delete m;
 }
@@ -269,7 +274,73 @@ rh_status rh_model_create(const rh_model_desc* desc, int device, void* stream, r
     for (int k = 0; k < 4; k++) d.koff[k] = P.kind_off[k];
     d.sym = P.sym;
     d.rowt_identity = P.rowt_identity;
+    if (P.mesh.active) {
+        const rh::MeshForm& F = P.mesh;
+        rh::MeshDev& md = d.mesh;
+        auto up = [&](auto** dst, const auto& v) {
+            using T = typename std::remove_reference<decltype(v)>::type::value_type;
+            T* ptr = nullptr;
+            hipError_t err = upload(&ptr, v, s);
+            m->mesh_bufs.push_back(ptr);
+            *dst = ptr;
+            return err;
+        };
+        md.ne = F.n_edges();
+        md.nf = F.n_faces();
+        md.ni = F.n_in();
+        md.ntiles = int64_t(F.tile_box.size() / 6);
+        e = up(&md.edge_geo, F.edge_geo);
+        if (e == hipSuccess) e = up(&md.edge_inc_off, F.edge_inc_off);
+        if (e == hipSuccess) e = up(&md.edge_inc_elem, F.edge_inc_elem);
+        if (e == hipSuccess) e = up(&md.edge_inc_w, F.edge_inc_w);
+        if (e == hipSuccess) e = up(&md.face_geo, F.face_geo);
+        if (e == hipSuccess) e = up(&md.face_inc_off, F.face_inc_off);
+        if (e == hipSuccess) e = up(&md.face_inc_elem, F.face_inc_elem);
+        if (e == hipSuccess) e = up(&md.face_inc_sign, F.face_inc_sign);
+        if (e == hipSuccess) e = up(&md.in_elem, F.in_elem);
+        if (e == hipSuccess) e = up(&md.in_geo, F.in_geo);
+        if (e == hipSuccess) e = up(&md.tile_box, F.tile_box);
+        double* ch = nullptr;
+        if (e == hipSuccess) e = dalloc(&ch, size_t(rh::kMeshEdge * md.ne + rh::kMeshFace * md.nf +
+                                                    rh::kMeshIn * md.ni) + 2);
+        if (ch) m->mesh_bufs.push_back(ch);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            std::string msg = std::string("mesh upload: ") + hipGetErrorString(e);
+            free_model(m);
+            return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_HIP, msg);
+        }
+        md.ch_edge = ch;
+        md.ch_face = ch + rh::kMeshEdge * md.ne;
+        md.ch_in = md.ch_face + rh::kMeshFace * md.nf;
+        md.active = true;
+        d.use_mesh = true;
+    }
     *out = m;
+    return RH_OK;
+}
+
+rh_status rh_model_set_field_form(rh_model* m, int32_t form)
+{
+    if (!m) return fail(RH_ERR_ARG, "null argument");
+    if (form != 0 && form != 1) return fail(RH_ERR_ARG, "form must be 0 (automatic) or 1 (per element)");
+    m->dev.use_mesh = (form == 0) && m->dev.mesh.active;
+    return RH_OK;
+}
+
+rh_status rh_model_mesh_counts(const rh_model* m, int64_t* counts3)
+{
+    if (!m || !counts3) return fail(RH_ERR_ARG, "null argument");
+    counts3[0] = m->dev.mesh.ne;
+    counts3[1] = m->dev.mesh.nf;
+    counts3[2] = m->dev.mesh.ni;
+    return RH_OK;
+}
+
+rh_status rh_model_field_form(const rh_model* m, int32_t* form)
+{
+    if (!m || !form) return fail(RH_ERR_ARG, "null argument");
+    *form = (m->dev.use_mesh && m->dev.mesh.active && !m->dev.sym) ? 2 : 1;
     return RH_OK;
 }
 

@@ -15,6 +15,7 @@
 #include <utility>
 
 #include "rh_math.hpp"
+#include "rh_mesh.hpp"
 
 namespace rh {
 
@@ -388,7 +389,210 @@ rh_status pack_model(const rh_model_desc* d, PackedModel* out, std::string* err)
             if (P.rowt[9 * e + k] != ((k % 4 == 0) ? 1.0 : 0.0)) P.rowt_identity = false;
         }
     }
+    build_mesh_form(P);
     return RH_OK;
+}
+
+namespace {
+using VKey = std::array<uint64_t, 3>;
+
+VKey vkey(const double* v)
+{
+    VKey k;
+    std::memcpy(k.data(), v, 3 * sizeof(double));
+    return k;
+}
+
+// One polyhedron seen through its packed record: vertices, edges (vertex
+// pair and W6), faces (vertex loop CCW about the outward normal, n).
+struct ElemView {
+    int nv = 0;
+    const double* verts = nullptr;
+    std::vector<std::array<int, 2>> edges;
+    std::vector<const double*> edge_w;
+    std::vector<std::vector<int>> faces;
+    std::vector<const double*> face_n;
+};
+
+ElemView view_elem(const PackedModel& P, int64_t j)
+{
+    ElemView v;
+    const double* rec = P.data.data() + P.off[j];
+    if (P.kind[j] == EK_TET) {
+        v.nv = 4;
+        v.verts = rec;
+        for (int e = 0; e < 6; e++) {
+            v.edges.push_back({kTetEdgeA[e], kTetEdgeB[e]});
+            v.edge_w.push_back(rec + 12 + kTetEdge * e + 4);
+        }
+        for (int f = 0; f < 4; f++) {
+            v.faces.push_back({kTetFace[f][0], kTetFace[f][1], kTetFace[f][2]});
+            v.face_n.push_back(rec + 12 + 6 * kTetEdge + kTetFaceRec * f);
+        }
+        return v;
+    }
+    const int nv = int(rec[0]), ne = int(rec[1]), nf = int(rec[2]);
+    v.nv = nv;
+    v.verts = rec + kGenHdr;
+    const double* E = v.verts + 3 * nv;
+    const double* F = E + kGenEdge * ne;
+    const double* fv = F + kGenFace * nf;
+    for (int e = 0; e < ne; e++) {
+        v.edges.push_back({int(E[kGenEdge * e]), int(E[kGenEdge * e + 1])});
+        v.edge_w.push_back(E + kGenEdge * e + 6);
+    }
+    for (int f = 0; f < nf; f++) {
+        const double* fr = F + kGenFace * f;
+        std::vector<int> loop;
+        for (int k = 0; k < int(fr[4]); k++) loop.push_back(int(fv[int(fr[3]) + k]));
+        v.faces.push_back(loop);
+        v.face_n.push_back(fr);
+    }
+    return v;
+}
+}  // namespace
+
+void build_mesh_form(PackedModel& P)
+{
+    MeshForm& M = P.mesh;
+    M = MeshForm();
+    if (P.sym || P.kind_off[3] - P.kind_off[1] < 2) return;
+    std::vector<int64_t> elems(P.kind_idx.begin() + P.kind_off[1], P.kind_idx.begin() + P.kind_off[3]);
+    std::vector<ElemView> views;
+    for (int64_t j : elems) {
+        views.push_back(view_elem(P, j));
+        if (int(views.back().faces.size()) > kMeshPlanes) return;
+    }
+    std::map<VKey, int32_t> vid;
+    std::vector<const double*> vpos;
+    std::map<std::pair<int32_t, int32_t>, int64_t> eid;
+    std::vector<std::vector<std::pair<int32_t, const double*>>> einc;
+    std::map<std::vector<int32_t>, int64_t> fid;
+    std::vector<std::vector<std::pair<int32_t, double>>> finc;
+    std::vector<std::vector<int32_t>> floop;  // unique-vertex loop of the first incidence
+    std::vector<const double*> fnorm;
+    for (size_t q = 0; q < elems.size(); q++) {
+        const int32_t j = int32_t(elems[q]);
+        const ElemView& v = views[q];
+        std::vector<int32_t> u(size_t(v.nv));
+        for (int k = 0; k < v.nv; k++) {
+            auto it = vid.find(vkey(v.verts + 3 * k));
+            if (it == vid.end()) {
+                it = vid.emplace(vkey(v.verts + 3 * k), int32_t(vpos.size())).first;
+                vpos.push_back(v.verts + 3 * k);
+            }
+            u[size_t(k)] = it->second;
+        }
+        for (size_t e = 0; e < v.edges.size(); e++) {
+            int32_t a = u[size_t(v.edges[e][0])], b = u[size_t(v.edges[e][1])];
+            auto key = std::make_pair(std::min(a, b), std::max(a, b));
+            auto it = eid.find(key);
+            if (it == eid.end()) {
+                it = eid.emplace(key, int64_t(einc.size())).first;
+                einc.emplace_back();
+            }
+            einc[size_t(it->second)].push_back({j, v.edge_w[e]});
+        }
+        for (size_t f = 0; f < v.faces.size(); f++) {
+            const std::vector<int>& loop = v.faces[f];
+            // polygons of more than 4 vertices become fan pieces of at most 4
+            for (size_t s = 0; s + 2 < loop.size(); s += 2) {
+                std::vector<int32_t> piece{u[size_t(loop[0])], u[size_t(loop[s + 1])],
+                                           u[size_t(loop[s + 2])]};
+                if (s + 3 < loop.size()) piece.push_back(u[size_t(loop[s + 3])]);
+                std::vector<int32_t> key = piece;
+                std::sort(key.begin(), key.end());
+                auto it = fid.find(key);
+                double sign = 1.0;
+                if (it == fid.end()) {
+                    it = fid.emplace(key, int64_t(finc.size())).first;
+                    finc.emplace_back();
+                    floop.push_back(piece);
+                    fnorm.push_back(v.face_n[f]);
+                } else {
+                    const double* n0 = fnorm[size_t(it->second)];
+                    const double* n1 = v.face_n[f];
+                    sign = (n0[0] * n1[0] + n0[1] * n1[1] + n0[2] * n1[2] > 0) ? 1.0 : -1.0;
+                }
+                finc[size_t(it->second)].push_back({j, sign});
+            }
+        }
+    }
+    // edges
+    M.edge_inc_off.push_back(0);
+    std::vector<std::pair<int32_t, int32_t>> ekeys(einc.size());
+    for (const auto& kv : eid) ekeys[size_t(kv.second)] = kv.first;
+    for (size_t e = 0; e < einc.size(); e++) {
+        const double* a = vpos[size_t(ekeys[e].first)];
+        const double* b = vpos[size_t(ekeys[e].second)];
+        double t[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double len = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        for (int k = 0; k < 3; k++) t[k] /= len;
+        for (int k = 0; k < 3; k++) M.edge_geo.push_back(a[k]);
+        for (int k = 0; k < 3; k++) M.edge_geo.push_back(b[k]);
+        for (int k = 0; k < 3; k++) M.edge_geo.push_back(t[k]);
+        M.edge_geo.push_back(len);
+        for (const auto& inc : einc[e]) {
+            M.edge_inc_elem.push_back(inc.first);
+            for (int k = 0; k < 6; k++) M.edge_inc_w.push_back(inc.second[k]);
+        }
+        M.edge_inc_off.push_back(int64_t(M.edge_inc_elem.size()));
+    }
+    // faces
+    M.face_inc_off.push_back(0);
+    for (size_t f = 0; f < finc.size(); f++) {
+        const std::vector<int32_t>& loop = floop[f];
+        const double* n = fnorm[f];
+        double vv[4][3] = {{0}};
+        for (size_t k = 0; k < loop.size(); k++)
+            for (int c = 0; c < 3; c++) vv[k][c] = vpos[size_t(loop[k])][c];
+        auto area2 = [&](int i0, int i1, int i2) {
+            const double e1[3] = {vv[i1][0] - vv[i0][0], vv[i1][1] - vv[i0][1], vv[i1][2] - vv[i0][2]};
+            const double e2[3] = {vv[i2][0] - vv[i0][0], vv[i2][1] - vv[i0][1], vv[i2][2] - vv[i0][2]};
+            const double c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                                 e1[0] * e2[1] - e1[1] * e2[0]};
+            return std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        };
+        for (int k = 0; k < 4; k++)
+            for (int c = 0; c < 3; c++) M.face_geo.push_back(vv[k][c]);
+        for (int c = 0; c < 3; c++) M.face_geo.push_back(n[c]);
+        M.face_geo.push_back(area2(0, 1, 2));
+        M.face_geo.push_back(loop.size() > 3 ? area2(0, 2, 3) : 0.0);
+        M.face_geo.push_back(double(loop.size()));
+        for (const auto& inc : finc[f]) {
+            M.face_inc_elem.push_back(inc.first);
+            M.face_inc_sign.push_back(inc.second);
+        }
+        M.face_inc_off.push_back(int64_t(M.face_inc_elem.size()));
+    }
+    // inside records and tile boxes
+    for (size_t q = 0; q < elems.size(); q++) {
+        const ElemView& v = views[q];
+        M.in_elem.push_back(int32_t(elems[q]));
+        M.in_geo.push_back(double(v.faces.size()));
+        for (int f = 0; f < kMeshPlanes; f++) {
+            if (f < int(v.faces.size())) {
+                const double* n = v.face_n[size_t(f)];
+                const double* p0 = v.verts + 3 * v.faces[size_t(f)][0];
+                for (int c = 0; c < 3; c++) M.in_geo.push_back(n[c]);
+                M.in_geo.push_back(n[0] * p0[0] + n[1] * p0[1] + n[2] * p0[2]);
+            } else {
+                for (int c = 0; c < 4; c++) M.in_geo.push_back(0.0);
+            }
+        }
+    }
+    for (size_t t0 = 0; t0 < elems.size(); t0 += kMeshTile) {
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t q = t0; q < std::min(elems.size(), t0 + kMeshTile); q++)
+            for (int k = 0; k < views[q].nv; k++)
+                for (int c = 0; c < 3; c++) {
+                    lo[c] = std::min(lo[c], views[q].verts[3 * k + c]);
+                    hi[c] = std::max(hi[c], views[q].verts[3 * k + c]);
+                }
+        for (int c = 0; c < 3; c++) M.tile_box.push_back(lo[c]);
+        for (int c = 0; c < 3; c++) M.tile_box.push_back(hi[c]);
+    }
+    M.active = true;
 }
 
 }  // namespace rh

@@ -13,6 +13,29 @@ namespace rh {
 // Per transform copy: Minv (9), V (3), F = s*M (9), pad (3).
 constexpr int kCopySize = 24;
 
+// Conforming-mesh form of the polyhedra for the field sweep (rh_mesh.hpp):
+// unique edges and faces with their element incidences, and per-element
+// plane sets for the inside test.  Built when the model has no transform
+// copies and every polyhedron has at most kMeshPlanes faces.
+struct MeshForm {
+    bool active = false;
+    std::vector<double> edge_geo;        // kMeshEdgeGeo per unique edge
+    std::vector<int64_t> edge_inc_off;   // [E+1]
+    std::vector<int32_t> edge_inc_elem;  // element of each incidence
+    std::vector<double> edge_inc_w;      // W_e of that element, 6 per incidence
+    std::vector<double> face_geo;        // kMeshFaceGeo per unique face
+    std::vector<int64_t> face_inc_off;   // [F+1]
+    std::vector<int32_t> face_inc_elem;
+    std::vector<double> face_inc_sign;   // +1: the element's outward normal is n_f
+    std::vector<int32_t> in_elem;        // element of each inside record
+    std::vector<double> in_geo;          // 1 + 4 kMeshPlanes per element: count, planes
+    std::vector<double> tile_box;        // 6 per tile of kMeshTile inside records
+    int64_t n_edges() const { return int64_t(edge_inc_off.empty() ? 0 : edge_inc_off.size() - 1); }
+    int64_t n_faces() const { return int64_t(face_inc_off.empty() ? 0 : face_inc_off.size() - 1); }
+    int64_t n_in() const { return int64_t(in_elem.size()); }
+};
+constexpr int kMeshTile = 32;
+
 struct PackedModel {
     int64_t n = 0;
     std::vector<int32_t> kind;      // [n] EK_*
@@ -33,7 +56,12 @@ struct PackedModel {
     std::vector<int64_t> adj_off;   // [n+1]
     std::vector<double> adj;
     std::vector<int64_t> vert_off;  // [n+1] first vertex of each element in the caller's list
+    MeshForm mesh;
 };
+
+// Builds P.mesh from the packed records (no-op when the model does not
+// qualify).
+void build_mesh_form(PackedModel& P);
 
 // Returns RH_OK or an error status with a message in *err.
 rh_status pack_model(const rh_model_desc* d, PackedModel* out, std::string* err);

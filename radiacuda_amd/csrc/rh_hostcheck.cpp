@@ -8,6 +8,7 @@
 #include "rh_current.hpp"
 #include "rh_eval.hpp"
 #include "rh_geom.hpp"
+#include "rh_mesh.hpp"
 
 namespace {
 thread_local std::string g_err;
@@ -171,6 +172,94 @@ int rhc_adj_grad(const rh_model_desc* d, const double* M, const double* pts, int
                 }
         }
         for (int a = 0; a < 3; a++) gX[3 * p + a] = gx[a];
+    }
+    return RH_OK;
+}
+
+// Field from the conforming-mesh form (host build of the device sweep):
+// returns 2 when the model has no mesh form.  counts3 (may be null) gets the
+// numbers of unique edges, unique faces and inside records.
+int rhc_field_mesh(const rh_model_desc* d, const double* M, const double* pts, int64_t np,
+                   int32_t fp32, double* B, double* H, int64_t* counts3)
+{
+    rh::PackedModel P;
+    rh_status st = rh::pack_model(d, &P, &g_err);
+    if (st != RH_OK) return st;
+    const rh::MeshForm& F = P.mesh;
+    if (!F.active) return 2;
+    const int64_t ne = F.n_edges(), nf = F.n_faces(), ni = F.n_in();
+    if (counts3) {
+        counts3[0] = ne;
+        counts3[1] = nf;
+        counts3[2] = ni;
+    }
+    std::vector<double> ce(size_t(rh::kMeshEdge * ne)), cf(size_t(rh::kMeshFace * nf)),
+        ci(size_t(rh::kMeshIn * ni));
+    for (int64_t i = 0; i < ne; i++) {
+        double w[3] = {0, 0, 0};
+        for (int64_t k = F.edge_inc_off[i]; k < F.edge_inc_off[i + 1]; k++) {
+            const double* W = F.edge_inc_w.data() + 6 * k;
+            const double* m = M + 3 * F.edge_inc_elem[k];
+            w[0] += W[0] * m[0] + W[3] * m[1] + W[4] * m[2];
+            w[1] += W[3] * m[0] + W[1] * m[1] + W[5] * m[2];
+            w[2] += W[4] * m[0] + W[5] * m[1] + W[2] * m[2];
+        }
+        double* o = ce.data() + rh::kMeshEdge * i;
+        for (int k = 0; k < rh::kMeshEdgeGeo; k++) o[k] = F.edge_geo[rh::kMeshEdgeGeo * i + k];
+        o[10] = w[0];
+        o[11] = w[1];
+        o[12] = w[2];
+    }
+    for (int64_t i = 0; i < nf; i++) {
+        const double* g = F.face_geo.data() + rh::kMeshFaceGeo * i;
+        double s = 0;
+        for (int64_t k = F.face_inc_off[i]; k < F.face_inc_off[i + 1]; k++) {
+            const double* m = M + 3 * F.face_inc_elem[k];
+            s += F.face_inc_sign[k] * (g[12] * m[0] + g[13] * m[1] + g[14] * m[2]);
+        }
+        double* o = cf.data() + rh::kMeshFace * i;
+        for (int k = 0; k < rh::kMeshFaceGeo; k++) o[k] = g[k];
+        o[18] = s * g[12];
+        o[19] = s * g[13];
+        o[20] = s * g[14];
+    }
+    for (int64_t i = 0; i < ni; i++) {
+        const double* g = F.in_geo.data() + (1 + 4 * rh::kMeshPlanes) * i;
+        double* o = ci.data() + rh::kMeshIn * i;
+        o[0] = g[0];
+        for (int k = 0; k < 3; k++) o[1 + k] = M[3 * F.in_elem[i] + k];
+        for (int k = 0; k < 4 * rh::kMeshPlanes; k++) o[4 + k] = g[1 + k];
+    }
+    // blocks through the per-element path
+    rh::ModelView mv = view(P);
+    for (int64_t p = 0; p < np; p++) {
+        const double* x = pts + 3 * p;
+        double h[3] = {0, 0, 0}, b[3] = {0, 0, 0}, mi[3] = {0, 0, 0};
+        for (int64_t i = 0; i < ne; i++) {
+            const double* r = ce.data() + rh::kMeshEdge * i;
+            const double L = fp32 ? rh::mesh_edge_L<float>(r, x) : rh::mesh_edge_L<double>(r, x);
+            for (int k = 0; k < 3; k++) h[k] += L * r[10 + k];
+        }
+        for (int64_t i = 0; i < nf; i++) {
+            const double* r = cf.data() + rh::kMeshFace * i;
+            const double om = fp32 ? rh::mesh_face_omega<float>(r, x) : rh::mesh_face_omega<double>(r, x);
+            for (int k = 0; k < 3; k++) h[k] -= om * r[18 + k];
+        }
+        for (int64_t i = 0; i < ni; i++) {
+            const double* r = ci.data() + rh::kMeshIn * i;
+            if (rh::mesh_inside(r, x))
+                for (int k = 0; k < 3; k++) mi[k] += r[1 + k];
+        }
+        for (int k = 0; k < 3; k++) {
+            h[k] *= rh::kInv4Pi;
+            b[k] = h[k] + mi[k];
+        }
+        double hb[3] = {0, 0, 0}, bb[3] = {0, 0, 0};
+        rh::field_add<double, true, 0>(mv, M, P.kind_idx.data(), P.kind_off[0], P.kind_off[1], x, bb, hb);
+        for (int k = 0; k < 3; k++) {
+            B[3 * p + k] = b[k] + bb[k];
+            H[3 * p + k] = h[k] + hb[k];
+        }
     }
     return RH_OK;
 }

@@ -16,7 +16,9 @@
 
 #include "rh_current.hpp"
 #include "rh_eval.hpp"
+#include "rh_geom.hpp"
 #include "rh_kernels.hpp"
+#include "rh_mesh.hpp"
 
 namespace rh {
 
@@ -217,6 +219,167 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH_TET_WAVE
     } else {
         for (int k = 0; k < 3; k++) {
             if (B) B[3 * p + k] = b[k];
+            if (H) H[3 * p + k] = h[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- mesh form
+// Charged records of the mesh form for magnetization M (one lane per entity).
+__global__ __launch_bounds__(256) void k_mesh_charge_edges(MeshDev md, const double* __restrict__ M)
+{
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= md.ne) return;
+    double w[3] = {0.0, 0.0, 0.0};
+    for (int64_t k = md.edge_inc_off[i]; k < md.edge_inc_off[i + 1]; k++) {
+        const int64_t j = md.edge_inc_elem[k];
+        const double* W = md.edge_inc_w + 6 * k;
+        const double m0 = M[3 * j], m1 = M[3 * j + 1], m2 = M[3 * j + 2];
+        w[0] += W[0] * m0 + W[3] * m1 + W[4] * m2;
+        w[1] += W[3] * m0 + W[1] * m1 + W[5] * m2;
+        w[2] += W[4] * m0 + W[5] * m1 + W[2] * m2;
+    }
+    const double* g = md.edge_geo + kMeshEdgeGeo * i;
+    double* o = md.ch_edge + kMeshEdge * i;
+    for (int k = 0; k < kMeshEdgeGeo; k++) o[k] = g[k];
+    o[10] = w[0];
+    o[11] = w[1];
+    o[12] = w[2];
+    o[13] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_mesh_charge_faces(MeshDev md, const double* __restrict__ M)
+{
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= md.nf) return;
+    const double* g = md.face_geo + kMeshFaceGeo * i;
+    const double n0 = g[12], n1 = g[13], n2 = g[14];
+    double s = 0.0;
+    for (int64_t k = md.face_inc_off[i]; k < md.face_inc_off[i + 1]; k++) {
+        const int64_t j = md.face_inc_elem[k];
+        s += md.face_inc_sign[k] * (n0 * M[3 * j] + n1 * M[3 * j + 1] + n2 * M[3 * j + 2]);
+    }
+    double* o = md.ch_face + kMeshFace * i;
+    for (int k = 0; k < kMeshFaceGeo; k++) o[k] = g[k];
+    o[18] = s * n0;
+    o[19] = s * n1;
+    o[20] = s * n2;
+    o[21] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_mesh_charge_in(MeshDev md, const double* __restrict__ M)
+{
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= md.ni) return;
+    const int64_t j = md.in_elem[i];
+    const double* g = md.in_geo + (1 + 4 * kMeshPlanes) * i;
+    double* o = md.ch_in + kMeshIn * i;
+    o[0] = g[0];
+    o[1] = M[3 * j];
+    o[2] = M[3 * j + 1];
+    o[3] = M[3 * j + 2];
+    for (int k = 0; k < 4 * kMeshPlanes; k++) o[4 + k] = g[1 + k];
+}
+
+// Field sweep of the mesh form: slice blockIdx.y of the unique edges, unique
+// faces and inside tiles, each staged through LDS in tiles of kMeshTile
+// records read at wave-uniform addresses.  Inside tiles whose bounding box
+// holds no point of the wave are skipped (wave-uniform vote).
+template <class R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH_TET_WAVES))) void
+k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, int64_t slice0,
+             double* __restrict__ B, double* __restrict__ H, double* __restrict__ part)
+{
+    __shared__ __align__(16) double tile[kMeshTile * kMeshIn];
+    const MeshDev& md = m.mesh;
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const bool live = p < np;
+    const int64_t y = blockIdx.y;
+    double x[3] = {0.0, 0.0, 0.0};
+    if (live) {
+        x[0] = pts[3 * p];
+        x[1] = pts[3 * p + 1];
+        x[2] = pts[3 * p + 2];
+    }
+    double h[3] = {0.0, 0.0, 0.0}, min[3] = {0.0, 0.0, 0.0};
+    double2* dst = reinterpret_cast<double2*>(tile);
+    // unique edges: + L_e w_e
+    {
+        const int64_t e0 = md.ne * y / S, e1 = md.ne * (y + 1) / S;
+        for (int64_t base = e0; base < e1; base += kMeshTile) {
+            const int cnt = int((e1 - base < kMeshTile) ? (e1 - base) : kMeshTile);
+            const double2* src = reinterpret_cast<const double2*>(md.ch_edge + kMeshEdge * base);
+            __syncthreads();
+            for (int i = threadIdx.x; i < cnt * (kMeshEdge / 2); i += 256) dst[i] = src[i];
+            __syncthreads();
+            if (!live) continue;
+#pragma unroll 1
+            for (int q = 0; q < cnt; q++) {
+                const double* rec = tile + kMeshEdge * q;
+                const double L = mesh_edge_L<R>(rec, x);
+                h[0] += L * rec[10];
+                h[1] += L * rec[11];
+                h[2] += L * rec[12];
+            }
+        }
+    }
+    // unique faces: - Omega_f g_f
+    {
+        const int64_t f0 = md.nf * y / S, f1 = md.nf * (y + 1) / S;
+        for (int64_t base = f0; base < f1; base += kMeshTile) {
+            const int cnt = int((f1 - base < kMeshTile) ? (f1 - base) : kMeshTile);
+            const double2* src = reinterpret_cast<const double2*>(md.ch_face + kMeshFace * base);
+            __syncthreads();
+            for (int i = threadIdx.x; i < cnt * (kMeshFace / 2); i += 256) dst[i] = src[i];
+            __syncthreads();
+            if (!live) continue;
+#pragma unroll 1
+            for (int q = 0; q < cnt; q++) {
+                const double* rec = tile + kMeshFace * q;
+                const double om = mesh_face_omega<R>(rec, x);
+                h[0] -= om * rec[18];
+                h[1] -= om * rec[19];
+                h[2] -= om * rec[20];
+            }
+        }
+    }
+    // inside tests: B gets the magnetization of the element holding the point
+    {
+        const int64_t t0 = md.ntiles * y / S, t1 = md.ntiles * (y + 1) / S;
+        for (int64_t t = t0; t < t1; t++) {
+            const double* bx = md.tile_box + 6 * t;
+            const bool near = live && x[0] >= bx[0] && x[1] >= bx[1] && x[2] >= bx[2] &&
+                              x[0] <= bx[3] && x[1] <= bx[4] && x[2] <= bx[5];
+            if (!__syncthreads_or(near ? 1 : 0)) continue;
+            const int64_t base = t * kMeshTile;
+            const int cnt = int((md.ni - base < kMeshTile) ? (md.ni - base) : kMeshTile);
+            const double2* src = reinterpret_cast<const double2*>(md.ch_in + kMeshIn * base);
+            for (int i = threadIdx.x; i < cnt * (kMeshIn / 2); i += 256) dst[i] = src[i];
+            __syncthreads();
+            if (near) {
+                for (int q = 0; q < cnt; q++) {
+                    const double* rec = tile + kMeshIn * q;
+                    if (mesh_inside(rec, x)) {
+                        min[0] += rec[1];
+                        min[1] += rec[2];
+                        min[2] += rec[3];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (!live) return;
+    for (int k = 0; k < 3; k++) h[k] *= kInv4Pi;
+    if (part) {
+        double* o = part + ((slice0 + y) * np + p) * 6;
+        for (int k = 0; k < 3; k++) {
+            o[k] = h[k] + min[k];
+            o[3 + k] = h[k];
+        }
+    } else {
+        for (int k = 0; k < 3; k++) {
+            if (B) B[3 * p + k] = h[k] + min[k];
             if (H) H[3 * p + k] = h[k];
         }
     }
@@ -630,20 +793,40 @@ void field_slot(const DevModel& m, const double* M, const double* pts, int64_t n
                        H, part);
 }
 
+bool mesh_on(const DevModel& m) { return m.use_mesh && m.mesh.active && !m.sym; }
+
+// slices per kind slot (0 blocks, 1 tetrahedra, 2 general polyhedra); in the
+// mesh form slot 1 carries the whole mesh and slot 2 is empty
+void slot_plan(const DevModel& m, int64_t np, int64_t sp[3])
+{
+    sp[0] = slot_splits(np, m.koff[1] - m.koff[0]);
+    if (mesh_on(m)) {
+        sp[1] = slot_splits(np, m.mesh.ne + m.mesh.nf);
+        sp[2] = 0;
+    } else {
+        sp[1] = slot_splits(np, m.koff[2] - m.koff[1]);
+        sp[2] = slot_splits(np, m.koff[3] - m.koff[2]);
+    }
+}
+
 template <class R, bool SYM>
 hipError_t field_all(const DevModel& m, const double* M, const double* pts, int64_t np, double* B,
                      double* H, double* work, hipStream_t s)
 {
-    int64_t sp[3], total = 0;
-    for (int slot = 0; slot < 3; slot++) {
-        sp[slot] = slot_splits(np, m.koff[slot + 1] - m.koff[slot]);
-        total += sp[slot];
-    }
+    int64_t sp[3];
+    slot_plan(m, np, sp);
+    const int64_t total = sp[0] + sp[1] + sp[2];
     double* part = (total > 1) ? work : nullptr;
     int64_t slice0 = 0;
     if (sp[0]) field_slot<R, SYM, 0>(m, M, pts, np, sp[0], slice0, B, H, part, s);
     slice0 += sp[0];
-    if (sp[1]) field_slot<R, SYM, 1>(m, M, pts, np, sp[1], slice0, B, H, part, s);
+    if (sp[1] && mesh_on(m)) {
+        const dim3 grid{unsigned((np + 255) / 256), unsigned(sp[1]), 1u};
+        hipLaunchKernelGGL((k_field_mesh<R>), grid, dim3(256), 0, s, m, pts, np, sp[1], slice0, B, H,
+                           part);
+    } else if (sp[1]) {
+        field_slot<R, SYM, 1>(m, M, pts, np, sp[1], slice0, B, H, part, s);
+    }
     slice0 += sp[1];
     if (sp[2]) field_slot<R, SYM, 2>(m, M, pts, np, sp[2], slice0, B, H, part, s);
     hipError_t e = hipGetLastError();
@@ -656,9 +839,9 @@ hipError_t field_all(const DevModel& m, const double* M, const double* pts, int6
 
 int64_t field_slices(const DevModel& m, int64_t np)
 {
-    int64_t total = 0;
-    for (int slot = 0; slot < 3; slot++) total += slot_splits(np, m.koff[slot + 1] - m.koff[slot]);
-    return std::max<int64_t>(1, total);
+    int64_t sp[3];
+    slot_plan(m, np, sp);
+    return std::max<int64_t>(1, sp[0] + sp[1] + sp[2]);
 }
 
 hipError_t launch_field(const DevModel& m0, const double* M, const double* pts, int64_t np,
@@ -673,7 +856,18 @@ hipError_t launch_field(const DevModel& m0, const double* M, const double* pts, 
     }
     DevModel m = m0;
     const int64_t ntet = m.koff[2] - m.koff[1];
-    if (ntet > 0 && charged) {
+    if (mesh_on(m)) {
+        const MeshDev& md = m.mesh;
+        if (md.ne)
+            hipLaunchKernelGGL(k_mesh_charge_edges, dim3(unsigned((md.ne + 255) / 256)), dim3(256), 0, s,
+                               md, M);
+        if (md.nf)
+            hipLaunchKernelGGL(k_mesh_charge_faces, dim3(unsigned((md.nf + 255) / 256)), dim3(256), 0, s,
+                               md, M);
+        if (md.ni)
+            hipLaunchKernelGGL(k_mesh_charge_in, dim3(unsigned((md.ni + 255) / 256)), dim3(256), 0, s, md,
+                               M);
+    } else if (ntet > 0 && charged) {
         hipLaunchKernelGGL(k_charge_tets, dim3(unsigned((ntet + 255) / 256)), dim3(256), 0, s, m, M,
                            charged);
         m.charged = charged;

@@ -9,6 +9,27 @@
 
 namespace rh {
 
+// Device arrays of the conforming-mesh field form (rh_mesh.hpp); charged
+// records ch_* are rebuilt from M before each field sweep.
+struct MeshDev {
+    bool active = false;
+    int64_t ne = 0, nf = 0, ni = 0, ntiles = 0;
+    const double* edge_geo = nullptr;
+    const int64_t* edge_inc_off = nullptr;
+    const int32_t* edge_inc_elem = nullptr;
+    const double* edge_inc_w = nullptr;
+    const double* face_geo = nullptr;
+    const int64_t* face_inc_off = nullptr;
+    const int32_t* face_inc_elem = nullptr;
+    const double* face_inc_sign = nullptr;
+    const int32_t* in_elem = nullptr;
+    const double* in_geo = nullptr;
+    const double* tile_box = nullptr;
+    double* ch_edge = nullptr;
+    double* ch_face = nullptr;
+    double* ch_in = nullptr;
+};
+
 struct DevModel {
     int64_t n = 0;
     const int32_t* kind = nullptr;
@@ -23,6 +44,8 @@ struct DevModel {
     const double* charged = nullptr;  // field-only tetrahedron records (per call)
     bool sym = false;
     bool rowt_identity = true;
+    MeshDev mesh;
+    bool use_mesh = false;  // field sweeps use the mesh form (when active)
 };
 
 // Interaction matrix columns [col0, col1) of all rows into Q (row-major, leading

@@ -1,0 +1,84 @@
+// rh_mesh.hpp -- field of a conforming polyhedral mesh as line terms on its
+// unique edges and charge sheets on its unique faces.
+//
+// Summed over the elements of a mesh, the per-element form
+//     4 pi H = sum_e L_e W_e M - sum_f Omega_f (n_f . M) n_f
+// regroups by mesh entity.  L_e depends only on the edge and the point, so
+// every edge is evaluated once with the net weight w_e = sum_{elements at e}
+// W_{e,j} M_j.  A face shared by two elements appears with opposite outward
+// normals and opposite signed solid angles, so it is evaluated once with the
+// net charge g_f = n_f sum_j s_j (n_f . M_j), s_j = +-1 the orientation of
+// element j's copy of the face.  The sum is the same; on the C-yoke mesh the
+// work per tetrahedron drops from 6 logs and 3 atan2 to about 1.4 and 2.2.
+// B = H + M of the element containing the point, found by plane tests.
+#pragma once
+
+#include "rh_math.hpp"
+
+namespace rh {
+
+// Charged edge record: a(3), b(3), t(3), len, w(3), pad.
+constexpr int kMeshEdge = 14;
+// Charged face record (triangle or quadrilateral, CCW about n): v0..v3 (12),
+// n(3), 2*area(v0,v1,v2), 2*area(v0,v2,v3), nv, g(3), pad.
+constexpr int kMeshFace = 22;
+// Inside-test record: nplanes, M(3), up to 8 planes (n(3), d) with outward n:
+// inside iff n.p < d for every plane.
+constexpr int kMeshPlanes = 8;
+constexpr int kMeshIn = 4 + 4 * kMeshPlanes;  // 36
+// Geometry-only sizes (host arrays the charged records are built from).
+constexpr int kMeshEdgeGeo = 10;  // a, b, t, len
+constexpr int kMeshFaceGeo = 18;  // v0..v3, n, a1, a2, nv
+
+// L_e of one charged edge at p.
+template <class R>
+RH_HD double mesh_edge_L(const double* __restrict__ rec, const double p[3])
+{
+    const V3<R> ra = rel<R>(rec, p), rb = rel<R>(rec + 3, p);
+    const R Ra = rh_sqrt(dot(ra, ra)), Rb = rh_sqrt(dot(rb, rb));
+    const V3<R> t = v3(R(rec[6]), R(rec[7]), R(rec[8]));
+    if (sizeof(R) == 4) return double(edge_log1p(ra, Ra, rb, Rb, t, R(rec[9])));
+    R num, den;
+    edge_ratio(ra, Ra, rb, Rb, t, num, den);
+    return double(rh_log(num / den));
+}
+
+// Signed solid angle of one charged face at p: the fan triangles (v0,v1,v2)
+// and (v0,v2,v3) multiplied as complex numbers, one atan2 per face.
+template <class R>
+RH_HD double mesh_face_omega(const double* __restrict__ rec, const double p[3])
+{
+    const V3<R> r0 = rel<R>(rec, p), r1 = rel<R>(rec + 3, p), r2 = rel<R>(rec + 6, p);
+    const R R0 = rh_sqrt(dot(r0, r0)), R1 = rh_sqrt(dot(r1, r1)), R2 = rh_sqrt(dot(r2, r2));
+    const V3<R> n = v3(R(rec[12]), R(rec[13]), R(rec[14]));
+    const R h = dot(r0, n);  // r0 . n, the same for every fan triangle
+    R im = R(rec[15]) * h;
+    R re = R0 * R1 * R2 + dot(r0, r1) * R2 + dot(r0, r2) * R1 + dot(r1, r2) * R0;
+    if (rec[17] > 3.5) {
+        const V3<R> r3 = rel<R>(rec + 9, p);
+        const R R3 = rh_sqrt(dot(r3, r3));
+        const R im2 = R(rec[16]) * h;
+        const R re2 = R0 * R2 * R3 + dot(r0, r2) * R3 + dot(r0, r3) * R2 + dot(r2, r3) * R0;
+        const R pr = re * re2 - im * im2;
+        const R pi = re * im2 + im * re2;
+        re = pr;
+        im = pi;
+    }
+    return 2.0 * double(rh_atan2(im, re));
+}
+
+// Plane test of one inside record.
+RH_HD bool mesh_inside(const double* __restrict__ rec, const double p[3])
+{
+    const int np = int(rec[0]);
+    bool in = true;
+    for (int k = 0; k < kMeshPlanes; k++) {
+        if (k < np) {
+            const double* q = rec + 4 + 4 * k;
+            in = in && (q[0] * p[0] + q[1] * p[1] + q[2] * p[2] < q[3]);
+        }
+    }
+    return in;
+}
+
+}  // namespace rh

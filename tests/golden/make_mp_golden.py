@@ -1,7 +1,9 @@
 """Writes mp50_tets.json: fields of single tetrahedra evaluated in 50-digit
 arithmetic (mpmath) from the face-charge closed form, for elements of the
 jittered C-yoke mesh whose faces have edges nearly parallel to a face-frame
-axis (where the reference's y = kx + b edge form loses k^2*eps).
+axis (where the reference's y = kx + b edge form loses k^2*eps), and for
+tetrahedra with a face whose outward normal is close to -z (where the
+reference's face rotation loses accuracy).
 
 The closed form is what the mathematics fixes: H = (1/4pi) sum_f sigma_f
 [sum_e L_e m_e - Omega_f n_f], sigma_f = M.n_f (face-charge model of a
@@ -78,6 +80,19 @@ def main():
             M = [0.3, -0.2, 1.0]
             cases.append({"verts": o.verts.tolist(), "faces": o.faces, "M": M, "p": x.tolist(),
                           "H": H_mp(o.verts, o.faces, M, x)})
+    # tetrahedra with a face whose outward normal lies within ~1e-3 of -z, where
+    # the reference's face rotation (radvlpgn.cpp:357-371) divides by 1 + n_z:
+    # one from the 40,000-tetrahedron yoke (c_yoke(ny=50), element 17209) at a
+    # gap point, and a synthetic one tilted by 1e-5
+    near = [([[40.0, 420.0, 80.0], [29.58917967, 419.07890003, 80.00427541], [40.0, 410.0, 80.0],
+              [40.0, 420.0, 90.0]], [58.42105263, 428.94736842, 72.21052632]),
+            ([[0.0, 0.0, 0.0], [10.0, 0.0, 1e-4], [0.0, 10.0, 0.0], [0.0, 0.0, 10.0]],
+             [13.0, 7.0, -4.0])]
+    for verts, x in near:
+        M = [0.3, -0.2, 1.0]
+        faces = [[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]]
+        cases.append({"verts": verts, "faces": faces, "M": M, "p": x, "near_minus_z": True,
+                      "H": H_mp(np.array(verts), faces, M, np.array(x))})
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mp50_tets.json")
     json.dump({"generator": "tests/golden/make_mp_golden.py", "digits": 50, "cases": cases},
               open(out, "w"), indent=1)

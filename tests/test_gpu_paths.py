@@ -206,3 +206,41 @@ def test_current_sources_match_oracle(oracle_off):
         b2, _ = OR.reccur_BH(xl, [4, -2, 1], [2, 3, 4], [1.5, -2.0, 0.7])
         Bo[i] = t.M @ (b1 + b2)
     assert np.abs(B - Bo).max() <= 1e-10 * np.abs(Bo).max()
+
+
+def test_field_mesh_form(oracle_off):
+    """A conforming tetrahedral mesh plus blocks, and a hexahedral ring: the
+    mesh-form sweep (unique edges and faces) equals the per-element sweep and
+    the oracle, inside and outside the elements, in FP64; FP32 stays within
+    its per-element error."""
+    print("case: field mesh form", flush=True)
+    from radiacuda_amd import workloads as W
+    rng = np.random.default_rng(41)
+    ids = models.tet_mesh((4, 3, 2), 4.0, jitter=0.1, seed=42)
+    ids.append(rad.ObjRecMag([24, 3, 2], [3, 4, 5], list(rng.uniform(-1, 1, 3))))
+    ring, _p = W.halbach(n_seg=4, n_az=3, n_rad=2, n_ax=3, r_in=30.0, r_out=40.0, n_points=10)
+    for top, box in ((rad.ObjCnt(ids), (-6, 30)), (ring, (-45, 45))):
+        P, items = models.packed(top)
+        M = rng.uniform(-1, 1, (len(items), 3))
+        model = N.Model(P)
+        ne, nf, ni = model.mesh_counts()
+        print("mesh form: %d unique edges, %d unique faces, %d polyhedra" % (ne, nf, ni), flush=True)
+        assert ni > 0 and model.field_form() == 2
+        pts = rng.uniform(box[0], box[1], (3000, 3))
+        Bm, Hm = model.field(M, pts)
+        Bm32, _ = model.field(M, pts, precision=N.RH_MIXED)
+        model.set_field_form(1)
+        assert model.field_form() == 1
+        Be, He = model.field(M, pts)
+        Be32, _ = model.field(M, pts, precision=N.RH_MIXED)
+        scale = np.abs(Be).max()
+        assert np.abs(Bm - Be).max() <= 1e-12 * scale
+        assert np.abs(Hm - He).max() <= 1e-12 * np.abs(He).max()
+        e_mesh = np.abs(Bm32 - Bm).max() / scale
+        e_elem = np.abs(Be32 - Be).max() / scale
+        print("FP32 error: mesh form %.2e, per element %.2e" % (e_mesh, e_elem), flush=True)
+        assert e_mesh <= max(4 * e_elem, 1e-6)
+        sub = pts[:200]
+        Bo, _Ho, _bad = models.oracle_field(P, M, sub)
+        assert np.abs(Bm[:200] - Bo).max() <= 1e-10 * np.abs(Bo).max()
+        model.close()

@@ -155,14 +155,17 @@ def test_symmetric_model_equals_explicit_mirror(L):
 
 
 def test_kernel_math_matches_50_digit_values_where_reference_form_loses_digits(L):
-    """Tetrahedra of the jittered C-yoke with edges nearly parallel to a face
-    frame axis: the kernel formulation stays within 1e-11 of 50-digit values
-    (tests/golden/mp50_tets.json); the reference's y = kx + b edge form
-    (radplnr2.cpp:139-170, restated by the oracle) loses up to ~1e-3 of these
-    far elements' contributions."""
+    """Tetrahedra of the jittered C-yoke whose faces have edges nearly parallel
+    to a face-frame axis, and tetrahedra with a face whose outward normal is
+    close to -z: the kernel formulation matches 50-digit evaluations
+    (tests/golden/mp50_tets.json) to <= 1e-11 relative.  The reference's
+    forms, restated by the oracle, lose accuracy on both kinds: the
+    y = kx + b edge form (radplnr2.cpp:139-170) up to ~1e-3 of these far
+    elements' contributions, and the face rotation (radvlpgn.cpp:357-371),
+    which divides by 1 + n_z, about 4e-4 on the -z cases."""
     import json
     gold = json.load(open(os.path.join(HERE, "golden", "mp50_tets.json")))
-    worst_k, worst_o = 0.0, 0.0
+    worst_k, worst_o, worst_oz = 0.0, 0.0, 0.0
     for c in gold["cases"]:
         o = O.Polyhedron(c["verts"], c["faces"], c["M"])
         P = O.pack_elements([(o, 0, ())])
@@ -171,6 +174,11 @@ def test_kernel_math_matches_50_digit_values_where_reference_form_loses_digits(L
         s = np.abs(Hm).max()
         worst_k = max(worst_k, np.abs(H[0] - Hm).max() / s)
         _b, Ho = OR.Poly(c["verts"], c["faces"]).BH(c["p"], c["M"])
-        worst_o = max(worst_o, np.abs(Ho - Hm).max() / s)
+        if c.get("near_minus_z"):
+            worst_oz = max(worst_oz, np.abs(Ho - Hm).max() / s)
+        else:
+            worst_o = max(worst_o, np.abs(Ho - Hm).max() / s)
     assert worst_k <= 1e-11
-    assert worst_o > 1e4 * worst_k  # documents the reference form's loss on these elements
+    # documents the reference forms' loss on these elements
+    assert worst_o > 1e4 * worst_k
+    assert worst_oz > 1e-5

@@ -1,0 +1,94 @@
+"""CPU checks of the conforming-mesh field form (rh_mesh.hpp, host build):
+the regrouped sum over unique edges and faces equals the per-element sum, and
+the unique-entity counts are those of the mesh."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from radiacuda_amd import _abi
+from radiacuda_amd import objects as O
+from radiacuda_amd import rad
+from radiacuda_amd import workloads as W
+from tests import models
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(os.path.dirname(HERE), "radiacuda_amd", "_rhcheck.so")
+dp = ctypes.POINTER(ctypes.c_double)
+
+
+@pytest.fixture(scope="module")
+def L():
+    if not os.path.exists(LIB):
+        from radiacuda_amd import build
+        build.build()
+    return ctypes.CDLL(LIB)
+
+
+def _fields(L, P, M, pts, fp32=0):
+    desc, keep = _abi.model_desc(P)
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    out = {}
+    for name in ("elem", "mesh"):
+        B = np.zeros_like(pts)
+        H = np.zeros_like(pts)
+        if name == "elem":
+            rc = L.rhc_field(ctypes.byref(desc), M.ctypes.data_as(dp), pts.ctypes.data_as(dp),
+                             ctypes.c_int64(len(pts)), ctypes.c_int32(fp32), B.ctypes.data_as(dp),
+                             H.ctypes.data_as(dp))
+            counts = None
+        else:
+            counts = np.zeros(3, np.int64)
+            rc = L.rhc_field_mesh(ctypes.byref(desc), M.ctypes.data_as(dp), pts.ctypes.data_as(dp),
+                                  ctypes.c_int64(len(pts)), ctypes.c_int32(fp32), B.ctypes.data_as(dp),
+                                  H.ctypes.data_as(dp), counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        assert rc == 0, rc
+        out[name] = (B, H, counts)
+    return out
+
+
+def test_mesh_form_tets_and_blocks_equals_per_element(L):
+    rng = np.random.default_rng(31)
+    ids = models.tet_mesh((3, 2, 2), 4.0, jitter=0.1, seed=32)
+    ids.append(rad.ObjRecMag([20, 3, 2], [3, 4, 5], list(rng.uniform(-1, 1, 3))))
+    P, items = models.packed(rad.ObjCnt(ids))
+    M = rng.uniform(-1, 1, (len(items), 3))
+    # points inside and outside the mesh (none on a face plane)
+    pts = np.concatenate([rng.uniform(0.2, 7.8, (80, 3)), rng.uniform(-10, 25, (80, 3))])
+    f = _fields(L, P, M, pts)
+    Be, He, _ = f["elem"]
+    Bm, Hm, counts = f["mesh"]
+    assert counts[2] == len(items) - 1
+    assert counts[1] < 4 * counts[2]  # interior faces are shared
+    assert np.abs(Hm - He).max() <= 1e-12 * np.abs(He).max()
+    assert np.abs(Bm - Be).max() <= 1e-12 * np.abs(Be).max()
+
+
+def test_mesh_form_hexahedra_equals_per_element(L):
+    rng = np.random.default_rng(33)
+    top, _pts = W.halbach(n_seg=4, n_az=2, n_rad=2, n_ax=2, n_points=10)
+    P, items = models.packed(top)
+    M = rng.uniform(-1, 1, (len(items), 3))
+    pts = np.concatenate([rng.uniform(-45, 45, (100, 3)),
+                          np.stack([31 * np.cos(np.linspace(0.1, 6.1, 40)),
+                                    31 * np.sin(np.linspace(0.1, 6.1, 40)),
+                                    np.linspace(-20, 20, 40)], 1)])
+    for fp32, tol in ((0, 1e-12), (1, 1e-5)):
+        f = _fields(L, P, M, pts, fp32)
+        Be, He, _ = f["elem"]
+        Bm, Hm, counts = f["mesh"]
+        assert np.abs(Hm - He).max() <= tol * np.abs(He).max()
+        assert np.abs(Bm - Be).max() <= tol * np.abs(Be).max()
+    assert counts[1] < 6 * counts[2]
+
+
+def test_mesh_counts_c_yoke(L):
+    top, iron, coil, info = W.c_yoke()
+    items = list(O.walk(rad._reg, iron))
+    P = O.pack_elements(items)
+    M = np.tile([0.0, 0.0, 1.0], (len(items), 3 // 3))
+    f = _fields(L, P, M, W.gap_grid(info, 2))
+    _B, _H, counts = f["mesh"]
+    assert list(counts) == [11454, 17200, 8000]
