@@ -241,6 +241,17 @@ def main():
         flops = float(npts) * n * FLOP_TET_FIELD
         fld_kernel = "k_field_tet<double> (FP64 integrals)"
     achieved = flops / (fld_ms * 1e-3) / 1e12 if fld_ms > 0 else 0.0
+    # HBM bytes per launch of the dominant kernel, measured by rocprofv3
+    # (FETCH_SIZE / WRITE_SIZE passes, calibrated on a device copy) and kept
+    # in profiles/field_traffic.json for this kernel and workload
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "field_traffic.json")) as fh:
+            tr = json.load(fh)
+        if fld_kernel.startswith(tr["kernel"]) and args.grid == 100 and n == 8000:
+            traffic = float(tr["traffic_bytes"])
+    except (OSError, ValueError, KeyError):
+        traffic = None
     asm_ms = r64["kernel_ms"]["asm"]
     mv_ms = r64["kernel_ms"]["matvec"]
     n3 = 3 * n
@@ -278,7 +289,7 @@ def main():
                   "B_err_vs_fp64": err_mixed, "M_err_vs_fp64": err_M},
         "roofline": {"bound": "valu", "kernel": fld_kernel,
                      "achieved": achieved, "peak": PEAK_FP64_VALU, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_VALU, "traffic": None,
+                     "frac": achieved / PEAK_FP64_VALU, "traffic": traffic,
                      "flops_per_launch": flops, "avg_launch_ms": fld_ms},
         "cpu_baseline": cpu,
     }

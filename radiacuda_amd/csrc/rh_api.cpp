@@ -4,9 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -46,12 +48,13 @@ hipError_t enter_device(int device)
     } while (0)
 
 // ---------------------------------------------------------------- timing
-bool g_timing = false;
+std::atomic<bool> g_timing{false};
 struct Family {
     double total_ms = 0.0;
     int64_t count = 0;
 };
 std::map<std::string, Family> g_fam;
+std::mutex g_fam_mu;  // the timing statistics are shared by all threads
 
 struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
@@ -81,6 +84,7 @@ struct Timer {
     ~Timer()
     {
         if (ev.empty()) return;
+        std::lock_guard<std::mutex> lock(g_fam_mu);
         Family& F = g_fam[fam];
         for (EventPair& p : ev) {
             (void)hipEventSynchronize(p.b);
@@ -206,6 +210,7 @@ int32_t rh_device_count(void)
 
 rh_status rh_set_timing(int32_t on)
 {
+    std::lock_guard<std::mutex> lock(g_fam_mu);
     g_timing = on != 0;
     g_fam.clear();
     return RH_OK;
@@ -214,6 +219,7 @@ rh_status rh_set_timing(int32_t on)
 rh_status rh_kernel_time(const char* family, double* avg_ms, int64_t* count)
 {
     if (!family || !avg_ms || !count) return fail(RH_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lock(g_fam_mu);
     auto it = g_fam.find(family);
     if (it == g_fam.end() || it->second.count == 0) {
         *avg_ms = 0.0;

@@ -518,6 +518,12 @@ void build_mesh_form(PackedModel& P)
             }
         }
     }
+    // the regrouped sweep pays off only when faces are shared: a face
+    // evaluated on its own recomputes the vertex distances the per-element
+    // sweep shares between the faces of an element
+    int64_t face_incidences = 0;
+    for (const auto& f : finc) face_incidences += int64_t(f.size());
+    if (double(finc.size()) > 0.8 * double(face_incidences)) return;
     // edges
     M.edge_inc_off.push_back(0);
     std::vector<std::pair<int32_t, int32_t>> ekeys(einc.size());
