@@ -104,10 +104,32 @@ typedef struct rh_matrix rh_matrix;
  * (cpp/src/core/radgpu_asm.cu:40-51, cpp/src/core/radgpurlx.cu:728-746): the
  * handle owns the device matrix, no host copy is made. */
 rh_status rh_matrix_assemble(rh_model *m, rh_precision prec, rh_matrix **out);
+
+/* Placement of the matrix rows.  Rows that do not fit in device memory can
+ * live in pinned host memory and be streamed through two 64 MiB device tiles
+ * in every matrix pass, the upload of one tile overlapping the pass over the
+ * other; results do not depend on the split (each row is reduced the same
+ * way wherever it lives).  Replaces the reference's streaming matrix context
+ * and its UtiGpuFallback 'gpu_streaming' policy (cpp/src/core/radgpurlx.cu:258-459,
+ * cpp/src/core/radintrc.cpp:796-805). */
+typedef struct {
+    int64_t device_bytes;     /* device-memory budget for the matrix; <= 0: the free
+                               * memory less 10 % and 1 GiB                        */
+    double resident_fraction; /* < 0: keep as many element rows resident as the
+                               * budget allows; else this fraction of them         */
+    int32_t allow_streaming;  /* 0: RH_ERR_NOMEM when the matrix exceeds the
+                               * budget; 1: stream the remaining rows               */
+    int32_t reserved;         /* 0                                                 */
+} rh_matrix_opts;
+rh_status rh_matrix_assemble_ex(rh_model *m, rh_precision prec, const rh_matrix_opts *opts,
+                                rh_matrix **out);
+/* Rows held on the device and rows streamed from host memory. */
+rh_status rh_matrix_layout(const rh_matrix *q, int64_t *resident_rows, int64_t *streamed_rows);
 rh_status rh_matrix_destroy(rh_matrix *q);
 /* Copies rows [row0, row0+nrows) of the 3n x 3n matrix to host doubles. */
 rh_status rh_matrix_export(const rh_matrix *q, int64_t row0, int64_t nrows, double *out);
-/* Raw device pointer and element size (4 or 8 bytes) of the stored matrix. */
+/* Raw device pointer and element size (4 or 8 bytes) of the stored matrix
+ * (RH_ERR_UNSUPPORTED when part of it is streamed). */
 rh_status rh_matrix_device_ptr(const rh_matrix *q, void **dptr, int32_t *elem_bytes);
 /* Re-assembles into an existing matrix (same model); the timed unit of work. */
 rh_status rh_matrix_reassemble(rh_matrix *q);

@@ -26,6 +26,11 @@ class RelaxOpts(ctypes.Structure):
                 ("method", ctypes.c_int32), ("refine", ctypes.c_int32)]
 
 
+class MatrixOpts(ctypes.Structure):
+    _fields_ = [("device_bytes", ctypes.c_int64), ("resident_fraction", ctypes.c_double),
+                ("allow_streaming", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class RelaxResult(ctypes.Structure):
     _fields_ = [("misfit", ctypes.c_double), ("max_m", ctypes.c_double),
                 ("max_h", ctypes.c_double), ("iters", ctypes.c_int32),

@@ -50,6 +50,9 @@ def lib():
     L.rh_model_colloc.argtypes = [_vp, _d]
     L.rh_model_counts.argtypes = [_vp, _i64]
     L.rh_matrix_assemble.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
+    L.rh_matrix_assemble_ex.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_abi.MatrixOpts),
+                                        ctypes.POINTER(_vp)]
+    L.rh_matrix_layout.argtypes = [_vp, _i64, _i64]
     L.rh_matrix_destroy.argtypes = [_vp]
     L.rh_matrix_export.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, _d]
     L.rh_matrix_device_ptr.argtypes = [_vp, ctypes.POINTER(_vp), _i32]
@@ -84,7 +87,7 @@ def lib():
                  "rh_set_timing", "rh_kernel_time", "rh_model_vertex_count",
                  "rh_field_adjoint", "rh_field_adjoint_dev", "rh_relax_adjoint_dev",
                  "rh_relax_adjoint", "rh_model_set_field_form", "rh_model_field_form",
-                 "rh_model_mesh_counts"]:
+                 "rh_model_mesh_counts", "rh_matrix_assemble_ex", "rh_matrix_layout"]:
         getattr(L, name).restype = st
     if L.rh_device_count() < 1:
         raise RadHipError("no HIP device visible; the HIP path has no CPU fallback")
@@ -160,8 +163,8 @@ class Model:
         check(lib().rh_model_counts(self.h, c.ctypes.data_as(_i64)))
         return c
 
-    def assemble(self, prec=RH_MIXED):
-        return Matrix(self, prec)
+    def assemble(self, prec=RH_MIXED, **placement):
+        return Matrix(self, prec, **placement)
 
     def set_field_form(self, form):
         """0: automatic (conforming-mesh form when held), 1: per element."""
@@ -212,11 +215,23 @@ class Model:
 class Matrix:
     """Device-resident interaction matrix (rh_matrix)."""
 
-    def __init__(self, model, prec=RH_MIXED):
+    def __init__(self, model, prec=RH_MIXED, device_bytes=0, resident_fraction=-1.0,
+                 allow_streaming=False):
+        """device_bytes / resident_fraction / allow_streaming: rh_matrix_opts
+        (rows beyond the device budget are streamed from pinned host memory
+        when allowed)."""
         self.model = model
         self.n = model.n
         self.h = _vp()
-        check(lib().rh_matrix_assemble(model.h, int(prec), ctypes.byref(self.h)))
+        o = _abi.MatrixOpts(int(device_bytes), float(resident_fraction), 1 if allow_streaming else 0, 0)
+        check(lib().rh_matrix_assemble_ex(model.h, int(prec), ctypes.byref(o), ctypes.byref(self.h)))
+
+    def layout(self):
+        """(rows resident on the device, rows streamed from host memory)."""
+        r = np.zeros(1, np.int64)
+        s = np.zeros(1, np.int64)
+        check(lib().rh_matrix_layout(self.h, r.ctypes.data_as(_i64), s.ctypes.data_as(_i64)))
+        return int(r[0]), int(s[0])
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:

@@ -156,8 +156,19 @@ struct rh_matrix {
     rh_precision prec = RH_MIXED;
     int elem_bytes = 4;
     int64_t n3 = 0, ld = 0;
-    void* d_Q = nullptr;
+    void* d_Q = nullptr;  // rows of elements [0, res_elems)
     double* d_qdiag = nullptr;
+    // out-of-core part: rows of elements [res_elems, n) in pinned host memory,
+    // streamed through two device tiles of tile_elems elements each
+    int64_t res_elems = 0;
+    void* h_Q = nullptr;
+    void* stage[2] = {nullptr, nullptr};
+    int64_t tile_elems = 0;
+    hipStream_t copy = nullptr;
+    hipEvent_t ev_ready[2] = {nullptr, nullptr};  // tile b holds data for the consumer
+    hipEvent_t ev_free[2] = {nullptr, nullptr};   // tile b may be overwritten
+    size_t row_bytes() const { return size_t(ld) * size_t(elem_bytes); }
+    bool streamed() const { return res_elems < n3 / 3; }
 };
 
 namespace {
@@ -375,46 +386,171 @@ static rh_status assemble_into(rh_matrix* q)
 {
     rh_model* m = q->m;
     RH_HIP(enter_device(m->device));
-    Timer t("asm", m->stream);
+    hipStream_t s = m->stream;
+    const int64_t n = m->host.n;
+    Timer t("asm", s);
     t.begin();
-    RH_HIP(rh::launch_assemble(m->dev, q->d_Q, q->elem_bytes, q->ld, q->d_qdiag, m->stream));
+    RH_HIP(rh::launch_assemble(m->dev, q->d_Q, q->elem_bytes, q->ld, q->d_qdiag, 0, q->res_elems, s));
+    // streamed rows: assemble a tile on the device, copy it to pinned host
+    // memory on the copy stream while the next tile is assembled
+    int64_t tile = 0;
+    for (int64_t e0 = q->res_elems; e0 < n; e0 += q->tile_elems, tile++) {
+        const int b = int(tile & 1);
+        const int64_t e1 = std::min(n, e0 + q->tile_elems);
+        RH_HIP(hipStreamWaitEvent(s, q->ev_free[b], 0));
+        RH_HIP(rh::launch_assemble(m->dev, q->stage[b], q->elem_bytes, q->ld, q->d_qdiag, e0, e1, s));
+        RH_HIP(hipEventRecord(q->ev_ready[b], s));
+        RH_HIP(hipStreamWaitEvent(q->copy, q->ev_ready[b], 0));
+        char* dst = static_cast<char*>(q->h_Q) + size_t(3 * (e0 - q->res_elems)) * q->row_bytes();
+        RH_HIP(hipMemcpyAsync(dst, q->stage[b], size_t(3 * (e1 - e0)) * q->row_bytes(),
+                              hipMemcpyDeviceToHost, q->copy));
+        RH_HIP(hipEventRecord(q->ev_free[b], q->copy));
+    }
     t.end();
-    RH_HIP(hipStreamSynchronize(m->stream));
+    if (q->copy) RH_HIP(hipStreamSynchronize(q->copy));
+    RH_HIP(hipStreamSynchronize(s));
     return RH_OK;
 }
 
-rh_status rh_matrix_assemble(rh_model* m, rh_precision prec, rh_matrix** out)
+// y = Q x over all rows: the resident rows in place, the streamed rows tile by
+// tile, the upload of tile t+1 overlapping the pass over tile t.
+static hipError_t matrix_pass(rh_matrix* q, const double* x, double* y, hipStream_t s)
+{
+    const int64_t n = q->m->host.n;
+    hipError_t e = rh::launch_matvec(q->d_Q, q->elem_bytes, 3 * q->res_elems, q->ld, x, nullptr, y, s);
+    int64_t tile = 0;
+    for (int64_t e0 = q->res_elems; e == hipSuccess && e0 < n; e0 += q->tile_elems, tile++) {
+        const int b = int(tile & 1);
+        const int64_t e1 = std::min(n, e0 + q->tile_elems);
+        const char* src = static_cast<const char*>(q->h_Q) + size_t(3 * (e0 - q->res_elems)) * q->row_bytes();
+        e = hipStreamWaitEvent(q->copy, q->ev_free[b], 0);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(q->stage[b], src, size_t(3 * (e1 - e0)) * q->row_bytes(),
+                               hipMemcpyHostToDevice, q->copy);
+        if (e == hipSuccess) e = hipEventRecord(q->ev_ready[b], q->copy);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, q->ev_ready[b], 0);
+        if (e == hipSuccess)
+            e = rh::launch_matvec(q->stage[b], q->elem_bytes, 3 * (e1 - e0), q->ld, x, nullptr, y + 3 * e0, s);
+        if (e == hipSuccess) e = hipEventRecord(q->ev_free[b], s);
+    }
+    return e;
+}
+
+static void free_matrix(rh_matrix* q)
+{
+    if (!q) return;
+    (void)hipSetDevice(q->device);
+    if (q->copy) (void)hipStreamSynchronize(q->copy);
+    dfree(q->d_Q);
+    dfree(q->d_qdiag);
+    dfree(q->stage[0]);
+    dfree(q->stage[1]);
+    if (q->h_Q) (void)hipHostFree(q->h_Q);
+    for (int b = 0; b < 2; b++) {
+        if (q->ev_ready[b]) (void)hipEventDestroy(q->ev_ready[b]);
+        if (q->ev_free[b]) (void)hipEventDestroy(q->ev_free[b]);
+    }
+    if (q->copy) (void)hipStreamDestroy(q->copy);
+    delete q;
+}
+
+rh_status rh_matrix_assemble_ex(rh_model* m, rh_precision prec, const rh_matrix_opts* opts,
+                                rh_matrix** out)
 {
     if (!m || !out) return fail(RH_ERR_ARG, "null argument");
     *out = nullptr;
     if (prec != RH_FP64 && prec != RH_MIXED) return fail(RH_ERR_ARG, "unknown precision");
+    rh_matrix_opts o{};
+    o.resident_fraction = -1.0;
+    if (opts) o = *opts;
+    if (o.resident_fraction > 1.0) return fail(RH_ERR_ARG, "resident_fraction above 1");
     RH_HIP(enter_device(m->device));
+    const int64_t n = m->host.n;
     rh_matrix* q = new rh_matrix();
     q->m = m;
     q->device = m->device;
     q->prec = prec;
     q->elem_bytes = (prec == RH_FP64) ? 8 : 4;
-    q->n3 = 3 * m->host.n;
+    q->n3 = 3 * n;
     q->ld = std::max<int64_t>(256, (q->n3 + 255) / 256 * 256);
-    const size_t bytes = size_t(q->n3) * size_t(q->ld) * size_t(q->elem_bytes);
-    hipError_t e = hipMalloc(&q->d_Q, std::max<size_t>(bytes, 16));
-    if (e == hipSuccess) e = dalloc(&q->d_qdiag, size_t(9 * m->host.n));
-    // padded columns must read as zero in the matrix pass
-    if (e == hipSuccess) e = hipMemsetAsync(q->d_Q, 0, bytes, m->stream);
-    if (e != hipSuccess) {
-        std::string msg = std::string("matrix allocation (") + std::to_string(bytes) +
-                          " bytes): " + hipGetErrorString(e);
-        dfree(q->d_Q);
-        dfree(q->d_qdiag);
+    const size_t elem_rows = 3 * q->row_bytes();  // one element's 3 rows
+    const size_t total = size_t(n) * elem_rows;
+    size_t budget = size_t(o.device_bytes);
+    if (o.device_bytes <= 0) {
+        size_t fr = 0, tot = 0;
+        RH_HIP(hipMemGetInfo(&fr, &tot));
+        budget = fr - std::min(fr, fr / 10 + (size_t(1) << 30));
+    }
+    int64_t res = n;
+    if (o.resident_fraction >= 0.0) {
+        res = std::min<int64_t>(n, int64_t(o.resident_fraction * double(n)));
+    } else if (total > budget) {
+        if (!o.allow_streaming) {
+            delete q;
+            return fail(RH_ERR_NOMEM, "the " + std::to_string(total) + "-byte matrix exceeds the device budget of " +
+                                          std::to_string(budget) + " bytes and streaming is not allowed");
+        }
+        const size_t stage_bytes = size_t(2) << 26;  // two 64 MiB tiles
+        res = (budget > stage_bytes) ? int64_t((budget - stage_bytes) / elem_rows) : 0;
+        res = std::min(res, n);
+    }
+    if (res < n && !o.allow_streaming && o.resident_fraction < 0.0) {
         delete q;
+        return fail(RH_ERR_NOMEM, "matrix does not fit and streaming is not allowed");
+    }
+    q->res_elems = res;
+    const size_t dev_bytes = size_t(res) * elem_rows;
+    hipError_t e = hipMalloc(&q->d_Q, std::max<size_t>(dev_bytes, 16));
+    if (e == hipSuccess) e = dalloc(&q->d_qdiag, size_t(9 * n));
+    // padded columns must read as zero in the matrix pass
+    if (e == hipSuccess && dev_bytes) e = hipMemsetAsync(q->d_Q, 0, dev_bytes, m->stream);
+    if (e == hipSuccess && res < n) {
+        q->tile_elems = std::max<int64_t>(1, int64_t((size_t(1) << 26) / elem_rows));
+        q->tile_elems = std::min(q->tile_elems, n - res);
+        const size_t tb = size_t(q->tile_elems) * elem_rows;
+        for (int b = 0; b < 2 && e == hipSuccess; b++) {
+            e = hipMalloc(&q->stage[b], tb);
+            if (e == hipSuccess) e = hipMemsetAsync(q->stage[b], 0, tb, m->stream);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&q->ev_ready[b], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&q->ev_free[b], hipEventDisableTiming);
+        }
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&q->copy, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipHostMalloc(&q->h_Q, size_t(n - res) * elem_rows, hipHostMallocDefault);
+        // events start out signalled
+        for (int b = 0; b < 2 && e == hipSuccess; b++) {
+            e = hipEventRecord(q->ev_free[b], m->stream);
+            if (e == hipSuccess) e = hipEventRecord(q->ev_ready[b], m->stream);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e != hipSuccess) {
+        std::string msg = std::string("matrix allocation (") + std::to_string(dev_bytes) +
+                          " device bytes, " + std::to_string(size_t(n - res) * elem_rows) +
+                          " pinned host bytes): " + hipGetErrorString(e);
+        free_matrix(q);
         return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_HIP, msg);
     }
     rh_status st = assemble_into(q);
     if (st != RH_OK) {
-        rh_matrix_destroy(q);
+        free_matrix(q);
         return st;
     }
     *out = q;
+    return RH_OK;
+}
+
+rh_status rh_matrix_assemble(rh_model* m, rh_precision prec, rh_matrix** out)
+{
+    rh_matrix_opts o{};
+    o.resident_fraction = -1.0;
+    return rh_matrix_assemble_ex(m, prec, &o, out);
+}
+
+rh_status rh_matrix_layout(const rh_matrix* q, int64_t* resident_rows, int64_t* streamed_rows)
+{
+    if (!q || !resident_rows || !streamed_rows) return fail(RH_ERR_ARG, "null argument");
+    *resident_rows = 3 * q->res_elems;
+    *streamed_rows = q->n3 - 3 * q->res_elems;
     return RH_OK;
 }
 
@@ -426,17 +562,14 @@ rh_status rh_matrix_reassemble(rh_matrix* q)
 
 rh_status rh_matrix_destroy(rh_matrix* q)
 {
-    if (!q) return RH_OK;
-    (void)hipSetDevice(q->device);
-    dfree(q->d_Q);
-    dfree(q->d_qdiag);
-    delete q;
+    free_matrix(q);
     return RH_OK;
 }
 
 rh_status rh_matrix_device_ptr(const rh_matrix* q, void** dptr, int32_t* elem_bytes)
 {
     if (!q || !dptr || !elem_bytes) return fail(RH_ERR_ARG, "null argument");
+    if (q->streamed()) return fail(RH_ERR_UNSUPPORTED, "the matrix is partly streamed from host memory");
     *dptr = q->d_Q;
     *elem_bytes = q->elem_bytes;
     return RH_OK;
@@ -451,8 +584,13 @@ rh_status rh_matrix_export(const rh_matrix* q, int64_t row0, int64_t nrows, doub
     const int64_t n3 = q->n3;
     std::vector<char> buf(size_t(q->ld) * size_t(q->elem_bytes));
     for (int64_t r = row0; r < row0 + nrows; r++) {
-        const char* src = static_cast<const char*>(q->d_Q) + size_t(r) * q->ld * q->elem_bytes;
-        RH_HIP(hipMemcpy(buf.data(), src, size_t(n3) * q->elem_bytes, hipMemcpyDeviceToHost));
+        if (r < 3 * q->res_elems) {
+            const char* src = static_cast<const char*>(q->d_Q) + size_t(r) * q->row_bytes();
+            RH_HIP(hipMemcpy(buf.data(), src, size_t(n3) * q->elem_bytes, hipMemcpyDeviceToHost));
+        } else {
+            const char* src = static_cast<const char*>(q->h_Q) + size_t(r - 3 * q->res_elems) * q->row_bytes();
+            std::memcpy(buf.data(), src, size_t(n3) * q->elem_bytes);
+        }
         double* o = out + (r - row0) * n3;
         for (int64_t c = 0; c < n3; c++)
             o[c] = (q->elem_bytes == 4) ? double(reinterpret_cast<float*>(buf.data())[c])
@@ -580,7 +718,7 @@ rh_status relax_run(rh_matrix* q, const double* d_mats, const double* d_hext, do
             RH_HIP(rh::launch_matvec_t(q->d_Q, q->elem_bytes, n3, q->ld, adj->mu, adj->tpart,
                                        adj->chunks, w.y, s));
         } else {
-            RH_HIP(rh::launch_matvec(q->d_Q, q->elem_bytes, n3, q->ld, w.x, nullptr, w.y, s));
+            RH_HIP(matrix_pass(q, w.x, w.y, s));
         }
         tm.end();
         if (adj)
@@ -775,7 +913,7 @@ rh_status rh_relax_dev(rh_matrix* q, const double* d_mats, const double* d_hext,
         st = field_dev_impl(m, d_M, m->d_colloc, n, nullptr, d_H64, false, "refine");
         if (st != RH_OK) return st;
         RH_HIP(hipMemcpyAsync(w.x, d_M, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
-        RH_HIP(rh::launch_matvec(q->d_Q, q->elem_bytes, n3, q->ld, w.x, nullptr, w.y, s));
+        RH_HIP(matrix_pass(q, w.x, w.y, s));
         RH_HIP(rh::launch_refine_defect(n, d_H64, m->host.rowt_identity ? nullptr : m->d_rowt, w.y,
                                         q->d_qdiag, d_M, d_hext, d_hx, s));
         res->matvecs += 1;
@@ -1015,6 +1153,7 @@ rh_status rh_relax_adjoint_dev(rh_matrix* q, const double* d_mats, const double*
                                const rh_relax_opts* opts, rh_relax_result* res)
 {
     if (!q || !d_mats || !d_rhs || !d_lam || !opts || !res) return fail(RH_ERR_ARG, "null argument");
+    if (q->streamed()) return fail(RH_ERR_UNSUPPORTED, "relaxation adjoint needs a device-resident matrix");
     rh_model* m = q->m;
     RH_HIP(enter_device(m->device));
     hipStream_t s = m->stream;

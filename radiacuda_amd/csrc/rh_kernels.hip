@@ -44,22 +44,24 @@ template <class T>
 __device__ __forceinline__ void store_out(T* p, double v) { *p = T(v); }
 
 // ---------------------------------------------------------------- assembly
-// Block columns are the elements kidx[k0 .. k1) of one kind slot.
+// Rows of elements [r0, r1) into Q (Q holds row 3*r0 first); block columns
+// are the elements kidx[k0 .. k1) of one kind slot.
 template <class R, bool SYM, bool ROWT, class OUT, int KIND>
 __global__ __launch_bounds__(64) void k_assemble(DevModel m, OUT* __restrict__ Q, int64_t ld,
-                                                 double* __restrict__ qdiag, int64_t jt)
+                                                 double* __restrict__ qdiag, int64_t jt, int64_t r0,
+                                                 int64_t r1)
 {
-    const int64_t i = int64_t(blockIdx.x) * 64 + threadIdx.x;
+    const int64_t i = r0 + int64_t(blockIdx.x) * 64 + threadIdx.x;
     const int64_t kend = m.koff[KIND + 1];
     const int64_t k0 = m.koff[KIND] + int64_t(blockIdx.y) * jt;
     const int64_t k1 = (k0 + jt < kend) ? (k0 + jt) : kend;
-    if (i >= m.n) return;
+    if (i >= r1) return;
     const ModelView mv = view(m);
     const double x[3] = {m.colloc[3 * i], m.colloc[3 * i + 1], m.colloc[3 * i + 2]};
     double rt[9];
     if (ROWT)
         for (int k = 0; k < 9; k++) rt[k] = m.rowt[9 * i + k];
-    OUT* row = Q + (3 * i) * ld;
+    OUT* row = Q + (3 * (i - r0)) * ld;
     for (int64_t k = k0; k < k1; k++) {
         const int64_t j = m.kidx[k];
         double q[9];
@@ -86,9 +88,11 @@ __global__ __launch_bounds__(64) void k_assemble(DevModel m, OUT* __restrict__ Q
 }
 
 template <bool SYM, bool ROWT, class OUT>
-hipError_t asm_dispatch(const DevModel& m, OUT* Q, int64_t ld, double* qdiag, hipStream_t s)
+hipError_t asm_dispatch(const DevModel& m, OUT* Q, int64_t ld, double* qdiag, int64_t r0, int64_t r1,
+                        hipStream_t s)
 {
-    const int64_t gx = (m.n + 63) / 64;
+    if (r1 <= r0) return hipSuccess;
+    const int64_t gx = (r1 - r0 + 63) / 64;
     for (int slot = 0; slot < 3; slot++) {
         const int64_t cnt = m.koff[slot + 1] - m.koff[slot];
         if (cnt == 0) continue;
@@ -98,11 +102,11 @@ hipError_t asm_dispatch(const DevModel& m, OUT* Q, int64_t ld, double* qdiag, hi
         jt = std::min<int64_t>(jt, 256);
         const dim3 grid{unsigned(gx), unsigned((cnt + jt - 1) / jt), 1u};
         if (slot == 0)
-            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 0>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt);
+            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 0>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1);
         else if (slot == 1)
-            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 1>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt);
+            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 1>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1);
         else
-            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 2>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt);
+            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 2>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -749,19 +753,19 @@ __global__ __launch_bounds__(256) void k_current(int64_t ns, const int32_t* __re
 
 // ---------------------------------------------------------------- launchers
 hipError_t launch_assemble(const DevModel& m, void* Q, int elem_bytes, int64_t ld, double* qdiag,
-                           hipStream_t s)
+                           int64_t r0, int64_t r1, hipStream_t s)
 {
-    if (m.n == 0) return hipSuccess;
+    if (m.n == 0 || r1 <= r0) return hipSuccess;
     if (elem_bytes == 4) {
         float* q = static_cast<float*>(Q);
-        if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, s);
-        if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, s);
-        return asm_dispatch<true, true>(m, q, ld, qdiag, s);
+        if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, r0, r1, s);
+        if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, r0, r1, s);
+        return asm_dispatch<true, true>(m, q, ld, qdiag, r0, r1, s);
     }
     double* q = static_cast<double*>(Q);
-    if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, s);
-    if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, s);
-    return asm_dispatch<true, true>(m, q, ld, qdiag, s);
+    if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, r0, r1, s);
+    if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, r0, r1, s);
+    return asm_dispatch<true, true>(m, q, ld, qdiag, r0, r1, s);
 }
 
 namespace {
@@ -881,6 +885,7 @@ hipError_t launch_field(const DevModel& m0, const double* M, const double* pts, 
 hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t ld, const double* x,
                          const double* add, double* y, hipStream_t s)
 {
+    if (nrows <= 0) return hipSuccess;
     constexpr int RW = 4;
     const int64_t waves = (nrows + RW - 1) / RW;
     const int64_t blocks = (waves + 3) / 4;

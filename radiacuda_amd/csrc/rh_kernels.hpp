@@ -48,11 +48,11 @@ struct DevModel {
     bool use_mesh = false;  // field sweeps use the mesh form (when active)
 };
 
-// Interaction matrix columns [col0, col1) of all rows into Q (row-major, leading
-// dimension ld elements of elem_bytes = 4 or 8); the 3x3 diagonal blocks go to
-// qdiag [9n] in FP64 and are stored as zeros in Q.
+// Interaction-matrix rows of elements [r0, r1) into Q (row-major, leading
+// dimension ld elements of elem_bytes = 4 or 8; Q points at row 3*r0); the
+// 3x3 diagonal blocks go to qdiag [9n] in FP64 and are stored as zeros in Q.
 hipError_t launch_assemble(const DevModel& m, void* Q, int elem_bytes, int64_t ld, double* qdiag,
-                           hipStream_t s);
+                           int64_t r0, int64_t r1, hipStream_t s);
 
 // Number of partial-sum slices the field kernels use for np points (>= 1);
 // the work buffer needs 6 * np * slices doubles when slices > 1.

@@ -379,8 +379,7 @@ def RlxPre(obj, srcobj=0, use_gpu=True):
     it.items = rel
     it.packed = O.pack_elements(rel)
     it.model = N.Model(it.packed)
-    it.matrix = it.model.assemble(N.RH_MIXED)
-    _last_asm[0] = "gpu"
+    it.matrix = _assemble_with_policy(it.model)
     X = it.model.colloc()
     Hlab = _ext_field_at(ext, X) if ext else np.zeros_like(X)
     # into element frames: s * Minv of the main copy (radintrc.cpp:1231)
@@ -393,6 +392,36 @@ def RlxPre(obj, srcobj=0, use_gpu=True):
     it.mats = np.stack([o.mat.record(o.M) for (o, _k, _s) in rel])
     it.H = np.zeros_like(hext)
     return _add(it)
+
+
+_ERROR602 = ("Radia::Error602::::GPU interaction-matrix assembly could not be completed "
+             "(e.g. insufficient GPU memory) and the GPU fallback policy is 'break'. Reduce the "
+             "model size, or set UtiGpuFallback('gpu_streaming') to stream the matrix rows that "
+             "do not fit from host memory.")
+
+
+def _assemble_with_policy(model):
+    """Device assembly under the UtiGpuFallback policy (radintrc.cpp:796-805):
+    'break' refuses a matrix beyond the device budget; 'gpu_streaming' and
+    'cpu' keep the rows that fit resident and stream the rest from pinned
+    host memory (there is no CPU assembly here: the device streams instead).
+    RADHIP_MATRIX_DEVICE_BYTES and RADHIP_MATRIX_RESIDENT_FRACTION override
+    the budget and the split (as the reference's RADGPU_FORCE_STREAM hooks)."""
+    import os
+    budget = int(os.environ.get("RADHIP_MATRIX_DEVICE_BYTES", "0"))
+    frac = float(os.environ.get("RADHIP_MATRIX_RESIDENT_FRACTION", "-1"))
+    stream = _fallback[0] in ("gpu_streaming", "cpu")
+    try:
+        Q = model.assemble(N.RH_MIXED, device_bytes=budget, resident_fraction=frac,
+                           allow_streaming=stream or frac >= 0)
+    except N.RadHipError as e:
+        if "exceeds the device budget" in str(e) or "does not fit" in str(e):
+            _last_asm[0] = "none"
+            raise RuntimeError(_ERROR602) from e
+        raise
+    _res, streamed = Q.layout()
+    _last_asm[0] = "gpu_streaming" if streamed else "gpu"
+    return Q
 
 
 def _opt(opts, name, default):
