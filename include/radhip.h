@@ -140,10 +140,11 @@ rh_status rh_matrix_reassemble(rh_matrix *q);
 
 /* Per-element material record, 16 doubles (radmater.h:164-696):
  *   [0] kind: 0 linear isotropic, 1 linear anisotropic, 2 nonlinear isotropic
- *       (tanh-sum formula MatSatIsoFrm)
+ *       (tanh-sum formula MatSatIsoFrm), 3 nonlinear isotropic (table, MatSatIsoTab)
  *   kind 0: [1] ksi
  *   kind 1: [4..12] susceptibility tensor, row-major (element frame)
  *   kind 2: [1] number of terms (1..3), [2..4] ms_i, [5..7] ksi_i
+ *   kind 3: [1] first node, [2] node count of a tabulated law (rh_relax_tab)
  *   [13..15] remanent magnetization (element frame) */
 #define RH_MAT_STRIDE 16
 
@@ -181,6 +182,20 @@ rh_status rh_relax(rh_matrix *q, const double *mats, const double *hext, double 
 /* Same on device-resident vectors (d_mats [16n], d_hext, d_M, d_H [3n] doubles). */
 rh_status rh_relax_dev(rh_matrix *q, const double *d_mats, const double *d_hext, double *d_M,
                        double *d_H, const rh_relax_opts *opts, rh_relax_result *res);
+
+/* Relaxation with tabulated isotropic laws (MatSatIsoTab, radmater.h:436-451;
+ * GPU counterpart interp_mh_dev, cpp/src/core/radgpurlx.cu:83-100): material
+ * kind 3 records hold [1] the first node and [2] the node count of a law in
+ * tab, a flat array of ntab doubles = node triples (|H|, |M|, d|M|/d|H|) as
+ * the reference builds them (radmater.cpp:460-609).  |M| is the cubic Hermite
+ * interpolant between nodes, |M(0)| below the first node and the last slope
+ * beyond the last (radmater.cpp:619-646). */
+rh_status rh_relax_tab(rh_matrix *q, const double *mats, const double *tab, int64_t ntab,
+                       const double *hext, double *M_inout, double *H_out, const rh_relax_opts *opts,
+                       rh_relax_result *res);
+rh_status rh_relax_tab_dev(rh_matrix *q, const double *d_mats, const double *d_tab, int64_t ntab,
+                           const double *d_hext, double *d_M, double *d_H, const rh_relax_opts *opts,
+                           rh_relax_result *res);
 
 /* ------------------------------------------------------------------------ */
 /* Field evaluation                                                           */

@@ -58,6 +58,11 @@ def lib():
                                     ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_int, _d, _d]
         L.ora_inv3.argtypes = [_d, _d]
+        L.ora_set_tables.argtypes = [_d]
+        L.ora_tab_prepare.argtypes = [_d, ctypes.c_int, _d, _d, _d]
+        L.ora_tab_prepare.restype = ctypes.c_int
+        L.ora_tab_eval.argtypes = [_d, ctypes.c_int, ctypes.c_double]
+        L.ora_tab_eval.restype = ctypes.c_double
         _LIB = L
     return _LIB
 
@@ -229,6 +234,33 @@ def field(m, Mloc, pts):
 
 # ---------------------------------------------------------------- materials / relaxation
 MAT_STRIDE = 16
+
+
+_TABLES = [None]
+
+
+def tab_prepare(points):
+    """Node table (x, y, d) of a tabulated law from (H, M) points (radmater.cpp:460-609)."""
+    pts = _v(points).reshape(-1, 2)
+    n = len(pts)
+    x, y, d = np.zeros(n + 1), np.zeros(n + 1), np.zeros(n + 1)
+    L = lib().ora_tab_prepare(_p(pts.ravel()), n, _p(x), _p(y), _p(d))
+    if L < 0:
+        raise ValueError("a tabulated law needs at least 4 nodes")
+    return x[:L], y[:L], d[:L]
+
+
+def tab_eval(table, h):
+    """|M|(|H|) of a node table given as flat (x, y, d) triples."""
+    t = _v(table).ravel()
+    return lib().ora_tab_eval(_p(t), len(t) // 3, float(h))
+
+
+def set_tables(tab):
+    """Flat (x, y, d) triples that kind-3 material records index."""
+    t = None if tab is None else _v(tab).ravel().copy()
+    _TABLES[0] = t  # keep alive while the oracle holds the pointer
+    lib().ora_set_tables(_p(t) if t is not None else None)
 
 
 def mat_M(mat, H):

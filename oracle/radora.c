@@ -693,6 +693,132 @@ void ora_model_colloc(ORA_MODEL_ARGS, double *X)
  */
 #define MAT_STRIDE 16
 
+/* ------------------------------------------------------------------------- */
+/* Tabulated isotropic law (MatSatIsoTab): radmater.h:436-451, 611-641,       */
+/* radmater.cpp:341-646.  Nodes (x = |H|, y = |M|) with node derivatives d;   */
+/* kind 3 records hold [1] first node (index into the table of x,y,d          */
+/* triples set by ora_set_tables), [2] node count.                            */
+/* ------------------------------------------------------------------------- */
+static const double *g_tab = 0;
+void ora_set_tables(const double *tab) { g_tab = tab; }
+
+/* Derivative5: radmater.cpp:341-424 (derivative of the 5-point interpolant at
+ * node k, clamped to 0 for increasing data). */
+static double tab_der5(const double *x, const double *y, int k)
+{
+    double x0 = x[0], x1 = x[1] - x0, x2 = x[2] - x0, x3 = x[3] - x0, x4 = x[4] - x0;
+    double f0 = y[0], f1 = y[1], f2 = y[2], f3 = y[3], f4 = y[4];
+    int inc = (x1 > 0. && x2 > x1 && x3 > x2 && x4 > x3) &&
+              (f0 <= f1 && f1 <= f2 && f2 <= f3 && f3 <= f4);
+    double x1m2 = x1 - x2, x1m3 = x1 - x3, x1m4 = x1 - x4, x2m3 = x2 - x3, x2m4 = x2 - x4, x3m4 = x3 - x4;
+    double x1e2 = x1*x1, x2e2 = x2*x2, x3e2 = x3*x3, x4e2 = x4*x4;
+    double den = x1*x1m2*x2*x1m3*x2m3*x3*x1m4*x2m4*x3m4*x4, der = 0.;
+    if (k == 0) {
+        der = (f4*x1e2*x1m2*x2e2*x1m3*x2m3*x3e2 +
+               (-f3*x1e2*x1m2*x2e2*x1m4*x2m4 + x3e2*(f2*x1e2*x1m3*x1m4 - f1*x2e2*x2m3*x2m4)*x3m4)*x4e2 -
+               f0*x1m2*x1m3*x2m3*x1m4*x2m4*x3m4*(x1*x2*x3 + x2*x3*x4 + x1*(x2 + x3)*x4))/den;
+    } else if (k == 1) {
+        double x1e3 = x1e2*x1, x1m2e2 = x1m2*x1m2, x1m3e2 = x1m3*x1m3, x1m4e2 = x1m4*x1m4;
+        der = (-f4*x1e2*x1m2e2*x2*x1m3e2*x2m3*x3 + f0*x1m2e2*x1m3e2*x2m3*x1m4e2*x2m4*x3m4 +
+               x4*(f3*x1e2*x1m2e2*x2*x1m4e2*x2m4 -
+                   x3*x3m4*(f2*x1e2*x1m3e2*x1m4e2 + f1*x2*x2m3*x2m4*
+                            (-4*x1e3 + x2*x3*x4 + 3*x1e2*(x2 + x3 + x4) - 2*x1*(x3*x4 + x2*(x3 + x4))))))/den;
+    } else if (k == 2) {
+        double x1m2e2 = x1m2*x1m2, x2m3e2 = x2m3*x2m3, x2m4e2 = x2m4*x2m4;
+        der = (f4*x1*x1m2e2*x2e2*x1m3*x2m3e2*x3 - f0*x1m2e2*x1m3*x2m3e2*x1m4*x2m4e2*x3m4 +
+               x4*(-f3*x1*x1m2e2*x2e2*x1m4*x2m4e2 +
+                   x3*x3m4*(f1*x2e2*x2m3e2*x2m4e2 + f2*x1*x1m3*x1m4*
+                            (x1*(3*x2e2 + x3*x4 - 2*x2*(x3 + x4)) + x2*(-4*x2e2 - 2*x3*x4 + 3*x2*(x3 + x4))))))/den;
+    }
+    if (inc && der < 0.) der = 0.;
+    return der;
+}
+
+/* Derivative3: radmater.cpp:426-455 */
+static double tab_der3(const double *x, const double *y, int k)
+{
+    double x1 = x[1] - x[0], x2 = x[2] - x[0];
+    double f0 = y[0], f1 = y[1], f2 = y[2];
+    int inc = (x1 > 0. && x2 > x1) && (f0 <= f1 && f1 <= f2);
+    double x1e2 = x1*x1, x2e2 = x2*x2, x1m2 = x1 - x2, der = 0.;
+    if (k == 0) der = (f0*x1e2 - f2*x1e2 - f0*x2e2 + f1*x2e2)/(-x1*x2*x1m2);
+    else if (k == 1) der = (-f2*x1e2 + f0*x1m2*x1m2 + f1*(2*x1 - x2)*x2)/(x1*x1m2*x2);
+    else if (k == 2) der = (f2*x1*(x1 - 2*x2) - f0*x1m2*x1m2 + f1*x2e2)/(x1*x1m2*x1m2*x2);
+    if (inc && der < 0.) der = 0.;
+    return der;
+}
+
+/* Node table from user points (H, M): prepends (0, 0) when both first values
+ * exceed 1e-10 (radmater.h:436-451); node derivatives as Compute_dMdH
+ * (radmater.cpp:460-550, including its index pattern for 4 to 6 nodes) and the
+ * monotone correction CheckAndCorrect_dMdH (radmater.cpp:556-609).  Writes
+ * x, y, d (capacity n + 1); returns the node count. */
+int ora_tab_prepare(const double *pts, int n, double *x, double *y, double *d)
+{
+    int L = 0;
+    if (pts[0] > 1e-10 && pts[1] > 1e-10) { x[0] = 0.; y[0] = 0.; L = 1; }
+    for (int i = 0; i < n; i++) { x[L] = pts[2*i]; y[L] = pts[2*i+1]; L++; }
+    if (L < 4) return -1;  /* the node derivatives read 4 nodes */
+    int w = 0, t = 0;
+    d[w++] = tab_der3(x, y, 0);
+    d[w++] = tab_der3(x, y, 1);
+    t = 1;
+    d[w++] = tab_der3(x + t, y + t, 1);
+    for (int i = 3; i < L - 4; i++) { d[w++] = tab_der5(x + t, y + t, 2); t++; }
+    if (L >= 7) {
+        d[w++] = tab_der5(x + t, y + t, 2); t += 2;
+        d[w++] = tab_der3(x + t, y + t, 1); t++;
+        d[w++] = tab_der3(x + t, y + t, 1); t++;
+        d[w] = (y[t+1] - y[t]) / (x[t+1] - x[t]);
+    } else if (L >= 6) {
+        d[w++] = tab_der5(x + t, y + t, 2); t += 2;
+        d[w++] = tab_der3(x + t, y + t, 1); t++;
+        d[w] = (y[t+1] - y[t]) / (x[t+1] - x[t]);
+    } else if (L >= 5) {
+        d[w++] = tab_der3(x + t, y + t, 1); t++;
+        d[w] = (y[t+1] - y[t]) / (x[t+1] - x[t]);
+    } else if (L >= 4) {
+        d[w] = (y[t+1] - y[t]) / (x[t+1] - x[t]);
+    }
+    for (int i = L - 2; i >= 0; i--) {
+        double x2 = x[i+1] - x[i], f1 = y[i], f2 = y[i+1], fd1 = d[i], fd2 = d[i+1];
+        double x2e2 = x2*x2, x2e3 = x2e2*x2, fm = f1 - f2;
+        double a1 = fd1, a2 = -((3*fm + (2*fd1 + fd2)*x2)/x2e2), a3 = -((-2*fm - (fd1 + fd2)*x2)/x2e3);
+        double D = a2*a2 - 3.*a1*a3, xc1 = 1.E+23, xc2 = 1.E+23;
+        if (a3 != 0. && D >= 0.) {
+            double R = sqrt(D), b = 1./(3.*a3);
+            xc1 = (-a2 - R)*b; xc2 = (-a2 + R)*b;
+        }
+        int mono = (f1 < f2 && fd1 > 0. && fd2 > 0.) || (f1 > f2 && fd1 < 0. && fd2 < 0.);
+        int in1 = 0, in2 = 0;
+        if (a3 != 0.) { in1 = (0. < xc1 && xc1 < x2); in2 = (0. < xc2 && xc2 < x2); }
+        if (mono && D > 0. && (in1 || in2)) {
+            double LD = -3*fd2*x2e3*(4*fm + fd2*x2);
+            if (LD >= 0.) {
+                double LR = sqrt(LD), b1 = 1./(2*x2e2), b2 = -x2*(6*fm + fd2*x2);
+                double fa = (b2 - LR)*b1, fb = (b2 + LR)*b1;
+                d[i] = (fabs(fd1 - fa) < fabs(fd1 - fb)) ? fa : fb;
+            }
+        }
+    }
+    return L;
+}
+
+/* AbsMvsAbsH_Interpol: radmater.cpp:619-646 with CubPln (radmater.h:631-639). */
+double ora_tab_eval(const double *tab, int L, double h)
+{
+    int idx = 0;
+    while (idx < L && !(tab[3*idx] > h)) idx++;
+    idx--;
+    if (idx < 0) return tab[1];
+    if (idx >= L - 1) return tab[3*(L-1)+1] + (h - tab[3*(L-1)]) * tab[3*(L-1)+2];
+    double arg = h - tab[3*idx], step = tab[3*(idx+1)] - tab[3*idx];
+    double f1 = tab[3*idx+1], f2 = tab[3*(idx+1)+1], p1 = tab[3*idx+2], p2 = tab[3*(idx+1)+2];
+    double inv = 1./step, s = (f2 - f1)*inv;
+    double a2 = (3.*s - 2.*p1 - p2)*inv, a3 = (-2.*s + p1 + p2)*inv*inv;
+    return f1 + arg*(p1 + arg*(a2 + arg*a3));
+}
+
 /* M(H): radmater.h:224 (aniso), :351 (iso), :644-657 (nonlinear) */
 void ora_mat_M(const double *mat, const double H[3], double M[3])
 {
@@ -707,7 +833,8 @@ void ora_mat_M(const double *mat, const double H[3], double M[3])
         int nt = (int)mat[1];
         double absH = sqrt(H[0]*H[0] + H[1]*H[1] + H[2]*H[2]);
         double absM = 0.;
-        for (int i = 0; i < nt; i++) {
+        if (kind == 3) absM = ora_tab_eval(g_tab + 3 * (int64_t)mat[1], (int)mat[2], absH);
+        else for (int i = 0; i < nt; i++) {
             double ms = mat[2 + i], ks = mat[5 + i];
             if (ms != 0.) absM += ms * tanh(ks * absH / ms);
         }
@@ -744,9 +871,13 @@ static void find_new_H(const double *mat, double H[3], const double Q[9], const 
     double misfit = 3;
     double absH = sqrt(H[0]*H[0] + H[1]*H[1] + H[2]*H[2]);
     double f, inst_ksi = 0.;
+    const double *tab = (kind == 3) ? g_tab + 3 * (int64_t)mat[1] : 0;
     for (int it = 0; it < max_iter; it++) {
         f = 0.;
-        if (absH <= abs_h_zero_tol) {
+        if (tab) {
+            if (absH <= abs_h_zero_tol) { inst_ksi = tab[2]; f = 0; }
+            else { f = ora_tab_eval(tab, (int)mat[2], absH); inst_ksi = f / absH; }
+        } else if (absH <= abs_h_zero_tol) {
             for (int j = 0; j < nt; j++) inst_ksi += mat[5 + j];
             f = 0;
         } else {

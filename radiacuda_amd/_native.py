@@ -61,6 +61,10 @@ def lib():
                            ctypes.POINTER(_abi.RelaxResult)]
     L.rh_relax_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_abi.RelaxOpts),
                                ctypes.POINTER(_abi.RelaxResult)]
+    L.rh_relax_tab.argtypes = [_vp, _d, _d, ctypes.c_int64, _d, _d, _d, ctypes.POINTER(_abi.RelaxOpts),
+                               ctypes.POINTER(_abi.RelaxResult)]
+    L.rh_relax_tab_dev.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, _vp, _vp,
+                                   ctypes.POINTER(_abi.RelaxOpts), ctypes.POINTER(_abi.RelaxResult)]
     L.rh_field.argtypes = [_vp, _d, _d, ctypes.c_int64, _d, _d, ctypes.c_int]
     L.rh_field_dev.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, _vp, ctypes.c_int]
     L.rh_current_field.argtypes = [ctypes.c_int, _vp, ctypes.c_int64, _i32, _d, _d, _d, _d, _d,
@@ -87,7 +91,8 @@ def lib():
                  "rh_set_timing", "rh_kernel_time", "rh_model_vertex_count",
                  "rh_field_adjoint", "rh_field_adjoint_dev", "rh_relax_adjoint_dev",
                  "rh_relax_adjoint", "rh_model_set_field_form", "rh_model_field_form",
-                 "rh_model_mesh_counts", "rh_matrix_assemble_ex", "rh_matrix_layout"]:
+                 "rh_model_mesh_counts", "rh_matrix_assemble_ex", "rh_matrix_layout",
+                 "rh_relax_tab", "rh_relax_tab_dev"]:
         getattr(L, name).restype = st
     if L.rh_device_count() < 1:
         raise RadHipError("no HIP device visible; the HIP path has no CPU fallback")
@@ -254,18 +259,20 @@ class Matrix:
         check(lib().rh_matrix_export(self.h, row0, nrows, _p(out)))
         return out
 
-    def relax(self, mats, hext, M0, prec, max_iter, keep_m=False, omega0=0.0, refine=0):
+    def relax(self, mats, hext, M0, prec, max_iter, keep_m=False, omega0=0.0, refine=0, tab=None):
         """refine: FP64 refinement passes of a mixed-precision matrix
-        (0 = library default of one pass, -1 = none)."""
+        (0 = library default of one pass, -1 = none); tab: node table of the
+        tabulated laws that kind-3 records index (objects.pack_materials)."""
         mats = _c(mats).reshape(-1)
+        tab = _c(np.zeros(0) if tab is None else tab).reshape(-1)
         hext = _c(hext).reshape(-1)
         M = _c(M0).reshape(-1).copy()
         H = np.zeros_like(M)
         opts = _abi.RelaxOpts(float(prec), int(max_iter), 1 if keep_m else 0, float(omega0), 9,
                               int(refine))
         res = _abi.RelaxResult()
-        check(lib().rh_relax(self.h, _p(mats), _p(hext), _p(M), _p(H), ctypes.byref(opts),
-                             ctypes.byref(res)))
+        check(lib().rh_relax_tab(self.h, _p(mats), _p(tab) if len(tab) else None, len(tab), _p(hext),
+                                 _p(M), _p(H), ctypes.byref(opts), ctypes.byref(res)))
         return res, M.reshape(-1, 3), H.reshape(-1, 3)
 
     def relax_adjoint(self, mats, rhs, prec, max_iter, lam0=None):
@@ -295,8 +302,9 @@ def current_field(cur, pts, device=0):
     return B
 
 
-def mat_M(rec, H):
-    """M(H) of a material record (radmater.h:224, :351, :644-657)."""
+def mat_M(rec, H, tab=None):
+    """M(H) of a material record (radmater.h:224, :351, :644-657; tabulated
+    laws radmater.cpp:619-646 with the node table tab)."""
     rec = np.asarray(rec, dtype=np.float64)
     H = np.asarray(H, dtype=np.float64)
     kind = int(rec[0])
@@ -306,6 +314,10 @@ def mat_M(rec, H):
     if kind == 1:
         return rec[4:13].reshape(3, 3) @ H + mr
     h = float(np.linalg.norm(H))
+    if kind == 3:
+        from .objects import tab_eval
+        t = np.asarray(tab, dtype=np.float64)[3 * int(rec[1]):3 * int(rec[1] + rec[2])].reshape(-1, 3)
+        return (tab_eval(t[:, 0], t[:, 1], t[:, 2], h) / h) * H + mr if h != 0 else mr.copy()
     f = 0.0
     for t in range(int(rec[1])):
         ms, ks = rec[2 + t], rec[5 + t]

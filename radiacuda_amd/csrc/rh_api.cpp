@@ -693,8 +693,8 @@ struct AdjointPass {
     int64_t chunks = 0;
 };
 
-rh_status relax_run(rh_matrix* q, const double* d_mats, const double* d_hext, double* d_M,
-                    double* d_H, const rh_relax_opts* opts, RelaxWork& w, double* hbuf,
+rh_status relax_run(rh_matrix* q, const double* d_mats, const double* d_tab, const double* d_hext,
+                    double* d_M, double* d_H, const rh_relax_opts* opts, RelaxWork& w, double* hbuf,
                     int64_t nb, int64_t db, rh_relax_result* res,
                     const AdjointPass* adj = nullptr, bool keep_if_converged = false)
 {
@@ -724,7 +724,8 @@ rh_status relax_run(rh_matrix* q, const double* d_mats, const double* d_hext, do
         if (adj)
             RH_HIP(rh::launch_adj_update(n, d_mats, d_hext, q->d_qdiag, w.y, w.x, w.F, w.part, s));
         else
-            RH_HIP(rh::launch_relax_update(n, d_mats, d_hext, q->d_qdiag, w.y, w.x, w.F, d_H, w.part, s));
+            RH_HIP(rh::launch_relax_update(n, d_mats, d_tab, d_hext, q->d_qdiag, w.y, w.x, w.F, d_H, w.part,
+                                           s));
         // tentatively extend the history with the newest differences
         if (it > 1) {
             const double* v1[2] = {w.F, w.Fp};
@@ -846,7 +847,15 @@ rh_status relax_run(rh_matrix* q, const double* d_mats, const double* d_hext, do
 rh_status rh_relax_dev(rh_matrix* q, const double* d_mats, const double* d_hext, double* d_M,
                        double* d_H, const rh_relax_opts* opts, rh_relax_result* res)
 {
+    return rh_relax_tab_dev(q, d_mats, nullptr, 0, d_hext, d_M, d_H, opts, res);
+}
+
+rh_status rh_relax_tab_dev(rh_matrix* q, const double* d_mats, const double* d_tab, int64_t ntab,
+                           const double* d_hext, double* d_M, double* d_H, const rh_relax_opts* opts,
+                           rh_relax_result* res)
+{
     if (!q || !d_mats || !d_hext || !d_M || !d_H || !opts || !res) return fail(RH_ERR_ARG, "null argument");
+    if (ntab < 0 || (ntab > 0 && !d_tab)) return fail(RH_ERR_ARG, "bad table");
     if (opts->method != 9 && opts->method != 0) return fail(RH_ERR_UNSUPPORTED, "relaxation method must be 9");
     rh_model* m = q->m;
     RH_HIP(enter_device(m->device));
@@ -856,6 +865,21 @@ rh_status rh_relax_dev(rh_matrix* q, const double* d_mats, const double* d_hext,
     if (n == 0) {
         res->converged = 1;
         return RH_OK;
+    }
+    {
+        // material kinds and table ranges (tabulated laws index d_tab)
+        std::vector<double> recs(size_t(16 * n), 0.0);
+        RH_HIP(hipMemcpyAsync(recs.data(), d_mats, recs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        RH_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n; i++) {
+            const double* r = recs.data() + 16 * i;
+            const int kind = int(r[0]);
+            if (kind < 0 || kind > 3 || double(kind) != r[0])
+                return fail(RH_ERR_UNSUPPORTED, "unknown material kind in record " + std::to_string(i));
+            if (kind == 2 && !(r[1] >= 1 && r[1] <= 3)) return fail(RH_ERR_ARG, "tanh law needs 1 to 3 terms");
+            if (kind == 3 && !(r[1] >= 0 && r[2] >= 2 && 3 * (r[1] + r[2]) <= double(ntab)))
+                return fail(RH_ERR_ARG, "tabulated law of record " + std::to_string(i) + " outside the table");
+        }
     }
     RelaxWork w;
     RH_HIP(dalloc(&w.x, size_t(q->ld)));
@@ -887,7 +911,7 @@ rh_status rh_relax_dev(rh_matrix* q, const double* d_mats, const double* d_hext,
         double* p;
         ~PinnedFree() { (void)hipHostFree(p); }
     } pinned_guard{hbuf};
-    rh_status st = relax_run(q, d_mats, d_hext, d_M, d_H, opts, w, hbuf, nb, db, res);
+    rh_status st = relax_run(q, d_mats, d_tab, d_hext, d_M, d_H, opts, w, hbuf, nb, db, res);
     if (st != RH_OK) return st;
     // FP64 refinement of a solve on the FP32-stored matrix: the defect between
     // the FP64 field at the collocation points and the stored-matrix field
@@ -918,7 +942,7 @@ rh_status rh_relax_dev(rh_matrix* q, const double* d_mats, const double* d_hext,
                                         q->d_qdiag, d_M, d_hext, d_hx, s));
         res->matvecs += 1;
         // a magnetization that already meets prec on the corrected system is kept
-        st = relax_run(q, d_mats, d_hx, d_M, d_H, &o2, w, hbuf, nb, db, res, nullptr, true);
+        st = relax_run(q, d_mats, d_tab, d_hx, d_M, d_H, &o2, w, hbuf, nb, db, res, nullptr, true);
         if (st != RH_OK) return st;
         res->refined += 1;
     }
@@ -928,22 +952,34 @@ rh_status rh_relax_dev(rh_matrix* q, const double* d_mats, const double* d_hext,
 rh_status rh_relax(rh_matrix* q, const double* mats, const double* hext, double* M_inout,
                    double* H_out, const rh_relax_opts* opts, rh_relax_result* res)
 {
+    return rh_relax_tab(q, mats, nullptr, 0, hext, M_inout, H_out, opts, res);
+}
+
+rh_status rh_relax_tab(rh_matrix* q, const double* mats, const double* tab, int64_t ntab,
+                       const double* hext, double* M_inout, double* H_out, const rh_relax_opts* opts,
+                       rh_relax_result* res)
+{
     if (!q || !mats || !hext || !M_inout || !H_out) return fail(RH_ERR_ARG, "null argument");
+    if (ntab < 0 || (ntab > 0 && !tab)) return fail(RH_ERR_ARG, "bad table");
     rh_model* m = q->m;
     RH_HIP(enter_device(m->device));
     const int64_t n = m->host.n;
-    double *dm = nullptr, *dh = nullptr, *dM = nullptr, *dH = nullptr;
+    double *dm = nullptr, *dh = nullptr, *dM = nullptr, *dH = nullptr, *dt = nullptr;
     hipStream_t s = m->stream;
     auto cleanup = [&]() {
         dfree(dm);
         dfree(dh);
         dfree(dM);
         dfree(dH);
+        dfree(dt);
     };
     hipError_t e = dalloc(&dm, size_t(16 * n));
     if (e == hipSuccess) e = dalloc(&dh, size_t(3 * n));
     if (e == hipSuccess) e = dalloc(&dM, size_t(3 * n));
     if (e == hipSuccess) e = dalloc(&dH, size_t(3 * n));
+    if (e == hipSuccess && ntab > 0) e = dalloc(&dt, size_t(ntab));
+    if (e == hipSuccess && ntab > 0)
+        e = hipMemcpyAsync(dt, tab, size_t(ntab) * sizeof(double), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(dm, mats, size_t(16 * n) * sizeof(double), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(dh, hext, size_t(3 * n) * sizeof(double), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(dM, M_inout, size_t(3 * n) * sizeof(double), hipMemcpyHostToDevice, s);
@@ -951,7 +987,7 @@ rh_status rh_relax(rh_matrix* q, const double* mats, const double* hext, double*
         cleanup();
         return fail(RH_ERR_HIP, std::string("relax upload: ") + hipGetErrorString(e));
     }
-    rh_status st = rh_relax_dev(q, dm, dh, dM, dH, opts, res);
+    rh_status st = rh_relax_tab_dev(q, dm, dt, ntab, dh, dM, dH, opts, res);
     if (st == RH_OK) {
         e = hipMemcpyAsync(M_inout, dM, size_t(3 * n) * sizeof(double), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipMemcpyAsync(H_out, dH, size_t(3 * n) * sizeof(double), hipMemcpyDeviceToHost, s);
@@ -1209,8 +1245,8 @@ rh_status rh_relax_adjoint_dev(rh_matrix* q, const double* d_mats, const double*
         double* p;
         ~PinnedFree() { (void)hipHostFree(p); }
     } pinned_guard{hbuf};
-    return relax_run(q, d_mats, d_rhs, d_lam, w.y /* unused field output */, opts, w, hbuf, nb, db,
-                     res, &ap);
+    return relax_run(q, d_mats, nullptr, d_rhs, d_lam, w.y /* unused field output */, opts, w, hbuf, nb,
+                     db, res, &ap);
 }
 
 rh_status rh_relax_adjoint(rh_matrix* q, const double* mats, const double* rhs, double* lam,

@@ -530,8 +530,40 @@ __device__ __forceinline__ double frm_chi(const double* mat, double h)
     return chi;
 }
 
+// |M|(h) of a tabulated law (node triples x, y, d): cubic Hermite between
+// nodes, y[0] below the first, the last slope beyond the last
+// (radmater.cpp:619-646)
+__device__ __forceinline__ double tab_f(const double* t, int L, double h)
+{
+    if (h < t[0]) return t[1];
+    if (h >= t[3 * (L - 1)]) return t[3 * (L - 1) + 1] + (h - t[3 * (L - 1)]) * t[3 * (L - 1) + 2];
+    int lo = 0, hi = L - 1;  // t[3 lo] <= h < t[3 hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (t[3 * mid] <= h) lo = mid;
+        else hi = mid;
+    }
+    const double* a = t + 3 * lo;
+    const double s = a[3] - a[0], u = h - a[0];
+    const double sl = (a[4] - a[1]) / s;
+    const double c2 = (3.0 * sl - 2.0 * a[2] - a[5]) / s;
+    const double c3 = (-2.0 * sl + a[2] + a[5]) / (s * s);
+    return a[1] + u * (a[2] + u * (c2 + u * c3));
+}
+
+// secant susceptibility |M|(h)/h of an isotropic nonlinear law (kind 2: tanh
+// sum, kind 3: table)
+__device__ __forceinline__ double law_chi(const double* mat, const double* tab, double h)
+{
+    if (int(mat[0]) == 3) {
+        const double* t = tab + 3 * int64_t(mat[1]);
+        return (h > 0.0) ? tab_f(t, int(mat[2]), h) / h : t[2];
+    }
+    return frm_chi(mat, h);
+}
+
 __device__ void local_solve(const double* mat, const double* q, const double* heff, double* Mp,
-                            double* Hl)
+                            double* Hl, const double* tab)
 {
     const int kind = int(mat[0]);
     const double* mr = mat + 13;
@@ -566,7 +598,7 @@ __device__ void local_solve(const double* mat, const double* q, const double* he
         return;
     }
     auto phi = [&](double h, double* Hv) -> double {
-        const double chi = frm_chi(mat, h);
+        const double chi = law_chi(mat, tab, h);
         double A[9];
         for (int k = 0; k < 9; k++) A[k] = chi * q[k];
         solve_IminusA(A, b, Hv);
@@ -605,7 +637,7 @@ __device__ void local_solve(const double* mat, const double* q, const double* he
     }
     phi(h, Hv);
     const double hn = sqrt(Hv[0] * Hv[0] + Hv[1] * Hv[1] + Hv[2] * Hv[2]);
-    const double chi = frm_chi(mat, hn);
+    const double chi = law_chi(mat, tab, hn);
     for (int k = 0; k < 3; k++) {
         Hl[k] = Hv[k];
         Mp[k] = chi * Hv[k] + mr[k];
@@ -639,6 +671,7 @@ __global__ __launch_bounds__(256) void k_refine_defect(int64_t n, const double* 
 }
 
 __global__ __launch_bounds__(256) void k_relax_update(int64_t n, const double* __restrict__ mats,
+                                                      const double* __restrict__ tab,
                                                       const double* __restrict__ hext,
                                                       const double* __restrict__ qdiag,
                                                       const double* __restrict__ y,
@@ -654,7 +687,7 @@ __global__ __launch_bounds__(256) void k_relax_update(int64_t n, const double* _
         const double heff[3] = {y[3 * i] + hext[3 * i], y[3 * i + 1] + hext[3 * i + 1],
                                 y[3 * i + 2] + hext[3 * i + 2]};
         double mp[3], hl[3];
-        local_solve(mats + 16 * i, qdiag + 9 * i, heff, mp, hl);
+        local_solve(mats + 16 * i, qdiag + 9 * i, heff, mp, hl, tab);
         for (int k = 0; k < 3; k++) {
             const double d = mp[k] - M[3 * i + k];
             F[3 * i + k] = d;
@@ -900,12 +933,12 @@ hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t l
 
 int64_t relax_blocks(int64_t n) { return (n + 255) / 256; }
 
-hipError_t launch_relax_update(int64_t n, const double* mats, const double* hext,
+hipError_t launch_relax_update(int64_t n, const double* mats, const double* tab, const double* hext,
                                const double* qdiag, const double* y, const double* M, double* F,
                                double* Hout, double* partials, hipStream_t s)
 {
     hipLaunchKernelGGL(k_relax_update, dim3(unsigned(relax_blocks(n))), dim3(256), 0, s, n, mats,
-                       hext, qdiag, y, M, F, Hout, partials);
+                       tab, hext, qdiag, y, M, F, Hout, partials);
     return hipGetLastError();
 }
 

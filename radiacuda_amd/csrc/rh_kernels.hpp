@@ -72,7 +72,8 @@ hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t l
 // H = y + hext + Qii M_prop solved locally; F = M_prop - M.  partials gets
 // per-block [sum |F|^2, max |M_prop|, max |H|].
 int64_t relax_blocks(int64_t n);
-hipError_t launch_relax_update(int64_t n, const double* mats, const double* hext,
+// tab: node triples of tabulated laws (material kind 3), may be null otherwise.
+hipError_t launch_relax_update(int64_t n, const double* mats, const double* tab, const double* hext,
                                const double* qdiag, const double* y, const double* M, double* F,
                                double* Hout, double* partials, hipStream_t s);
 

@@ -188,9 +188,145 @@ class Material:
             r[1] = len(ks)
             r[2:2 + len(ms)] = ms
             r[5:5 + len(ks)] = ks
+        elif self.kind == MAT_NONLIN_TAB:
+            # [1] first node in the solve's table (set by pack_materials), [2] nodes
+            r[1] = -1.0
+            r[2] = len(self.p["x"])
         else:
-            raise NotImplementedError("tabulated M(H) materials are not supported yet")
+            raise ValueError("unknown material kind %r" % self.kind)
         return r
+
+    def table(self):
+        """Flat (|H|, |M|, d|M|/d|H|) node triples of a tabulated law."""
+        return np.stack([self.p["x"], self.p["y"], self.p["d"]], 1).ravel()
+
+
+def _node_derivative(x, y, k):
+    """Derivative at node k of the polynomial through the points (x, y), set to
+    0 when it is negative on increasing data (radmater.cpp:341-455)."""
+    n = len(x)
+    der = 0.0
+    for j in range(n):
+        if j == k:
+            der += y[j] * sum(1.0 / (x[k] - x[m]) for m in range(n) if m != k)
+        else:
+            w = 1.0 / (x[j] - x[k])
+            for m in range(n):
+                if m != j and m != k:
+                    w *= (x[k] - x[m]) / (x[j] - x[m])
+            der += y[j] * w
+    inc = all(x[i + 1] > x[i] for i in range(n - 1)) and all(y[i + 1] >= y[i] for i in range(n - 1))
+    return 0.0 if (inc and der < 0.0) else der
+
+
+def tab_nodes(points):
+    """Node table of a tabulated isotropic law M(H) from (H, M) points, as the
+    reference builds it (radmater.h:436-451, radmater.cpp:460-609): (0, 0) is
+    prepended unless the first point is at the origin; node derivatives come
+    from 3- and 5-point interpolants in the reference's pattern (including its
+    index use for 4 to 6 nodes) and are corrected so that each cubic Hermite
+    segment over monotone data stays monotone.  Returns x, y, d."""
+    pts = [(float(h), float(m)) for h, m in points]
+    if pts[0][0] > 1e-10 and pts[0][1] > 1e-10:
+        pts = [(0.0, 0.0)] + pts
+    x = [p[0] for p in pts]
+    y = [p[1] for p in pts]
+    L = len(x)
+    if L < 4:
+        raise ValueError("a tabulated law needs at least 4 nodes (3 points above the origin)")
+    d = [0.0] * L
+    D3 = lambda t, k: _node_derivative(x[t:t + 3], y[t:t + 3], k)
+    D5 = lambda t, k: _node_derivative(x[t:t + 5], y[t:t + 5], k)
+    slope = lambda t: (y[t + 1] - y[t]) / (x[t + 1] - x[t])
+    d[0], d[1], d[2] = D3(0, 0), D3(0, 1), D3(1, 1)
+    w, t = 3, 1
+    for _i in range(3, L - 4):
+        d[w] = D5(t, 2)
+        w += 1
+        t += 1
+    if L >= 7:
+        d[w] = D5(t, 2)
+        t += 2
+        d[w + 1] = D3(t, 1)
+        t += 1
+        d[w + 2] = D3(t, 1)
+        t += 1
+        d[w + 3] = slope(t)
+    elif L >= 6:
+        d[w] = D5(t, 2)
+        t += 2
+        d[w + 1] = D3(t, 1)
+        t += 1
+        d[w + 2] = slope(t)
+    elif L >= 5:
+        d[w] = D3(t, 1)
+        t += 1
+        d[w + 1] = slope(t)
+    else:
+        d[w] = slope(t)
+    # monotone correction, last segment first
+    for i in range(L - 2, -1, -1):
+        h = x[i + 1] - x[i]
+        fm = y[i] - y[i + 1]
+        p1, p2 = d[i], d[i + 1]
+        c1 = p1
+        c2 = -(3 * fm + (2 * p1 + p2) * h) / (h * h)
+        c3 = (2 * fm + (p1 + p2) * h) / (h * h * h)
+        disc = c2 * c2 - 3.0 * c1 * c3
+        roots = []
+        if c3 != 0.0 and disc >= 0.0:
+            r = np.sqrt(disc)
+            roots = [(-c2 - r) / (3.0 * c3), (-c2 + r) / (3.0 * c3)]
+        mono = (y[i] < y[i + 1] and p1 > 0 and p2 > 0) or (y[i] > y[i + 1] and p1 < 0 and p2 < 0)
+        inside = any(0.0 < z < h for z in roots)
+        if mono and disc > 0.0 and inside:
+            q = -3 * p2 * h ** 3 * (4 * fm + p2 * h)
+            if q >= 0.0:
+                sq = np.sqrt(q)
+                base = -h * (6 * fm + p2 * h)
+                fa = (base - sq) / (2 * h * h)
+                fb = (base + sq) / (2 * h * h)
+                d[i] = fa if abs(p1 - fa) < abs(p1 - fb) else fb
+    return np.array(x), np.array(y), np.array(d)
+
+
+def tab_eval(x, y, d, h):
+    """|M| at |H| = h of a node table: cubic Hermite between nodes, y[0] below
+    the first node, the last node's slope beyond the last (radmater.cpp:619-646)."""
+    L = len(x)
+    i = int(np.searchsorted(x, h, side="right")) - 1
+    if i < 0:
+        return float(y[0])
+    if i >= L - 1:
+        return float(y[-1] + (h - x[-1]) * d[-1])
+    s = x[i + 1] - x[i]
+    u = h - x[i]
+    sl = (y[i + 1] - y[i]) / s
+    c2 = (3 * sl - 2 * d[i] - d[i + 1]) / s
+    c3 = (-2 * sl + d[i] + d[i + 1]) / (s * s)
+    return float(y[i] + u * (d[i] + u * (c2 + u * c3)))
+
+
+def pack_materials(items):
+    """Material records [n, 16] and the node table their tabulated laws index
+    (flat (x, y, d) triples; record [1] = first node)."""
+    recs = []
+    tables = []
+    offsets = {}
+    n_nodes = 0
+    for (o, _k, _s) in items:
+        r = o.mat.record(o.M)
+        if o.mat.kind == MAT_NONLIN_TAB:
+            key = id(o.mat)
+            if key not in offsets:
+                offsets[key] = n_nodes
+                t = o.mat.table()
+                tables.append(t)
+                n_nodes += len(t) // 3
+            r[1] = offsets[key]
+        recs.append(r)
+    tab = np.concatenate(tables) if tables else np.zeros(0)
+    return np.stack(recs) if recs else np.zeros((0, 16)), tab
 
 
 # ---------------------------------------------------------------- flattening

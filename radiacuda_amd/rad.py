@@ -316,6 +316,16 @@ def MatSatIsoFrm(*pairs):
     return _add(O.Material(O.MAT_NONLIN_FRM, ks=ks, ms=ms))
 
 
+def MatSatIsoTab(points):
+    """MatSatIsoTab([[H1, M1], [H2, M2], ...]): isotropic law |M|(|H|) from a
+    table, H and M in tesla (radentry.cpp:817-846, radmater.h:436-451)."""
+    pts = [(float(p[0]), float(p[1])) for p in points]
+    if pts[0][0] < 0 or pts[0][1] < 0 or any(b[0] < a[0] for a, b in zip(pts, pts[1:])):
+        raise RuntimeError("Radia::Error071: H values must be non-negative and increasing")
+    x, y, d = O.tab_nodes(pts)
+    return _add(O.Material(O.MAT_NONLIN_TAB, x=x, y=y, d=d))
+
+
 def MatApl(obj, mat):
     m = _obj(mat)
     if not isinstance(m, O.Material):
@@ -334,8 +344,8 @@ def MatMvsH(obj, comp, H):
             break
     if o is None or o.mat is None:
         raise RuntimeError("object has no material")
-    rec = o.mat.record(o.M)
-    M = N.mat_M(rec, _vec3(H))
+    rec, tab = O.pack_materials([(o, None, None)])
+    M = N.mat_M(rec[0], _vec3(H), tab)
     return _component(M, comp or "m")
 
 
@@ -389,7 +399,7 @@ def RlxPre(obj, srcobj=0, use_gpu=True):
         c = co[e]
         hext[e] = it.packed["cS"][c] * np.linalg.solve(it.packed["cM"][c], Hlab[e])
     it.hext = hext
-    it.mats = np.stack([o.mat.record(o.M) for (o, _k, _s) in rel])
+    it.mats, it.tab = O.pack_materials(rel)
     it.H = np.zeros_like(hext)
     return _add(it)
 
@@ -448,7 +458,7 @@ def RlxAuto(intrc, prec, maxiter, meth=4, *opts):
     refine = int(_opt(opts, "Refine", "0"))
     M0 = np.stack([o.M for (o, _k, _s) in it.items])
     res, M, H = it.matrix.relax(it.mats, it.hext, M0, prec, maxiter, keep_m=not zero_m,
-                                omega0=omega, refine=refine)
+                                omega0=omega, refine=refine, tab=it.tab)
     for e, (o, _k, _s) in enumerate(it.items):
         o.M = M[e].copy()
     it.H = H

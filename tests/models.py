@@ -112,9 +112,28 @@ def oracle_field(P, M, pts):
     return B, H, bad
 
 
-def mat_M_and_D(rec, H):
+def _tab_f_and_df(t, h):
+    """|M|(h) and d|M|/dh of a node table t [L, 3] (cubic Hermite between
+    nodes, constant below the first, last slope beyond the last)."""
+    x, y, d = t[:, 0], t[:, 1], t[:, 2]
+    L = len(x)
+    i = int(np.searchsorted(x, h, side="right")) - 1
+    if i < 0:
+        return float(y[0]), 0.0
+    if i >= L - 1:
+        return float(y[-1] + (h - x[-1]) * d[-1]), float(d[-1])
+    s = x[i + 1] - x[i]
+    u = h - x[i]
+    sl = (y[i + 1] - y[i]) / s
+    c2 = (3 * sl - 2 * d[i] - d[i + 1]) / s
+    c3 = (-2 * sl + d[i] + d[i + 1]) / (s * s)
+    return float(y[i] + u * (d[i] + u * (c2 + u * c3))), float(d[i] + u * (2 * c2 + 3 * u * c3))
+
+
+def mat_M_and_D(rec, H, tab=None):
     """M(H) and dM/dH of a material record (kinds 0 linear iso, 1 linear
-    aniso, 2 tanh-sum formula), FP64, for the test-side reference solver."""
+    aniso, 2 tanh-sum formula, 3 tabulated law in tab), FP64, for the
+    test-side reference solver."""
     kind = int(rec[0])
     mr = rec[13:16]
     if kind == 0:
@@ -123,6 +142,14 @@ def mat_M_and_D(rec, H):
         K = rec[4:13].reshape(3, 3)
         return K @ H + mr, K
     h = float(np.linalg.norm(H))
+    if kind == 3:
+        t = np.asarray(tab, dtype=np.float64)[3 * int(rec[1]):3 * int(rec[1] + rec[2])].reshape(-1, 3)
+        if h == 0:
+            return mr.copy(), t[0, 2] * np.eye(3)
+        f, fp = _tab_f_and_df(t, h)
+        u = H / h
+        chi = f / h
+        return chi * H + mr, chi * np.eye(3) + (fp - chi) * np.outer(u, u)
     ms, ks = rec[2:2 + int(rec[1])], rec[5:5 + int(rec[1])]
     f = sum(m * np.tanh(k * h / m) for m, k in zip(ms, ks) if m != 0)
     def sech2(x):
@@ -136,7 +163,7 @@ def mat_M_and_D(rec, H):
     return chi * H + mr, chi * np.eye(3) + (fp - chi) * np.outer(u, u)
 
 
-def fp64_fixed_point(Q, mats, hext, M0=None, tol=1e-14, max_iter=100):
+def fp64_fixed_point(Q, mats, hext, M0=None, tol=1e-14, max_iter=100, tab=None):
     """Newton polish of M = M_mat(Q M + hext) in FP64 with the exact Jacobian
     (dense, small n), started from M0 (e.g. the oracle's Gauss-Seidel result):
     the reference the device solver is checked against on the very matrix the
@@ -151,7 +178,7 @@ def fp64_fixed_point(Q, mats, hext, M0=None, tol=1e-14, max_iter=100):
         F = np.zeros(3 * n)
         D = np.zeros((3 * n, 3 * n))
         for e in range(n):
-            m, d = mat_M_and_D(mats[e], H[e])
+            m, d = mat_M_and_D(mats[e], H[e], tab)
             F[3 * e:3 * e + 3] = M[3 * e:3 * e + 3] - m
             D[3 * e:3 * e + 3, 3 * e:3 * e + 3] = d
         if np.abs(F).max() <= tol:
@@ -159,3 +186,57 @@ def fp64_fixed_point(Q, mats, hext, M0=None, tol=1e-14, max_iter=100):
         J = np.eye(3 * n) - D @ Q
         M = M - np.linalg.solve(J, F)
     return M.reshape(n, 3)
+
+
+# Steel M(H) curve of the reference's hybrid undulator example
+# (examples/RADIA_Example03.py:59-69): H in A/m, M in tesla.
+STEEL_H_AM = [0.8, 1.5, 2.2, 3.6, 5, 6.8, 9.8, 18, 28, 37.5, 42, 55, 71.5, 80, 85, 88, 92, 100, 120,
+              150, 200, 300, 400, 600, 800, 1000, 2000, 4000, 6000, 10000, 25000, 40000]
+STEEL_M_T = [0.000998995, 0.00199812, 0.00299724, 0.00499548, 0.00699372, 0.00999145, 0.0149877,
+             0.0299774, 0.0499648, 0.0799529, 0.0999472, 0.199931, 0.49991, 0.799899, 0.999893,
+             1.09989, 1.19988, 1.29987, 1.41985, 1.49981, 1.59975, 1.72962, 1.7995, 1.89925,
+             1.96899, 1.99874, 2.09749, 2.19497, 2.24246, 2.27743, 2.28958, 2.28973]
+MU0 = 4e-7 * np.pi
+
+
+def steel_points():
+    """(|H| in tesla, |M| in tesla) pairs of the example's steel."""
+    return [[h * MU0, m] for h, m in zip(STEEL_H_AM, STEEL_M_T)]
+
+
+def hybrid_undulator(n_periods=2, period=46.0, gap=20.0, offset=1.0, pole=(45.0, 5.0, 25.0),
+                     magnet_xz=(65.0, 45.0), pole_div=(2, 2, 5), magnet_div=(1, 3, 1)):
+    """Hybrid undulator of the reference's third example: steel poles with the
+    tabulated curve above, NdFeB blocks (MatLin([0.06, 0.17], 1.2)) magnetized
+    along -+y in turn; the quarter x > 0, z < 0 is built and three symmetry
+    planes complete it.  Pieces along y from the mid-plane: a half pole, then
+    n_periods of (magnet, pole), then a half magnet.  Returns (top, steel)."""
+    steel = rad.MatSatIsoTab(steel_points())
+    ndfeb = rad.MatLin([0.06, 0.17], 1.2)
+    t_mag = 0.5 * period - pole[1]
+    seq = [("pole", 0.5 * pole[1], 0)]
+    sign = -1
+    for _ in range(n_periods):
+        seq += [("magnet", t_mag, sign), ("pole", pole[1], 0)]
+        sign = -sign
+    seq.append(("magnet", 0.5 * t_mag, sign))
+    top = rad.ObjCnt([])
+    y = 0.0
+    for kind, t, s in seq:
+        yc = y + 0.5 * t
+        y += t
+        if kind == "pole":
+            b = rad.ObjRecMag([0.25 * pole[0], yc, -0.5 * (pole[2] + gap)], [0.5 * pole[0], t, pole[2]],
+                              [0, 0, 0])
+            rad.ObjDivMag(b, list(pole_div))
+            rad.MatApl(b, steel)
+        else:
+            b = rad.ObjRecMag([0.25 * magnet_xz[0], yc, -0.5 * (magnet_xz[1] + gap) - offset],
+                              [0.5 * magnet_xz[0], t, magnet_xz[1]], [0, s, 0])
+            rad.ObjDivMag(b, list(magnet_div))
+            rad.MatApl(b, ndfeb)
+        rad.ObjAddToCnt(top, [b])
+    rad.TrfZerPerp(top, [0, 0, 0], [1, 0, 0])
+    rad.TrfZerPara(top, [0, 0, 0], [0, 0, 1])
+    rad.TrfZerPerp(top, [0, 0, 0], [0, 1, 0])
+    return top, steel
