@@ -153,22 +153,30 @@ typedef struct {
     int32_t max_iter;  /* iteration limit                                      */
     int32_t keep_m;    /* 1: start from M_inout ('ZeroM->False'), 0: from M_r  */
     double omega0;     /* mixing factor of the iteration (0 -> 0.5)            */
-    int32_t method;    /* 9 (or 0): Anderson-accelerated block Jacobi          */
+    int32_t method;    /* 9 (or 0): Anderson-accelerated block Jacobi;         *
+                        * 11: Newton-Krylov (GMRES on the analytic Jacobian,    *
+                        * radGPU_RelaxNK, cpp/src/core/radgpurlx.cu:1351-2285)  */
     int32_t refine;    /* RH_MIXED matrix only: FP64 refinement passes after   *
                         * convergence (0 -> 1, -1 -> none); each pass takes the *
                         * FP64 field at the collocation points, corrects the   *
                         * external field by its defect against the stored       *
                         * matrix and resumes the iteration                     */
+    int32_t krylov_m;  /* method 11: GMRES restart length (0 -> 60, max 63)    */
+    int32_t reserved;  /* 0                                                    */
 } rh_relax_opts;
 
 typedef struct {
-    double misfit;     /* RMS |M_proposal - M| over elements (T)   */
-    double max_m;      /* max |M| (T)                               */
-    double max_h;      /* max |H| (T)                               */
-    int32_t iters;     /* iterations performed                      */
-    int32_t matvecs;   /* matrix passes (each streams the matrix)   */
-    int32_t converged; /* 1 if misfit <= prec                       */
-    int32_t refined;   /* FP64 refinement passes performed          */
+    double misfit;     /* RMS over elements of |M_proposal - M| (method 9)     *
+                        * or of |M_mat(H) - M| (method 11), in T               */
+    double max_m;      /* max |M| (T)                                          */
+    double max_h;      /* max |H| (T)                                          */
+    int32_t iters;     /* iterations (method 11: matrix passes, as the          *
+                        * reference counts them)                               */
+    int32_t matvecs;   /* matrix passes (each streams the matrix)              */
+    int32_t converged; /* 1 if misfit <= prec                                  */
+    int32_t refined;   /* FP64 refinement passes performed                     */
+    int32_t newton;    /* method 11: Newton steps                              */
+    int32_t krylov;    /* method 11: GMRES iterations                          */
 } rh_relax_result;
 
 /* Relaxes the magnetization to M = M_mat(H_ext + Q*M).  mats: [16n] host

@@ -23,7 +23,8 @@ class ModelDesc(ctypes.Structure):
 class RelaxOpts(ctypes.Structure):
     _fields_ = [("prec", ctypes.c_double), ("max_iter", ctypes.c_int32),
                 ("keep_m", ctypes.c_int32), ("omega0", ctypes.c_double),
-                ("method", ctypes.c_int32), ("refine", ctypes.c_int32)]
+                ("method", ctypes.c_int32), ("refine", ctypes.c_int32),
+                ("krylov_m", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class MatrixOpts(ctypes.Structure):
@@ -35,7 +36,8 @@ class RelaxResult(ctypes.Structure):
     _fields_ = [("misfit", ctypes.c_double), ("max_m", ctypes.c_double),
                 ("max_h", ctypes.c_double), ("iters", ctypes.c_int32),
                 ("matvecs", ctypes.c_int32), ("converged", ctypes.c_int32),
-                ("refined", ctypes.c_int32)]
+                ("refined", ctypes.c_int32), ("newton", ctypes.c_int32),
+                ("krylov", ctypes.c_int32)]
 
 
 def _c(a, dtype):

@@ -259,17 +259,20 @@ class Matrix:
         check(lib().rh_matrix_export(self.h, row0, nrows, _p(out)))
         return out
 
-    def relax(self, mats, hext, M0, prec, max_iter, keep_m=False, omega0=0.0, refine=0, tab=None):
+    def relax(self, mats, hext, M0, prec, max_iter, keep_m=False, omega0=0.0, refine=0, tab=None,
+              method=9, krylov_m=0):
         """refine: FP64 refinement passes of a mixed-precision matrix
         (0 = library default of one pass, -1 = none); tab: node table of the
-        tabulated laws that kind-3 records index (objects.pack_materials)."""
+        tabulated laws that kind-3 records index (objects.pack_materials);
+        method 9 (Anderson block Jacobi) or 11 (Newton-Krylov, GMRES restart
+        krylov_m, 0 = 60)."""
         mats = _c(mats).reshape(-1)
         tab = _c(np.zeros(0) if tab is None else tab).reshape(-1)
         hext = _c(hext).reshape(-1)
         M = _c(M0).reshape(-1).copy()
         H = np.zeros_like(M)
-        opts = _abi.RelaxOpts(float(prec), int(max_iter), 1 if keep_m else 0, float(omega0), 9,
-                              int(refine))
+        opts = _abi.RelaxOpts(float(prec), int(max_iter), 1 if keep_m else 0, float(omega0), int(method),
+                              int(refine), int(krylov_m), 0)
         res = _abi.RelaxResult()
         check(lib().rh_relax_tab(self.h, _p(mats), _p(tab) if len(tab) else None, len(tab), _p(hext),
                                  _p(M), _p(H), ctypes.byref(opts), ctypes.byref(res)))

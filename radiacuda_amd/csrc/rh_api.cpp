@@ -844,6 +844,268 @@ rh_status relax_run(rh_matrix* q, const double* d_mats, const double* d_tab, con
 }
 }  // namespace
 
+namespace {
+// ---------------------------------------------------------------- Newton-Krylov
+// Method 11 (the root finding of radGPU_RelaxNK, cpp/src/core/radgpurlx.cu:1351-2285,
+// with a driver of our own): Newton on F(M) = M_mat(H_ext + Q M) - M with the
+// analytic Jacobian J = D Q - I; each step solves (I - D Q) delta = F by
+// restarted GMRES right-preconditioned with the element blocks
+// (I - D_i Q_ii)^{-1} (classical Gram-Schmidt with one re-orthogonalisation,
+// Givens rotations on the host), forcing terms after Eisenstat and Walker
+// (choice 2), and a backtracking line search on |F|.  A step whose line search
+// fails falls back to Anderson iterations.
+constexpr int kKrylovDefault = 60;
+constexpr int kKrylovPerStep = 400;
+
+struct NKWork {
+    double* Mc = nullptr;  // [ld] current magnetization (matrix-pass input)
+    double* Mt = nullptr;  // [ld] trial magnetization
+    double *F = nullptr, *Ft = nullptr;  // [n3]
+    double *D = nullptr, *Dt = nullptr;  // [9n]
+    double *P = nullptr, *Pt = nullptr;  // [9n]
+    double *H = nullptr, *Ht = nullptr;  // [n3]
+    double* y = nullptr;   // [n3] matrix-pass output
+    double* z = nullptr;   // [ld] preconditioned vector (matrix-pass input)
+    double* w = nullptr;   // [n3]
+    double* u = nullptr;   // [n3] GMRES solution in the preconditioned space
+    double* V = nullptr;   // [(m + 1) * ldv] Krylov basis
+    double* part = nullptr;  // [3 nb]
+    double* dots = nullptr;  // [(m + 2) * db]
+    double* hpin = nullptr;  // pinned [max(3 nb, (m + 2) db)]
+    int64_t ldv = 0;
+    int m = 0;
+    ~NKWork()
+    {
+        for (double* p : {Mc, Mt, F, Ft, D, Dt, P, Pt, H, Ht, y, z, w, u, V, part, dots}) dfree(p);
+        if (hpin) (void)hipHostFree(hpin);
+    }
+    hipError_t alloc(int64_t n, int64_t ld, int krylov_m, int64_t nb, int64_t db, hipStream_t s)
+    {
+        const int64_t n3 = 3 * n;
+        m = krylov_m;
+        ldv = (n3 + 63) / 64 * 64;
+        hipError_t e = dalloc(&Mc, size_t(ld));
+        if (e == hipSuccess) e = dalloc(&Mt, size_t(ld));
+        if (e == hipSuccess) e = dalloc(&z, size_t(ld));
+        for (double** p : {&F, &Ft, &H, &Ht, &y, &w, &u})
+            if (e == hipSuccess) e = dalloc(p, size_t(n3));
+        for (double** p : {&D, &Dt, &P, &Pt})
+            if (e == hipSuccess) e = dalloc(p, size_t(9 * n));
+        if (e == hipSuccess) e = dalloc(&V, size_t((m + 1) * ldv));
+        if (e == hipSuccess) e = dalloc(&part, size_t(3 * nb));
+        if (e == hipSuccess) e = dalloc(&dots, size_t((m + 2) * db));
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&hpin),
+                              size_t(std::max<int64_t>(3 * nb, (m + 2) * db)) * sizeof(double));
+        // padded tails of the matrix-pass inputs read as zero
+        for (double* p : {Mc, Mt, z})
+            if (e == hipSuccess) e = hipMemsetAsync(p, 0, size_t(ld) * sizeof(double), s);
+        return e;
+    }
+};
+
+rh_status nk_run(rh_matrix* q, const double* d_mats, const double* d_tab, const double* d_hext, double* d_M,
+                 double* d_H, const rh_relax_opts* opts, NKWork& k, RelaxWork& rw, double* hbuf, int64_t nb,
+                 int64_t db, rh_relax_result* res)
+{
+    rh_model* m = q->m;
+    hipStream_t s = m->stream;
+    const int64_t n = m->host.n, n3 = q->n3;
+    const double prec = opts->prec;
+    const int max_mv = std::max(1, int(opts->max_iter));
+    const double prec2 = prec * std::sqrt(double(n));  // misfit prec as a 2-norm of F
+    Timer tm("matvec", s);
+    int mv = 0, newton = 0, krylov = 0;
+    double maxM = 0.0, maxH = 0.0;
+    // F, D, P, H at magnetization M; returns |F|^2
+    auto evaluate = [&](double* M, double* F, double* D, double* P, double* H, double& f2) -> rh_status {
+        tm.begin();
+        RH_HIP(matrix_pass(q, M, k.y, s));
+        tm.end();
+        RH_HIP(rh::launch_nk_material(n, d_mats, d_tab, d_hext, q->d_qdiag, k.y, M, F, D, P, H, k.part, s));
+        RH_HIP(hipMemcpyAsync(k.hpin, k.part, size_t(3 * nb) * sizeof(double), hipMemcpyDeviceToHost, s));
+        RH_HIP(hipStreamSynchronize(s));
+        mv++;
+        f2 = 0.0;
+        maxM = 1e-17;
+        maxH = 1e-17;
+        for (int64_t b = 0; b < nb; b++) {
+            f2 += k.hpin[3 * b];
+            maxM = std::max(maxM, k.hpin[3 * b + 1]);
+            maxH = std::max(maxH, k.hpin[3 * b + 2]);
+        }
+        if (!std::isfinite(f2)) return fail(RH_ERR_NOTCONV, "Newton-Krylov produced a non-finite residual");
+        return RH_OK;
+    };
+    // dot products of w with V_0..V_{nv-1} and with itself
+    auto mdot = [&](int nv, const double* w, std::vector<double>& out) -> rh_status {
+        RH_HIP(rh::launch_mdot(nv, k.V, k.ldv, w, n3, k.dots, s));
+        RH_HIP(hipMemcpyAsync(k.hpin, k.dots, size_t((nv + 1) * db) * sizeof(double), hipMemcpyDeviceToHost, s));
+        RH_HIP(hipStreamSynchronize(s));
+        out.assign(size_t(nv + 1), 0.0);
+        for (int j = 0; j <= nv; j++)
+            for (int64_t b = 0; b < db; b++) out[size_t(j)] += k.hpin[j * db + b];
+        return RH_OK;
+    };
+    // w = (I - D Q) P v
+    auto apply = [&](const double* v, double* w) -> rh_status {
+        RH_HIP(rh::launch_nk_prec(n, k.P, v, k.z, s));
+        tm.begin();
+        RH_HIP(matrix_pass(q, k.z, k.y, s));
+        tm.end();
+        RH_HIP(rh::launch_nk_apply(n, k.D, q->d_qdiag, k.z, k.y, w, s));
+        mv++;
+        return RH_OK;
+    };
+    rh_status st;
+    RH_HIP(hipMemcpyAsync(k.Mc, rw.x, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    double f2 = 0.0;
+    if ((st = evaluate(k.Mc, k.F, k.D, k.P, k.H, f2)) != RH_OK) return st;
+    double fn = std::sqrt(f2), fn_prev = 0.0, eta = 0.5;
+    const int mk = k.m;
+    const size_t um = static_cast<size_t>(mk);
+    std::vector<double> Hs((um + 1) * um), cs(um), sn(um), g(um + 1), yv(um), h1, h2, coef(um + 1);
+    while (fn > prec2 && mv < max_mv) {
+        newton++;
+        // forcing term (Eisenstat-Walker choice 2, gamma 0.9, alpha 2, with safeguards)
+        if (newton > 1) {
+            const double r = fn / fn_prev;
+            double e2 = 0.9 * r * r;
+            const double sg = 0.9 * eta * eta;
+            if (sg > 0.1) e2 = std::max(e2, sg);
+            eta = e2;
+        }
+        eta = std::min(eta, 0.5);
+        eta = std::max(eta, std::min(0.5, 0.5 * prec2 / fn));
+        const double tol = eta * fn;
+        // restarted GMRES on (I - D Q) P u = F, u0 = 0
+        RH_HIP(hipMemsetAsync(k.u, 0, size_t(n3) * sizeof(double), s));
+        const double* r0 = k.F;
+        double beta = fn, resid = fn;
+        int its = 0;
+        while (true) {
+            const double one[1] = {0.0};
+            RH_HIP(rh::launch_vcomb(0, k.V, k.ldv, one, r0, 1.0 / beta, n3, k.V, s));
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            int j = 0;
+            bool stop = false;
+            for (; j < mk; j++) {
+                if ((st = apply(k.V + j * k.ldv, k.w)) != RH_OK) return st;
+                if ((st = mdot(j + 1, k.w, h1)) != RH_OK) return st;
+                for (int i = 0; i <= j; i++) coef[size_t(i)] = -h1[size_t(i)];
+                RH_HIP(rh::launch_vcomb(j + 1, k.V, k.ldv, coef.data(), k.w, 1.0, n3, k.w, s));
+                if ((st = mdot(j + 1, k.w, h2)) != RH_OK) return st;
+                double hh2 = 0.0;
+                for (int i = 0; i <= j; i++) hh2 += h2[size_t(i)] * h2[size_t(i)];
+                const double bnext = std::sqrt(std::max(h2[size_t(j + 1)] - hh2, 0.0));
+                double* col = Hs.data() + size_t(j) * (mk + 1);
+                for (int i = 0; i <= j; i++) col[i] = h1[size_t(i)] + h2[size_t(i)];
+                const bool breakdown = !(bnext > 1e-14 * fn);
+                if (!breakdown && j + 1 <= mk) {
+                    for (int i = 0; i <= j; i++) coef[size_t(i)] = -h2[size_t(i)] / bnext;
+                    RH_HIP(rh::launch_vcomb(j + 1, k.V, k.ldv, coef.data(), k.w, 1.0 / bnext, n3,
+                                            k.V + (j + 1) * k.ldv, s));
+                }
+                // Givens rotations
+                for (int i = 0; i < j; i++) {
+                    const double a = col[i], b = col[i + 1];
+                    col[i] = cs[size_t(i)] * a + sn[size_t(i)] * b;
+                    col[i + 1] = -sn[size_t(i)] * a + cs[size_t(i)] * b;
+                }
+                const double t = std::hypot(col[j], bnext);
+                cs[size_t(j)] = (t > 0.0) ? col[j] / t : 1.0;
+                sn[size_t(j)] = (t > 0.0) ? bnext / t : 0.0;
+                col[j] = t;
+                g[size_t(j + 1)] = -sn[size_t(j)] * g[size_t(j)];
+                g[size_t(j)] = cs[size_t(j)] * g[size_t(j)];
+                resid = std::fabs(g[size_t(j + 1)]);
+                its++;
+                if (resid <= tol || breakdown || its >= kKrylovPerStep || mv >= max_mv) {
+                    stop = true;
+                    j++;
+                    break;
+                }
+            }
+            // y = R^{-1} g, u += V y
+            for (int i = j - 1; i >= 0; i--) {
+                double v = g[size_t(i)];
+                for (int c = i + 1; c < j; c++) v -= Hs[size_t(c) * (mk + 1) + i] * yv[size_t(c)];
+                const double rii = Hs[size_t(i) * (mk + 1) + i];
+                yv[size_t(i)] = (rii != 0.0) ? v / rii : 0.0;
+            }
+            RH_HIP(rh::launch_vcomb(j, k.V, k.ldv, yv.data(), k.u, 1.0, n3, k.u, s));
+            if (stop) break;
+            // restart from the true residual F - (I - D Q) P u
+            if ((st = apply(k.u, k.w)) != RH_OK) return st;
+            const double* vv[2] = {k.F, k.w};
+            const double cc[2] = {1.0, -1.0};
+            RH_HIP(rh::launch_lincomb(2, vv, cc, n3, k.w, s));
+            std::vector<double> rr;
+            if ((st = mdot(0, k.w, rr)) != RH_OK) return st;
+            beta = std::sqrt(rr[0]);
+            resid = beta;
+            if (beta <= tol || mv >= max_mv) break;
+            r0 = k.w;
+        }
+        krylov += its;
+        // delta = P u; backtracking line search on |F|
+        RH_HIP(rh::launch_nk_prec(n, k.P, k.u, k.w, s));
+        double alpha = 1.0, ft2 = 0.0;
+        bool accepted = false;
+        for (int ls = 0; ls < 12 && mv < max_mv; ls++) {
+            const double* vv[2] = {k.Mc, k.w};
+            const double cc[2] = {1.0, alpha};
+            RH_HIP(rh::launch_lincomb(2, vv, cc, n3, k.Mt, s));
+            if ((st = evaluate(k.Mt, k.Ft, k.Dt, k.Pt, k.Ht, ft2)) != RH_OK) return st;
+            if (std::sqrt(ft2) <= (1.0 - 1e-4 * alpha * (1.0 - eta)) * fn) {
+                accepted = true;
+                break;
+            }
+            alpha *= 0.5;
+        }
+        fn_prev = fn;
+        if (accepted) {
+            std::swap(k.Mc, k.Mt);
+            std::swap(k.F, k.Ft);
+            std::swap(k.D, k.Dt);
+            std::swap(k.P, k.Pt);
+            std::swap(k.H, k.Ht);
+            fn = std::sqrt(ft2);
+            continue;
+        }
+        if (mv >= max_mv) break;
+        // no descent along the Newton direction: Anderson iterations from the
+        // current magnetization, then Newton again
+        rh_relax_opts o2 = *opts;
+        o2.max_iter = std::min(50, max_mv - mv);
+        rh_relax_result r2{};
+        RH_HIP(hipMemcpyAsync(rw.x, k.Mc, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if ((st = relax_run(q, d_mats, d_tab, d_hext, d_M, d_H, &o2, rw, hbuf, nb, db, &r2)) != RH_OK) return st;
+        mv += r2.matvecs;
+        RH_HIP(hipMemcpyAsync(k.Mc, d_M, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if ((st = evaluate(k.Mc, k.F, k.D, k.P, k.H, f2)) != RH_OK) return st;
+        fn = std::sqrt(f2);
+        eta = 0.5;
+    }
+    RH_HIP(hipMemcpyAsync(d_M, k.Mc, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    RH_HIP(hipMemcpyAsync(d_H, k.H, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // the next call's start (refinement passes resume from here)
+    RH_HIP(hipMemcpyAsync(rw.x, k.Mc, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    RH_HIP(hipStreamSynchronize(s));
+    const double misfit = fn / std::sqrt(double(n));
+    res->misfit = misfit;
+    res->max_m = maxM;
+    res->max_h = maxH;
+    res->iters += mv;
+    res->matvecs += mv;
+    res->newton += newton;
+    res->krylov += krylov;
+    res->converged = misfit <= prec ? 1 : 0;
+    return RH_OK;
+}
+}  // namespace
+
 rh_status rh_relax_dev(rh_matrix* q, const double* d_mats, const double* d_hext, double* d_M,
                        double* d_H, const rh_relax_opts* opts, rh_relax_result* res)
 {
@@ -856,7 +1118,10 @@ rh_status rh_relax_tab_dev(rh_matrix* q, const double* d_mats, const double* d_t
 {
     if (!q || !d_mats || !d_hext || !d_M || !d_H || !opts || !res) return fail(RH_ERR_ARG, "null argument");
     if (ntab < 0 || (ntab > 0 && !d_tab)) return fail(RH_ERR_ARG, "bad table");
-    if (opts->method != 9 && opts->method != 0) return fail(RH_ERR_UNSUPPORTED, "relaxation method must be 9");
+    if (opts->method != 9 && opts->method != 0 && opts->method != 11)
+        return fail(RH_ERR_UNSUPPORTED, "relaxation method must be 9 or 11");
+    if (opts->krylov_m < 0 || opts->krylov_m > rh::kCoefMax - 1)
+        return fail(RH_ERR_ARG, "krylov_m must be 0 to " + std::to_string(rh::kCoefMax - 1));
     rh_model* m = q->m;
     RH_HIP(enter_device(m->device));
     hipStream_t s = m->stream;
@@ -911,7 +1176,18 @@ rh_status rh_relax_tab_dev(rh_matrix* q, const double* d_mats, const double* d_t
         double* p;
         ~PinnedFree() { (void)hipHostFree(p); }
     } pinned_guard{hbuf};
-    rh_status st = relax_run(q, d_mats, d_tab, d_hext, d_M, d_H, opts, w, hbuf, nb, db, res);
+    const bool nk_method = (opts->method == 11);
+    NKWork nk;
+    if (nk_method) {
+        const int km = opts->krylov_m > 0 ? std::max(2, int(opts->krylov_m)) : kKrylovDefault;
+        hipError_t e = nk.alloc(n, q->ld, km, nb, db, s);
+        if (e != hipSuccess) return fail(RH_ERR_NOMEM, std::string("Newton-Krylov work: ") + hipGetErrorString(e));
+    }
+    auto run = [&](const double* hx, const rh_relax_opts* o, bool keep) -> rh_status {
+        if (nk_method) return nk_run(q, d_mats, d_tab, hx, d_M, d_H, o, nk, w, hbuf, nb, db, res);
+        return relax_run(q, d_mats, d_tab, hx, d_M, d_H, o, w, hbuf, nb, db, res, nullptr, keep);
+    };
+    rh_status st = run(d_hext, opts, false);
     if (st != RH_OK) return st;
     // FP64 refinement of a solve on the FP32-stored matrix: the defect between
     // the FP64 field at the collocation points and the stored-matrix field
@@ -942,7 +1218,7 @@ rh_status rh_relax_tab_dev(rh_matrix* q, const double* d_mats, const double* d_t
                                         q->d_qdiag, d_M, d_hext, d_hx, s));
         res->matvecs += 1;
         // a magnetization that already meets prec on the corrected system is kept
-        st = relax_run(q, d_mats, d_tab, d_hx, d_M, d_H, &o2, w, hbuf, nb, db, res, nullptr, true);
+        st = run(d_hx, &o2, true);
         if (st != RH_OK) return st;
         res->refined += 1;
     }

@@ -716,6 +716,204 @@ __global__ __launch_bounds__(256) void k_relax_update(int64_t n, const double* _
     }
 }
 
+// ---------------------------------------------------------------- Newton-Krylov
+// |M|(h) and d|M|/dh of an isotropic nonlinear law (kind 2 tanh sum, kind 3
+// table), h > 0.
+__device__ __forceinline__ void law_f_df(const double* mat, const double* tab, double h, double& f,
+                                         double& df)
+{
+    if (int(mat[0]) == 3) {
+        const double* t = tab + 3 * int64_t(mat[1]);
+        const int L = int(mat[2]);
+        if (h < t[0]) {
+            f = t[1];
+            df = 0.0;
+            return;
+        }
+        if (h >= t[3 * (L - 1)]) {
+            f = t[3 * (L - 1) + 1] + (h - t[3 * (L - 1)]) * t[3 * (L - 1) + 2];
+            df = t[3 * (L - 1) + 2];
+            return;
+        }
+        int lo = 0, hi = L - 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (t[3 * mid] <= h) lo = mid;
+            else hi = mid;
+        }
+        const double* a = t + 3 * lo;
+        const double sg = a[3] - a[0], u = h - a[0];
+        const double sl = (a[4] - a[1]) / sg;
+        const double c2 = (3.0 * sl - 2.0 * a[2] - a[5]) / sg;
+        const double c3 = (-2.0 * sl + a[2] + a[5]) / (sg * sg);
+        f = a[1] + u * (a[2] + u * (c2 + u * c3));
+        df = a[2] + u * (2.0 * c2 + 3.0 * c3 * u);
+        return;
+    }
+    const int nt = int(mat[1]);
+    f = 0.0;
+    df = 0.0;
+    for (int t = 0; t < nt; t++) {
+        const double ms = mat[2 + t], ks = mat[5 + t];
+        if (ms == 0.0) continue;
+        const double u = ks * h / ms;
+        f += ms * tanh(u);
+        const double c = cosh(u);
+        df += ks / (c * c);
+    }
+}
+
+// Material law at the total field of the current magnetization: H = y + Qii M
+// + hext, F = M_mat(H) - M, tangent D = dM_mat/dH (D = mp hh^T + sc (I - hh^T)
+// for isotropic laws, hh = H/|H|, sc = |M|/|H|, mp = d|M|/d|H|; chi0 I at
+// H = 0; chi I or K for linear laws) and the block preconditioner
+// P = (I - D Qii)^{-1}.  partials per block: [sum |F|^2, max |M_mat|, max |H|].
+__global__ __launch_bounds__(256) void k_nk_material(int64_t n, const double* __restrict__ mats,
+                                                     const double* __restrict__ tab,
+                                                     const double* __restrict__ hext,
+                                                     const double* __restrict__ qdiag,
+                                                     const double* __restrict__ y,
+                                                     const double* __restrict__ M,
+                                                     double* __restrict__ F, double* __restrict__ D,
+                                                     double* __restrict__ P, double* __restrict__ Hout,
+                                                     double* __restrict__ partials)
+{
+    __shared__ double red[3][256];
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    double f2 = 0.0, mm = 0.0, hh = 0.0;
+    if (i < n) {
+        const double* mat = mats + 16 * i;
+        const double* q = qdiag + 9 * i;
+        const double m[3] = {M[3 * i], M[3 * i + 1], M[3 * i + 2]};
+        double qm[3];
+        mat3_vec(q, m, qm);
+        double H[3];
+        for (int k = 0; k < 3; k++) H[k] = y[3 * i + k] + qm[k] + hext[3 * i + k];
+        const int kind = int(mat[0]);
+        double d[9], mn[3];
+        if (kind == 1) {
+            for (int k = 0; k < 9; k++) d[k] = mat[4 + k];
+            mat3_vec(d, H, mn);
+        } else {
+            double sc, mp;
+            const double h = sqrt(H[0] * H[0] + H[1] * H[1] + H[2] * H[2]);
+            if (kind == 0) {
+                sc = mp = mat[1];
+            } else if (h > 1e-300) {
+                double f, df;
+                law_f_df(mat, tab, h, f, df);
+                sc = f / h;
+                mp = df;
+            } else {
+                sc = mp = law_chi(mat, tab, 0.0);
+            }
+            const double e[3] = {h > 1e-300 ? H[0] / h : 0.0, h > 1e-300 ? H[1] / h : 0.0,
+                                 h > 1e-300 ? H[2] / h : 0.0};
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) d[3 * a + b] = (mp - sc) * e[a] * e[b] + (a == b ? sc : 0.0);
+            for (int k = 0; k < 3; k++) mn[k] = sc * H[k];
+        }
+        for (int k = 0; k < 3; k++) mn[k] += mat[13 + k];
+        // P = (I - D Qii)^{-1}
+        double a[9];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++)
+                a[3 * r + c] = (r == c ? 1.0 : 0.0) -
+                               (d[3 * r] * q[c] + d[3 * r + 1] * q[3 + c] + d[3 * r + 2] * q[6 + c]);
+        double pinv[9];
+        inv3d(a, pinv);
+        for (int k = 0; k < 9; k++) {
+            D[9 * i + k] = d[k];
+            P[9 * i + k] = pinv[k];
+        }
+        for (int k = 0; k < 3; k++) {
+            const double fk = mn[k] - m[k];
+            F[3 * i + k] = fk;
+            Hout[3 * i + k] = H[k];
+            f2 += fk * fk;
+        }
+        mm = sqrt(mn[0] * mn[0] + mn[1] * mn[1] + mn[2] * mn[2]);
+        hh = sqrt(H[0] * H[0] + H[1] * H[1] + H[2] * H[2]);
+    }
+    red[0][threadIdx.x] = f2;
+    red[1][threadIdx.x] = mm;
+    red[2][threadIdx.x] = hh;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + st];
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + st]);
+            red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x] = red[0][0];
+        partials[3 * blockIdx.x + 1] = red[1][0];
+        partials[3 * blockIdx.x + 2] = red[2][0];
+    }
+}
+
+// z = P v per element (3x3 blocks).
+__global__ __launch_bounds__(256) void k_nk_prec(int64_t n, const double* __restrict__ P,
+                                                 const double* __restrict__ v, double* __restrict__ z)
+{
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double x[3] = {v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+    double o[3];
+    mat3_vec(P + 9 * i, x, o);
+    for (int k = 0; k < 3; k++) z[3 * i + k] = o[k];
+}
+
+// w = z - D (y + Qii z): the Newton operator (I - D Q) applied to z, with
+// y = offdiag(Q) z from the matrix pass.
+__global__ __launch_bounds__(256) void k_nk_apply(int64_t n, const double* __restrict__ D,
+                                                  const double* __restrict__ qdiag,
+                                                  const double* __restrict__ z,
+                                                  const double* __restrict__ y, double* __restrict__ w)
+{
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double zz[3] = {z[3 * i], z[3 * i + 1], z[3 * i + 2]};
+    double qz[3];
+    mat3_vec(qdiag + 9 * i, zz, qz);
+    const double g[3] = {y[3 * i] + qz[0], y[3 * i + 1] + qz[1], y[3 * i + 2] + qz[2]};
+    double dg[3];
+    mat3_vec(D + 9 * i, g, dg);
+    for (int k = 0; k < 3; k++) w[3 * i + k] = zz[k] - dg[k];
+}
+
+// Dot products of w with nv basis vectors (V, leading dimension ldv) and with
+// itself: grid (chunks, nv + 1), partials[j * chunks + chunk].
+__global__ __launch_bounds__(256) void k_mdot(int nv, const double* __restrict__ V, int64_t ldv,
+                                              const double* __restrict__ w, int64_t len,
+                                              double* __restrict__ partials)
+{
+    __shared__ double red[4];
+    const int j = blockIdx.y;
+    const double* a = (j < nv) ? V + int64_t(j) * ldv : w;
+    double acc = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < len; i += int64_t(gridDim.x) * 256)
+        acc += a[i] * w[i];
+    const double v = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[int64_t(j) * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// out = cb * base + sum_j c_j V_j (base may be null; out may alias base).
+__global__ __launch_bounds__(256) void k_vcomb(int nv, const double* __restrict__ V, int64_t ldv,
+                                               Coef64 c, const double* base, double cb, int64_t len,
+                                               double* out)
+{
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < len; i += int64_t(gridDim.x) * 256) {
+        double acc = base ? cb * base[i] : 0.0;
+        for (int j = 0; j < nv; j++) acc += c.c[j] * V[int64_t(j) * ldv + i];
+        out[i] = acc;
+    }
+}
+
 // ---------------------------------------------------------------- vector helpers
 struct DotPack {
     const double* a[17];
@@ -948,6 +1146,48 @@ hipError_t launch_refine_defect(int64_t n, const double* H64, const double* rowt
 {
     hipLaunchKernelGGL(k_refine_defect, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, H64, rowt,
                        y, qdiag, M, hext, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_nk_material(int64_t n, const double* mats, const double* tab, const double* hext,
+                              const double* qdiag, const double* y, const double* M, double* F, double* D,
+                              double* P, double* Hout, double* partials, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_nk_material, dim3(unsigned(relax_blocks(n))), dim3(256), 0, s, n, mats, tab, hext,
+                       qdiag, y, M, F, D, P, Hout, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_nk_prec(int64_t n, const double* P, const double* v, double* z, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_nk_prec, dim3(unsigned(relax_blocks(n))), dim3(256), 0, s, n, P, v, z);
+    return hipGetLastError();
+}
+
+hipError_t launch_nk_apply(int64_t n, const double* D, const double* qdiag, const double* z, const double* y,
+                           double* w, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_nk_apply, dim3(unsigned(relax_blocks(n))), dim3(256), 0, s, n, D, qdiag, z, y, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_mdot(int nv, const double* V, int64_t ldv, const double* w, int64_t len, double* partials,
+                       hipStream_t s)
+{
+    if (nv < 0 || nv > kCoefMax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mdot, dim3(unsigned(dot_blocks(len)), unsigned(nv + 1)), dim3(256), 0, s, nv, V, ldv,
+                       w, len, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_vcomb(int nv, const double* V, int64_t ldv, const double* c, const double* base, double cb,
+                        int64_t len, double* out, hipStream_t s)
+{
+    if (nv < 0 || nv > kCoefMax) return hipErrorInvalidValue;
+    Coef64 cc{};
+    for (int j = 0; j < nv; j++) cc.c[j] = c[j];
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(4096, (len + 255) / 256));
+    hipLaunchKernelGGL(k_vcomb, dim3(unsigned(blocks)), dim3(256), 0, s, nv, V, ldv, cc, base, cb, len, out);
     return hipGetLastError();
 }
 

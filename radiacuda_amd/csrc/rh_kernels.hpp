@@ -82,6 +82,29 @@ hipError_t launch_refine_defect(int64_t n, const double* H64, const double* rowt
                                 const double* qdiag, const double* M, const double* hext,
                                 double* out, hipStream_t s);
 
+// ---------------------------------------------------------------- Newton-Krylov
+// Coefficients passed by value to the basis kernels (at most kCoefMax vectors).
+constexpr int kCoefMax = 64;
+struct Coef64 {
+    double c[kCoefMax];
+};
+// F = M_mat(H) - M, D = dM_mat/dH, P = (I - D Qii)^{-1}, Hout = H at
+// H = y + Qii M + hext; partials as launch_relax_update.
+hipError_t launch_nk_material(int64_t n, const double* mats, const double* tab, const double* hext,
+                              const double* qdiag, const double* y, const double* M, double* F, double* D,
+                              double* P, double* Hout, double* partials, hipStream_t s);
+// z = P v (3x3 blocks).
+hipError_t launch_nk_prec(int64_t n, const double* P, const double* v, double* z, hipStream_t s);
+// w = z - D (y + Qii z).
+hipError_t launch_nk_apply(int64_t n, const double* D, const double* qdiag, const double* z, const double* y,
+                           double* w, hipStream_t s);
+// partials[j * dot_blocks(len) + b]: w . V_j (j < nv) and w . w (j = nv).
+hipError_t launch_mdot(int nv, const double* V, int64_t ldv, const double* w, int64_t len, double* partials,
+                       hipStream_t s);
+// out = cb * base + sum_j c_j V_j (base may be null; out may alias base).
+hipError_t launch_vcomb(int nv, const double* V, int64_t ldv, const double* c, const double* base, double cb,
+                        int64_t len, double* out, hipStream_t s);
+
 // Batched dot products: out[k*nb + b] = partial of sum_i a_k[i]*b_k[i] over block b.
 int64_t dot_blocks(int64_t len);
 hipError_t launch_dots(int nd, const double* const* a, const double* const* b, int64_t len,

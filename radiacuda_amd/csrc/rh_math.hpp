@@ -79,9 +79,6 @@ RH_HD V3<R> cross(const V3<R>& a, const V3<R>& b)
 
 RH_HD double rh_log(double x) { return log(x); }
 RH_HD float rh_log(float x) { return logf(x); }
-RH_HD double rh_sqrt(double x) { return sqrt(x); }
-RH_HD double rh_rcp(double x) { return 1.0 / x; }
-RH_HD double rh_atan2(double y, double x) { return atan2(y, x); }
 RH_HD float rh_atan2(float y, float x) { return atan2f(y, x); }
 // FP32 square root and reciprocal: the hardware instructions (about 1 ulp) on
 // the device instead of the correctly rounded library sequences.
@@ -112,6 +109,138 @@ RH_HD float rh_log1p(float x)
     const float d = u - 1.0f;
     const float r = rh_log(u) * (x * rh_rcp(d));
     return (d == 0.0f) ? x : ((u < 3.0e38f) ? r : u);
+}
+
+// ---------------------------------------------------------------- FP64 elementary functions
+// The FP64 sweeps need these to about 1e-16 absolute on positive, finite,
+// normal arguments.  The library routines round correctly over the whole
+// domain and cost about 90 (log), 100 (atan2) and 15 (sqrt) vector
+// instructions each on gfx950; the forms below cost about a third:
+//  * reciprocal: the hardware seed refined by two Newton-Raphson steps;
+//  * square root: the hardware reciprocal-square-root seed, one Goldschmidt
+//    step and two Newton corrections;
+//  * ln(num/den): exponents split off, the mantissa ratio q brought into
+//    [1/sqrt2, sqrt2), ln q = 2 atanh(s) with s = (a - b)/(a + b) exact in
+//    the numerator, |s| <= 3 - 2 sqrt2, and a degree-6 polynomial in s^2;
+//  * atan2: the octant reduced to |t| <= tan(pi/8) (atan r = pi/4 +
+//    atan((r - 1)/(r + 1)) above it, one division either way) and a
+//    degree-10 polynomial in t^2.
+// The polynomials are Chebyshev interpolants of (2 atanh(s)/s - 2)/s^2 and
+// (atan(t)/t - 1)/t^2 on those ranges, fitted at 60 digits; evaluated in
+// double they are within 3e-17 absolute of ln q and atan t.  The host build
+// runs the same code with exactly rounded seeds.
+constexpr double kLogP[7] = {0.666666666666667,   0.39999999999899505, 0.28571428625975487,
+                             0.2222221113479508,  0.18182889125261723, 0.15331721600556042,
+                             0.14616449685043406};
+constexpr double kAtanP[11] = {-0.3333333333333333,  0.1999999999999552,   -0.14285714284666542,
+                               0.11111111015256361,  -0.09090904578123903, 0.07692183190826087,
+                               -0.06664511447381948, 0.0585814891280221,   -0.0508544973794026,
+                               0.03923165829558719,  -0.01917688711906226};
+#ifdef __HIP_DEVICE_COMPILE__
+// The same coefficients in constant memory: the kernels then hold them in
+// scalar registers and the Horner steps take them as the scalar operand of
+// v_fma_f64 (as immediates they are copied into vector registers per step).
+static __constant__ double d_log_p[7] = {0.666666666666667,   0.39999999999899505, 0.28571428625975487,
+                                         0.2222221113479508,  0.18182889125261723, 0.15331721600556042,
+                                         0.14616449685043406};
+static __constant__ double d_atan_p[11] = {
+    -0.3333333333333333,  0.1999999999999552,   -0.14285714284666542, 0.11111111015256361,
+    -0.09090904578123903, 0.07692183190826087,  -0.06664511447381948, 0.0585814891280221,
+    -0.0508544973794026,  0.03923165829558719,  -0.01917688711906226};
+#endif
+constexpr double kLn2Hi = 0.6931471803691238;  // ln 2 to 32 bits: k * kLn2Hi is exact
+constexpr double kLn2Lo = 1.9082149292705877e-10;
+constexpr double kSqrt2 = 1.4142135623730951;
+constexpr double kTanPi8 = 0.41421356237309503;
+constexpr double kPi4 = 0.7853981633974483, kPi2 = 1.5707963267948966, kPi = 3.141592653589793;
+
+RH_HD double rh_rcp(double d)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    double r = __builtin_amdgcn_rcp(d);
+#else
+    double r = 1.0 / d;
+#endif
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+
+RH_HD double rh_sqrt(double x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    const double y = __builtin_amdgcn_rsq(x);
+#else
+    const double y = 1.0 / sqrt(x);
+#endif
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
+RH_HD void rh_frexp(double x, double& m, int& e)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    m = __builtin_amdgcn_frexp_mant(x);
+    e = __builtin_amdgcn_frexp_exp(x);
+#else
+    m = frexp(x, &e);
+#endif
+}
+
+// ln(num/den), num and den positive, finite and normal.
+RH_HD double rh_log_ratio(double num, double den)
+{
+    double a, b;
+    int ea, eb;
+    rh_frexp(num, a, ea);
+    rh_frexp(den, b, eb);
+    const bool hi = a > kSqrt2 * b, lo = b > kSqrt2 * a;
+    b = hi ? 2.0 * b : b;
+    a = lo ? 2.0 * a : a;
+    const double k = double(ea - eb + (hi ? 1 : 0) - (lo ? 1 : 0));
+    const double s = (a - b) * rh_rcp(a + b);
+    const double v = s * s;
+#ifdef __HIP_DEVICE_COMPILE__
+    const double* c = d_log_p;
+#else
+    const double* c = kLogP;
+#endif
+    double p = c[6];
+#pragma unroll
+    for (int i = 5; i >= 0; i--) p = fma(p, v, c[i]);
+    const double r = fma(s * v, p, 2.0 * s);
+    return fma(k, kLn2Hi, fma(k, kLn2Lo, r));
+}
+RH_HD float rh_log_ratio(float num, float den) { return logf(num / den); }
+
+RH_HD double rh_atan2(double y, double x)
+{
+    const double ax = fabs(x), ay = fabs(y);
+    const double mx = fmax(ax, ay), mn = fmin(ax, ay);
+    const bool big = mn > kTanPi8 * mx;
+    const double t0 = (big ? mn - mx : mn) * rh_rcp(big ? mn + mx : mx);
+    const double t = (mx > 0.0) ? t0 : 0.0;
+    const double u = t * t;
+#ifdef __HIP_DEVICE_COMPILE__
+    const double* c = d_atan_p;
+#else
+    const double* c = kAtanP;
+#endif
+    double p = c[10];
+#pragma unroll
+    for (int i = 9; i >= 0; i--) p = fma(p, u, c[i]);
+    double a = fma(t * u, p, t);
+    a = big ? a + kPi4 : a;
+    a = (ay > ax) ? kPi2 - a : a;
+    a = (x < 0.0) ? kPi - a : a;
+    return copysign(a, y);
 }
 
 // Vertex relative to the observation point, formed in FP64 then rounded to R.
@@ -209,7 +338,7 @@ RH_HD bool eval_rec(const double* __restrict__ rec, const double p[3], double q6
                            v3(R(1), R(0), R(0)), num, den);
                 if (j == k) { nP *= num; dP *= den; } else { nP *= den; dP *= num; }
             }
-        lx = double(rh_log(nP / dP));
+        lx = double(rh_log_ratio(nP, dP));
     }
     {
         R nP = R(1), dP = R(1);
@@ -222,7 +351,7 @@ RH_HD bool eval_rec(const double* __restrict__ rec, const double p[3], double q6
                            v3(R(0), R(1), R(0)), num, den);
                 if (i == k) { nP *= num; dP *= den; } else { nP *= den; dP *= num; }
             }
-        ly = double(rh_log(nP / dP));
+        ly = double(rh_log_ratio(nP, dP));
     }
     {
         R nP = R(1), dP = R(1);
@@ -235,7 +364,7 @@ RH_HD bool eval_rec(const double* __restrict__ rec, const double p[3], double q6
                            v3(R(0), R(0), R(1)), num, den);
                 if (i == j) { nP *= num; dP *= den; } else { nP *= den; dP *= num; }
             }
-        lz = double(rh_log(nP / dP));
+        lz = double(rh_log_ratio(nP, dP));
     }
     // Diagonals: Q_aa = -(Omega(-a face) + Omega(+a face)); faces as two triangles.
     double om[3];
@@ -313,7 +442,7 @@ RH_HD bool eval_tet(const double* __restrict__ rec, const double p[3], double q6
         } else {
             R num, den;
             edge_ratio(r[a], Rv[a], r[b], Rv[b], t, num, den);
-            L = double(rh_log(num / den));
+            L = double(rh_log_ratio(num, den));
         }
 #pragma unroll
         for (int k = 0; k < 6; k++) acc[k] += L * E[4 + k];
@@ -415,7 +544,7 @@ RH_HD bool field_tet(const double* __restrict__ rec, const double p[3], double H
         } else {
             R num, den;
             edge_ratio(r[a], Rv[a], r[b], Rv[b], t, num, den);
-            L = double(rh_log(num / den));
+            L = double(rh_log_ratio(num, den));
         }
         h0 += L * E[4];
         h1 += L * E[5];
@@ -478,7 +607,7 @@ RH_HD bool eval_gen(const double* __restrict__ rec, const double p[3], double q6
         } else {
             R num, den;
             edge_ratio(ra, Ra, rb, Rb, t, num, den);
-            L = double(rh_log(num / den));
+            L = double(rh_log_ratio(num, den));
         }
         for (int k = 0; k < 6; k++) acc[k] += L * E[6 + k];
     }

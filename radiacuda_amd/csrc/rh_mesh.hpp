@@ -40,7 +40,7 @@ RH_HD double mesh_edge_L(const double* __restrict__ rec, const double p[3])
     if (sizeof(R) == 4) return double(edge_log1p(ra, Ra, rb, Rb, t, R(rec[9])));
     R num, den;
     edge_ratio(ra, Ra, rb, Rb, t, num, den);
-    return double(rh_log(num / den));
+    return double(rh_log_ratio(num, den));
 }
 
 // Signed solid angle of one charged face at p: the fan triangles (v0,v1,v2)

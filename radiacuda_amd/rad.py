@@ -446,10 +446,13 @@ def _opt(opts, name, default):
 
 def RlxAuto(intrc, prec, maxiter, meth=4, *opts):
     """Relaxes to prec (T) within maxiter iterations; returns
-    [misfit, max|M|, max|H|, iterations] (radentry.h:746-761).  Every method
-    number runs the device solver (Anderson-accelerated block Jacobi on the
-    FP32-stored matrix, then one FP64 refinement pass; option 'Refine->k'
-    sets k passes, 'Refine->-1' none)."""
+    [misfit, max|M|, max|H|, iterations] (radentry.h:746-761).  meth 11 runs
+    the device Newton-Krylov solver (misfit = RMS |M_mat(H) - M|, iterations
+    = matrix passes, maxiter caps them, as radGPU_RelaxNK reports,
+    radgpurlx.h:78-86); every other method number runs the device
+    Anderson-accelerated block Jacobi.  Both work on the FP32-stored matrix,
+    then one FP64 refinement pass (option 'Refine->k' sets k passes,
+    'Refine->-1' none)."""
     it = _obj(intrc)
     if not isinstance(it, _Interaction):
         raise RuntimeError("Radia::Error014: not an interaction structure")
@@ -458,7 +461,8 @@ def RlxAuto(intrc, prec, maxiter, meth=4, *opts):
     refine = int(_opt(opts, "Refine", "0"))
     M0 = np.stack([o.M for (o, _k, _s) in it.items])
     res, M, H = it.matrix.relax(it.mats, it.hext, M0, prec, maxiter, keep_m=not zero_m,
-                                omega0=omega, refine=refine, tab=it.tab)
+                                omega0=omega, refine=refine, tab=it.tab,
+                                method=11 if int(meth) == 11 else 9)
     for e, (o, _k, _s) in enumerate(it.items):
         o.M = M[e].copy()
     it.H = H
