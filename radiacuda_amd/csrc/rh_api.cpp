@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -958,7 +960,21 @@ rh_status nk_run(rh_matrix* q, const double* d_mats, const double* d_tab, const 
         return RH_OK;
     };
     rh_status st;
-    RH_HIP(hipMemcpyAsync(k.Mc, rw.x, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    // RADHIP_NK_PRESMOOTH: Anderson iterations before the first Newton step
+    // (default 10); RADHIP_NK_DEBUG=1: one line per Newton step on stderr.
+    const char* env_ps = std::getenv("RADHIP_NK_PRESMOOTH");
+    const int presmooth = env_ps ? std::max(0, std::atoi(env_ps)) : 10;
+    const bool debug = std::getenv("RADHIP_NK_DEBUG") != nullptr;
+    if (presmooth > 0) {
+        rh_relax_opts o2 = *opts;
+        o2.max_iter = std::min(presmooth, max_mv);
+        rh_relax_result r2{};
+        if ((st = relax_run(q, d_mats, d_tab, d_hext, d_M, d_H, &o2, rw, hbuf, nb, db, &r2)) != RH_OK) return st;
+        mv += r2.matvecs;
+        RH_HIP(hipMemcpyAsync(k.Mc, d_M, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+        RH_HIP(hipMemcpyAsync(k.Mc, rw.x, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
     double f2 = 0.0;
     if ((st = evaluate(k.Mc, k.F, k.D, k.P, k.H, f2)) != RH_OK) return st;
     double fn = std::sqrt(f2), fn_prev = 0.0, eta = 0.5;
@@ -1064,6 +1080,9 @@ rh_status nk_run(rh_matrix* q, const double* d_mats, const double* d_tab, const 
             }
             alpha *= 0.5;
         }
+        if (debug)
+            std::fprintf(stderr, "nk step %d: |F| %.3e eta %.2e gmres %d (resid %.2e) alpha %g %s, matvecs %d\n",
+                         newton, fn, eta, its, resid, alpha, accepted ? "accepted" : "rejected", mv);
         fn_prev = fn;
         if (accepted) {
             std::swap(k.Mc, k.Mt);

@@ -79,7 +79,6 @@ RH_HD V3<R> cross(const V3<R>& a, const V3<R>& b)
 
 RH_HD double rh_log(double x) { return log(x); }
 RH_HD float rh_log(float x) { return logf(x); }
-RH_HD float rh_atan2(float y, float x) { return atan2f(y, x); }
 // FP32 square root and reciprocal: the hardware instructions (about 1 ulp) on
 // the device instead of the correctly rounded library sequences.
 RH_HD float rh_sqrt(float x)
@@ -98,6 +97,27 @@ RH_HD float rh_rcp(float x)
     return 1.0f / x;
 #endif
 }
+// FP32 atan2: the octant reduction of the FP64 form below with a degree-4
+// polynomial in t^2 (Chebyshev interpolant of (atan(t)/t - 1)/t^2 on
+// |t| <= tan(pi/8), within 1.8e-8 of atan t in float arithmetic).
+RH_HD float rh_atan2(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const bool big = mn > 0.41421356f * mx;
+    const float t0 = (big ? mn - mx : mn) * rh_rcp(big ? mn + mx : mx);
+    const float t = (mx > 0.0f) ? t0 : 0.0f;
+    const float u = t * t;
+    const float p =
+        -0.3333333134651184f +
+        u * (0.19999539852142334f + u * (-0.14263956248760223f + u * (0.10743731260299683f + u * -0.06451927870512009f)));
+    float a = fmaf(t * u, p, t);
+    a = big ? a + 0.78539816f : a;
+    a = (ay > ax) ? 1.5707963f - a : a;
+    a = (x < 0.0f) ? 3.1415927f - a : a;
+    return copysignf(a, y);
+}
+
 // ln(1 + x) for x >= 0.  FP64: the library routine.  FP32: with u = fl(1 + x),
 // ln(u) * x / (u - 1) is accurate to a few ulp because u - 1 is exact and
 // ln(1 + y)/y varies slowly (Goldberg, "What every computer scientist should
@@ -107,7 +127,13 @@ RH_HD float rh_log1p(float x)
 {
     const float u = 1.0f + x;
     const float d = u - 1.0f;
-    const float r = rh_log(u) * (x * rh_rcp(d));
+#ifdef __HIP_DEVICE_COMPILE__
+    // u >= 1 is never subnormal: the hardware log2 (about 1 ulp) times ln 2
+    const float lu = __builtin_amdgcn_logf(u) * 0.69314718f;
+#else
+    const float lu = logf(u);
+#endif
+    const float r = lu * (x * rh_rcp(d));
     return (d == 0.0f) ? x : ((u < 3.0e38f) ? r : u);
 }
 
@@ -118,7 +144,7 @@ RH_HD float rh_log1p(float x)
 // instructions each on gfx950; the forms below cost about a third:
 //  * reciprocal: the hardware seed refined by two Newton-Raphson steps;
 //  * square root: the hardware reciprocal-square-root seed, one Goldschmidt
-//    step and two Newton corrections;
+//    step and one Newton correction;
 //  * ln(num/den): exponents split off, the mantissa ratio q brought into
 //    [1/sqrt2, sqrt2), ln q = 2 atanh(s) with s = (a - b)/(a + b) exact in
 //    the numerator, |s| <= 3 - 2 sqrt2, and a degree-6 polynomial in s^2;
@@ -174,13 +200,13 @@ RH_HD double rh_sqrt(double x)
 #else
     const double y = 1.0 / sqrt(x);
 #endif
+    // the seed is within 2^-24 (measured on gfx950): the Goldschmidt step
+    // leaves 2^-48, the Newton correction the rounding error
     double g = x * y, h = 0.5 * y;
     const double r = fma(-h, g, 0.5);
     g = fma(g, r, g);
     h = fma(h, r, h);
-    double d = fma(-g, g, x);
-    g = fma(d, h, g);
-    d = fma(-g, g, x);
+    const double d = fma(-g, g, x);
     return fma(d, h, g);
 }
 

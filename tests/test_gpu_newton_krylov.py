@@ -154,16 +154,20 @@ def test_nk_iron_cube_full_size():
 
 def test_nk_option_checks():
     print("case: newton-krylov option checks", flush=True)
-    top, _steel = models.hybrid_undulator(n_periods=1)
-    k = rad.RlxPre(top)
+    ids = models.tet_mesh((2, 2, 2), 4.0, jitter=0.1, seed=4, M=[0, 0, 1.0])
+    iron = rad.ObjCnt(ids)
+    rad.MatApl(iron, rad.MatLin([0.06, 0.17], 1.2))
+    k = rad.RlxPre(iron)
     it = rad._reg[k]
     n = len(it.items)
     with pytest.raises(N.RadHipError, match="krylov_m"):
-        it.matrix.relax(it.mats, it.hext, np.zeros((n, 3)), 1e-6, 10, tab=it.tab, method=11, krylov_m=64)
+        it.matrix.relax(it.mats, it.hext, np.zeros((n, 3)), 1e-6, 10, method=11, krylov_m=64)
     with pytest.raises(N.RadHipError, match="method"):
-        it.matrix.relax(it.mats, it.hext, np.zeros((n, 3)), 1e-6, 10, tab=it.tab, method=7)
-    # a small restart length still converges (restarted GMRES)
-    r, _M, _H = it.matrix.relax(it.mats, it.hext, np.zeros((n, 3)), 1e-8, 5000, tab=it.tab, method=11,
-                                krylov_m=4)
-    assert r.converged
+        it.matrix.relax(it.mats, it.hext, np.zeros((n, 3)), 1e-6, 10, method=7)
+    # restart lengths 2 and 60 reach the same state (restarted GMRES)
+    r2, M2, _H = it.matrix.relax(it.mats, it.hext, np.zeros((n, 3)), 1e-12, 2000, method=11, krylov_m=2)
+    r60, M60, _H = it.matrix.relax(it.mats, it.hext, np.zeros((n, 3)), 1e-12, 2000, method=11)
+    print("krylov_m 2: %d matvecs; 60: %d matvecs" % (r2.matvecs, r60.matvecs), flush=True)
+    assert r2.converged and r60.converged
+    assert np.abs(M2 - M60).max() <= 1e-10 * np.abs(M60).max()
     rad.UtiDel(k)
