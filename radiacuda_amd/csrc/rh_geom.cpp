@@ -524,13 +524,104 @@ void build_mesh_form(PackedModel& P)
     int64_t face_incidences = 0;
     for (const auto& f : finc) face_incidences += int64_t(f.size());
     if (double(finc.size()) > 0.8 * double(face_incidences)) return;
-    // edges
-    M.edge_inc_off.push_back(0);
+    // Edge paths: each unique edge oriented and ordered so that, where a path
+    // continues, its first vertex is the previous edge's second vertex (L_e
+    // and W_e do not depend on the orientation).  Triangle strips: where a
+    // strip continues, a triangle's first two vertices are the previous
+    // triangle's last two (the solid angle of a triangle does not depend on
+    // the order of its vertices once the normal is fixed).  The sweeps then
+    // compute one vertex distance per continued edge or triangle.
     std::vector<std::pair<int32_t, int32_t>> ekeys(einc.size());
     for (const auto& kv : eid) ekeys[size_t(kv.second)] = kv.first;
-    for (size_t e = 0; e < einc.size(); e++) {
-        const double* a = vpos[size_t(ekeys[e].first)];
-        const double* b = vpos[size_t(ekeys[e].second)];
+    std::vector<int32_t> eorder;
+    std::vector<char> eflip, econt;
+    {
+        std::vector<std::vector<int32_t>> vadj(vpos.size());
+        for (size_t e = 0; e < ekeys.size(); e++) {
+            vadj[size_t(ekeys[e].first)].push_back(int32_t(e));
+            vadj[size_t(ekeys[e].second)].push_back(int32_t(e));
+        }
+        std::vector<char> used(ekeys.size(), 0);
+        std::vector<size_t> next(vpos.size(), 0);
+        auto take = [&](int32_t v) -> int32_t {
+            std::vector<int32_t>& a = vadj[size_t(v)];
+            while (next[size_t(v)] < a.size() && used[size_t(a[next[size_t(v)]])]) next[size_t(v)]++;
+            return next[size_t(v)] < a.size() ? a[next[size_t(v)]] : -1;
+        };
+        for (size_t e0 = 0; e0 < ekeys.size(); e0++) {
+            if (used[e0]) continue;
+            int32_t cur = ekeys[e0].first, e = int32_t(e0);
+            bool first = true;
+            while (e >= 0) {
+                used[size_t(e)] = 1;
+                const bool flip = ekeys[size_t(e)].first != cur;
+                eorder.push_back(e);
+                eflip.push_back(flip ? 1 : 0);
+                econt.push_back(first ? 0 : 1);
+                first = false;
+                cur = flip ? ekeys[size_t(e)].first : ekeys[size_t(e)].second;
+                e = take(cur);
+            }
+        }
+    }
+    std::vector<int32_t> forder;
+    std::vector<std::vector<int32_t>> fverts;
+    std::vector<char> fcont;
+    {
+        std::map<std::pair<int32_t, int32_t>, std::vector<int32_t>> tri_at;
+        for (size_t f = 0; f < floop.size(); f++) {
+            if (floop[f].size() != 3) continue;
+            for (int k = 0; k < 3; k++) {
+                const int32_t a = floop[f][size_t(k)], b = floop[f][size_t((k + 1) % 3)];
+                tri_at[{std::min(a, b), std::max(a, b)}].push_back(int32_t(f));
+            }
+        }
+        std::vector<char> used(floop.size(), 0);
+        auto other = [&](int32_t a, int32_t b) -> int32_t {
+            auto it = tri_at.find({std::min(a, b), std::max(a, b)});
+            if (it == tri_at.end()) return -1;
+            for (int32_t g : it->second)
+                if (!used[size_t(g)]) return g;
+            return -1;
+        };
+        for (size_t f0 = 0; f0 < floop.size(); f0++) {
+            if (used[f0] || floop[f0].size() != 3) continue;
+            std::vector<int32_t> v = floop[f0];
+            used[f0] = 1;
+            // start with the rotation whose last edge continues the strip
+            for (int r = 0; r < 3; r++) {
+                if (other(v[1], v[2]) >= 0) break;
+                std::rotate(v.begin(), v.begin() + 1, v.end());
+            }
+            forder.push_back(int32_t(f0));
+            fverts.push_back(v);
+            fcont.push_back(0);
+            int32_t p = v[1], q = v[2];
+            for (int32_t g = other(p, q); g >= 0; g = other(p, q)) {
+                used[size_t(g)] = 1;
+                int32_t r = -1;
+                for (int32_t x : floop[size_t(g)])
+                    if (x != p && x != q) r = x;
+                forder.push_back(g);
+                fverts.push_back({p, q, r});
+                fcont.push_back(1);
+                p = q;
+                q = r;
+            }
+        }
+        for (size_t f = 0; f < floop.size(); f++)
+            if (floop[f].size() != 3) {
+                forder.push_back(int32_t(f));
+                fverts.push_back(floop[f]);
+                fcont.push_back(2);  // quadrilateral
+            }
+    }
+    // edges
+    M.edge_inc_off.push_back(0);
+    for (size_t o = 0; o < eorder.size(); o++) {
+        const size_t e = size_t(eorder[o]);
+        const double* a = vpos[size_t(eflip[o] ? ekeys[e].second : ekeys[e].first)];
+        const double* b = vpos[size_t(eflip[o] ? ekeys[e].first : ekeys[e].second)];
         double t[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
         const double len = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
         for (int k = 0; k < 3; k++) t[k] /= len;
@@ -538,6 +629,7 @@ void build_mesh_form(PackedModel& P)
         for (int k = 0; k < 3; k++) M.edge_geo.push_back(b[k]);
         for (int k = 0; k < 3; k++) M.edge_geo.push_back(t[k]);
         M.edge_geo.push_back(len);
+        M.edge_cont.push_back(double(econt[o]));
         for (const auto& inc : einc[e]) {
             M.edge_inc_elem.push_back(inc.first);
             for (int k = 0; k < 6; k++) M.edge_inc_w.push_back(inc.second[k]);
@@ -546,8 +638,9 @@ void build_mesh_form(PackedModel& P)
     }
     // faces
     M.face_inc_off.push_back(0);
-    for (size_t f = 0; f < finc.size(); f++) {
-        const std::vector<int32_t>& loop = floop[f];
+    for (size_t o = 0; o < forder.size(); o++) {
+        const size_t f = size_t(forder[o]);
+        const std::vector<int32_t>& loop = fverts[o];
         const double* n = fnorm[f];
         double vv[4][3] = {{0}};
         for (size_t k = 0; k < loop.size(); k++)
@@ -565,6 +658,7 @@ void build_mesh_form(PackedModel& P)
         M.face_geo.push_back(area2(0, 1, 2));
         M.face_geo.push_back(loop.size() > 3 ? area2(0, 2, 3) : 0.0);
         M.face_geo.push_back(double(loop.size()));
+        M.face_cont.push_back(double(fcont[o]));
         for (const auto& inc : finc[f]) {
             M.face_inc_elem.push_back(inc.first);
             M.face_inc_sign.push_back(inc.second);

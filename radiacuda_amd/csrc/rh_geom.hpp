@@ -30,6 +30,9 @@ struct MeshForm {
     std::vector<int32_t> in_elem;        // element of each inside record
     std::vector<double> in_geo;          // 1 + 4 kMeshPlanes per element: count, planes
     std::vector<double> tile_box;        // 6 per tile of kMeshTile inside records
+    std::vector<double> edge_cont;       // 1: the edge starts at the previous edge's end
+    std::vector<double> face_cont;       // 0: triangle, 1: triangle whose vertices 0, 1 are the
+                                         // previous face's 1, 2; 2: quadrilateral
     int64_t n_edges() const { return int64_t(edge_inc_off.empty() ? 0 : edge_inc_off.size() - 1); }
     int64_t n_faces() const { return int64_t(face_inc_off.empty() ? 0 : face_inc_off.size() - 1); }
     int64_t n_in() const { return int64_t(in_elem.size()); }

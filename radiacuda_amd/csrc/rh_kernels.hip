@@ -249,7 +249,7 @@ __global__ __launch_bounds__(256) void k_mesh_charge_edges(MeshDev md, const dou
     o[10] = w[0];
     o[11] = w[1];
     o[12] = w[2];
-    o[13] = 0.0;
+    o[13] = md.edge_cont[i];
 }
 
 __global__ __launch_bounds__(256) void k_mesh_charge_faces(MeshDev md, const double* __restrict__ M)
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_mesh_charge_faces(MeshDev md, const dou
     o[18] = s * n0;
     o[19] = s * n1;
     o[20] = s * n2;
-    o[21] = 0.0;
+    o[21] = md.face_cont[i];
 }
 
 __global__ __launch_bounds__(256) void k_mesh_charge_in(MeshDev md, const double* __restrict__ M)
@@ -307,8 +307,11 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
     }
     double h[3] = {0.0, 0.0, 0.0}, min[3] = {0.0, 0.0, 0.0};
     double2* dst = reinterpret_cast<double2*>(tile);
-    // unique edges: + L_e w_e
+    // unique edges: + L_e w_e, along paths: a continued edge takes its first
+    // vertex's offset and distance from the previous edge
     {
+        V3<R> cb = v3(R(0), R(0), R(0));
+        R Cb = R(0);
         const int64_t e0 = md.ne * y / S, e1 = md.ne * (y + 1) / S;
         for (int64_t base = e0; base < e1; base += kMeshTile) {
             const int cnt = int((e1 - base < kMeshTile) ? (e1 - base) : kMeshTile);
@@ -320,15 +323,31 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
 #pragma unroll 1
             for (int q = 0; q < cnt; q++) {
                 const double* rec = tile + kMeshEdge * q;
-                const double L = mesh_edge_L<R>(rec, x);
+                V3<R> ra;
+                R Ra;
+                if (mesh_code(rec + 13) == 1 && base + q != e0) {
+                    ra = cb;
+                    Ra = Cb;
+                } else {
+                    ra = rel<R>(rec, x);
+                    Ra = rh_sqrt(dot(ra, ra));
+                }
+                const V3<R> rb = rel<R>(rec + 3, x);
+                const R Rb = rh_sqrt(dot(rb, rb));
+                const double L = mesh_edge_L_from<R>(ra, Ra, rb, Rb, rec);
                 h[0] += L * rec[10];
                 h[1] += L * rec[11];
                 h[2] += L * rec[12];
+                cb = rb;
+                Cb = Rb;
             }
         }
     }
-    // unique faces: - Omega_f g_f
+    // unique faces: - Omega_f g_f, triangles along strips: a continued
+    // triangle takes its first two vertices from the previous one
     {
+        V3<R> c1 = v3(R(0), R(0), R(0)), c2 = c1;
+        R C1 = R(0), C2 = R(0), D12 = R(0);
         const int64_t f0 = md.nf * y / S, f1 = md.nf * (y + 1) / S;
         for (int64_t base = f0; base < f1; base += kMeshTile) {
             const int cnt = int((f1 - base < kMeshTile) ? (f1 - base) : kMeshTile);
@@ -340,7 +359,36 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
 #pragma unroll 1
             for (int q = 0; q < cnt; q++) {
                 const double* rec = tile + kMeshFace * q;
-                const double om = mesh_face_omega<R>(rec, x);
+                const int code = mesh_code(rec + 21);
+                double om;
+                if (code == 2) {
+                    om = mesh_face_omega<R>(rec, x);
+                } else {
+                    V3<R> r0, r1;
+                    R R0, R1, d01;
+                    if (code == 1 && base + q != f0) {
+                        r0 = c1;
+                        R0 = C1;
+                        r1 = c2;
+                        R1 = C2;
+                        d01 = D12;
+                    } else {
+                        r0 = rel<R>(rec, x);
+                        R0 = rh_sqrt(dot(r0, r0));
+                        r1 = rel<R>(rec + 3, x);
+                        R1 = rh_sqrt(dot(r1, r1));
+                        d01 = dot(r0, r1);
+                    }
+                    const V3<R> r2 = rel<R>(rec + 6, x);
+                    const R R2 = rh_sqrt(dot(r2, r2));
+                    const R d12 = dot(r1, r2);
+                    om = mesh_tri_omega_from<R>(r0, R0, r1, R1, r2, R2, d01, d12, rec);
+                    c1 = r1;
+                    C1 = R1;
+                    c2 = r2;
+                    C2 = R2;
+                    D12 = d12;
+                }
                 h[0] -= om * rec[18];
                 h[1] -= om * rec[19];
                 h[2] -= om * rec[20];

@@ -25,6 +25,8 @@ struct MeshDev {
     const int32_t* in_elem = nullptr;
     const double* in_geo = nullptr;
     const double* tile_box = nullptr;
+    const double* edge_cont = nullptr;
+    const double* face_cont = nullptr;
     double* ch_edge = nullptr;
     double* ch_face = nullptr;
     double* ch_in = nullptr;

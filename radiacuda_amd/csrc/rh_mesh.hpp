@@ -17,10 +17,14 @@
 
 namespace rh {
 
-// Charged edge record: a(3), b(3), t(3), len, w(3), pad.
+// Charged edge record: a(3), b(3), t(3), len, w(3), cont (1: a is the
+// previous record's b).
 constexpr int kMeshEdge = 14;
-// Charged face record (triangle or quadrilateral, CCW about n): v0..v3 (12),
-// n(3), 2*area(v0,v1,v2), 2*area(v0,v2,v3), nv, g(3), pad.
+// Charged face record: v0..v3 (12), n(3), 2*area(v0,v1,v2), 2*area(v0,v2,v3),
+// nv, g(3), code (0 triangle, 1 triangle whose v0, v1 are the previous
+// record's v1, v2, 2 quadrilateral).  A
+// quadrilateral is CCW about n; a triangle's vertices may come in either
+// order (its solid angle below uses n and the area, not the order).
 constexpr int kMeshFace = 22;
 // Inside-test record: nplanes, M(3), up to 8 planes (n(3), d) with outward n:
 // inside iff n.p < d for every plane.
@@ -64,6 +68,41 @@ RH_HD double mesh_face_omega(const double* __restrict__ rec, const double p[3])
         re = pr;
         im = pi;
     }
+    return 2.0 * double(rh_atan2(im, re));
+}
+
+// Wave-uniform integer of a 0/1/2 code stored as a double: its high word
+// (0, 0x3ff00000, 0x40000000) read into a scalar register.
+RH_HD int mesh_code(const double* __restrict__ slot)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    const int hi = __builtin_amdgcn_readfirstlane(reinterpret_cast<const int*>(slot)[1]);
+#else
+    const int hi = int(uint64_t(__builtin_bit_cast(uint64_t, *slot)) >> 32);
+#endif
+    return hi == 0 ? 0 : (hi == 0x3ff00000 ? 1 : 2);
+}
+
+// L_e of a charged edge from the offsets and distances of its ends.
+template <class R>
+RH_HD double mesh_edge_L_from(const V3<R>& ra, R Ra, const V3<R>& rb, R Rb, const double* __restrict__ rec)
+{
+    const V3<R> t = v3(R(rec[6]), R(rec[7]), R(rec[8]));
+    if (sizeof(R) == 4) return double(edge_log1p(ra, Ra, rb, Rb, t, R(rec[9])));
+    R num, den;
+    edge_ratio(ra, Ra, rb, Rb, t, num, den);
+    return double(rh_log_ratio(num, den));
+}
+
+// Signed solid angle of a charged triangle from the offsets and distances of
+// its vertices and r0 . r1.
+template <class R>
+RH_HD double mesh_tri_omega_from(const V3<R>& r0, R R0, const V3<R>& r1, R R1, const V3<R>& r2, R R2, R d01,
+                                 R d12, const double* __restrict__ rec)
+{
+    const V3<R> n = v3(R(rec[12]), R(rec[13]), R(rec[14]));
+    const R im = R(rec[15]) * dot(r0, n);
+    const R re = R0 * R1 * R2 + d01 * R2 + dot(r0, r2) * R1 + d12 * R0;
     return 2.0 * double(rh_atan2(im, re));
 }
 
