@@ -308,11 +308,20 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
     double h[3] = {0.0, 0.0, 0.0}, min[3] = {0.0, 0.0, 0.0};
     double2* dst = reinterpret_cast<double2*>(tile);
     // unique edges: + L_e w_e, along paths: a continued edge takes its first
-    // vertex's offset and distance from the previous edge
+    // vertex's offset and distance from the previous edge.  The loop is
+    // unrolled by two with the vertex slots P, Q swapping roles, so the
+    // carried values stay in place (no register copies per edge).
     {
-        V3<R> cb = v3(R(0), R(0), R(0));
-        R Cb = R(0);
+        MeshVtx<R> P{}, Q{};
         const int64_t e0 = md.ne * y / S, e1 = md.ne * (y + 1) / S;
+        auto edge = [&](int64_t g, const double* rec, MeshVtx<R>& a, MeshVtx<R>& b) {
+            if (!(mesh_code(rec + 13) == 1 && g != e0)) a = mesh_vtx<R>(rec, x);
+            b = mesh_vtx<R>(rec + 3, x);
+            const double L = mesh_edge_L_from<R>(a.r, a.d, b.r, b.d, rec);
+            h[0] += L * rec[10];
+            h[1] += L * rec[11];
+            h[2] += L * rec[12];
+        };
         for (int64_t base = e0; base < e1; base += kMeshTile) {
             const int cnt = int((e1 - base < kMeshTile) ? (e1 - base) : kMeshTile);
             const double2* src = reinterpret_cast<const double2*>(md.ch_edge + kMeshEdge * base);
@@ -320,35 +329,45 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
             for (int i = threadIdx.x; i < cnt * (kMeshEdge / 2); i += 256) dst[i] = src[i];
             __syncthreads();
             if (!live) continue;
+            int q = 0;
 #pragma unroll 1
-            for (int q = 0; q < cnt; q++) {
-                const double* rec = tile + kMeshEdge * q;
-                V3<R> ra;
-                R Ra;
-                if (mesh_code(rec + 13) == 1 && base + q != e0) {
-                    ra = cb;
-                    Ra = Cb;
-                } else {
-                    ra = rel<R>(rec, x);
-                    Ra = rh_sqrt(dot(ra, ra));
-                }
-                const V3<R> rb = rel<R>(rec + 3, x);
-                const R Rb = rh_sqrt(dot(rb, rb));
-                const double L = mesh_edge_L_from<R>(ra, Ra, rb, Rb, rec);
-                h[0] += L * rec[10];
-                h[1] += L * rec[11];
-                h[2] += L * rec[12];
-                cb = rb;
-                Cb = Rb;
+            for (; q + 2 <= cnt; q += 2) {
+                edge(base + q, tile + kMeshEdge * q, P, Q);
+                edge(base + q + 1, tile + kMeshEdge * (q + 1), Q, P);
+            }
+            if (q < cnt) {
+                edge(base + q, tile + kMeshEdge * q, P, Q);
+                P = Q;
             }
         }
     }
     // unique faces: - Omega_f g_f, triangles along strips: a continued
-    // triangle takes its first two vertices from the previous one
+    // triangle takes its first two vertices (and their dot product) from the
+    // previous one.  Unrolled by three with the slots A, B, C rotating.
     {
-        V3<R> c1 = v3(R(0), R(0), R(0)), c2 = c1;
-        R C1 = R(0), C2 = R(0), D12 = R(0);
+        MeshVtx<R> A{}, Bv{}, C{};
+        R dAB = R(0), dBC = R(0), dCA = R(0);
         const int64_t f0 = md.nf * y / S, f1 = md.nf * (y + 1) / S;
+        auto face = [&](int64_t g, const double* rec, MeshVtx<R>& a, MeshVtx<R>& b, MeshVtx<R>& c, R& dab,
+                        R& dbc) {
+            const int code = mesh_code(rec + 21);
+            double om;
+            if (code == 2) {
+                om = mesh_face_omega<R>(rec, x);
+            } else {
+                if (!(code == 1 && g != f0)) {
+                    a = mesh_vtx<R>(rec, x);
+                    b = mesh_vtx<R>(rec + 3, x);
+                    dab = dot(a.r, b.r);
+                }
+                c = mesh_vtx<R>(rec + 6, x);
+                dbc = dot(b.r, c.r);
+                om = mesh_tri_omega_from<R>(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, rec);
+            }
+            h[0] -= om * rec[18];
+            h[1] -= om * rec[19];
+            h[2] -= om * rec[20];
+        };
         for (int64_t base = f0; base < f1; base += kMeshTile) {
             const int cnt = int((f1 - base < kMeshTile) ? (f1 - base) : kMeshTile);
             const double2* src = reinterpret_cast<const double2*>(md.ch_face + kMeshFace * base);
@@ -356,42 +375,26 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
             for (int i = threadIdx.x; i < cnt * (kMeshFace / 2); i += 256) dst[i] = src[i];
             __syncthreads();
             if (!live) continue;
+            int q = 0;
 #pragma unroll 1
-            for (int q = 0; q < cnt; q++) {
-                const double* rec = tile + kMeshFace * q;
-                const int code = mesh_code(rec + 21);
-                double om;
-                if (code == 2) {
-                    om = mesh_face_omega<R>(rec, x);
+            for (; q + 3 <= cnt; q += 3) {
+                face(base + q, tile + kMeshFace * q, A, Bv, C, dAB, dBC);
+                face(base + q + 1, tile + kMeshFace * (q + 1), Bv, C, A, dBC, dCA);
+                face(base + q + 2, tile + kMeshFace * (q + 2), C, A, Bv, dCA, dAB);
+            }
+            // remainder, then the carried pair back into (A, B)
+            if (q < cnt) {
+                face(base + q, tile + kMeshFace * q, A, Bv, C, dAB, dBC);
+                if (q + 1 < cnt) {
+                    face(base + q + 1, tile + kMeshFace * (q + 1), Bv, C, A, dBC, dCA);
+                    Bv = A;
+                    A = C;
+                    dAB = dCA;
                 } else {
-                    V3<R> r0, r1;
-                    R R0, R1, d01;
-                    if (code == 1 && base + q != f0) {
-                        r0 = c1;
-                        R0 = C1;
-                        r1 = c2;
-                        R1 = C2;
-                        d01 = D12;
-                    } else {
-                        r0 = rel<R>(rec, x);
-                        R0 = rh_sqrt(dot(r0, r0));
-                        r1 = rel<R>(rec + 3, x);
-                        R1 = rh_sqrt(dot(r1, r1));
-                        d01 = dot(r0, r1);
-                    }
-                    const V3<R> r2 = rel<R>(rec + 6, x);
-                    const R R2 = rh_sqrt(dot(r2, r2));
-                    const R d12 = dot(r1, r2);
-                    om = mesh_tri_omega_from<R>(r0, R0, r1, R1, r2, R2, d01, d12, rec);
-                    c1 = r1;
-                    C1 = R1;
-                    c2 = r2;
-                    C2 = R2;
-                    D12 = d12;
+                    A = Bv;
+                    Bv = C;
+                    dAB = dBC;
                 }
-                h[0] -= om * rec[18];
-                h[1] -= om * rec[19];
-                h[2] -= om * rec[20];
             }
         }
     }

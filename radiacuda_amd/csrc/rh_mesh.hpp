@@ -83,6 +83,19 @@ RH_HD int mesh_code(const double* __restrict__ slot)
     return hi == 0 ? 0 : (hi == 0x3ff00000 ? 1 : 2);
 }
 
+// A vertex relative to the observation point and its distance.
+template <class R>
+struct MeshVtx {
+    V3<R> r;
+    R d;
+};
+template <class R>
+RH_HD MeshVtx<R> mesh_vtx(const double* __restrict__ v, const double p[3])
+{
+    const V3<R> r = rel<R>(v, p);
+    return MeshVtx<R>{r, rh_sqrt(dot(r, r))};
+}
+
 // L_e of a charged edge from the offsets and distances of its ends.
 template <class R>
 RH_HD double mesh_edge_L_from(const V3<R>& ra, R Ra, const V3<R>& rb, R Rb, const double* __restrict__ rec)
