@@ -962,13 +962,17 @@ rh_status nk_run(rh_matrix* q, const double* d_mats, const double* d_tab, const 
         return RH_OK;
     };
     rh_status st;
-    // RADHIP_NK_PRESMOOTH: Anderson iterations before the first Newton step
-    // (default 10); RADHIP_NK_DEBUG=1: one line per Newton step on stderr.
+    // Newton steps from a far start need heavy damping on stiff (high-chi)
+    // models, so Anderson iterations run first until their misfit is within
+    // 100 x prec (at most RADHIP_NK_PRESMOOTH iterations, default 200);
+    // Newton then converges in a few steps.  RADHIP_NK_DEBUG=1: one line
+    // per Newton step on stderr.
     const char* env_ps = std::getenv("RADHIP_NK_PRESMOOTH");
-    const int presmooth = env_ps ? std::max(0, std::atoi(env_ps)) : 10;
+    const int presmooth = env_ps ? std::max(0, std::atoi(env_ps)) : 200;
     const bool debug = std::getenv("RADHIP_NK_DEBUG") != nullptr;
     if (presmooth > 0) {
         rh_relax_opts o2 = *opts;
+        o2.prec = 100.0 * prec;
         o2.max_iter = std::min(presmooth, max_mv);
         rh_relax_result r2{};
         if ((st = relax_run(q, d_mats, d_tab, d_hext, d_M, d_H, &o2, rw, hbuf, nb, db, &r2)) != RH_OK) return st;
