@@ -244,3 +244,60 @@ def test_field_mesh_form(oracle_off):
         Bo, _Ho, _bad = models.oracle_field(P, M, sub)
         assert np.abs(Bm[:200] - Bo).max() <= 1e-10 * np.abs(Bo).max()
         model.close()
+
+
+@pytest.mark.parametrize("t", [1.0, 1e-2, 1e-4, 1e-6, 1e-8])
+def test_sliver_tetrahedron_on_device(oracle_off, t):
+    """The reference's flatness sweep (tests/test_sliver_tet_asm.py) on the
+    device: assembly (FP64, FP32 storage) and fields finite down to 1e-8 mm,
+    the FP64 self block and H within 1e-10 of 50-digit values, FP32 within the
+    reference's 5e-5; the drop-in solve runs on the GPU backend and stays
+    finite."""
+    from tests.test_hostcheck import SLIVER_OBS, sliver_model, sliver_mp50
+    print("case: sliver t=%g" % t, flush=True)
+    top = sliver_model(t)
+    P, items = models.packed(top)
+    H50, Q50 = sliver_mp50(t, P["M"][0], P["M"][1], SLIVER_OBS)
+    model = N.Model(P)
+    for prec in (N.RH_FP64, N.RH_MIXED):
+        Q = model.assemble(prec)
+        Qs = Q.export()
+        Q.close()
+        assert np.isfinite(Qs).all()
+        tol = 1e-10 if prec == N.RH_FP64 else 1e-6
+        assert np.abs(Qs[0:3, 0:3] - Q50).max() <= tol
+    for prec, tol in ((N.RH_FP64, 1e-10), (N.RH_MIXED, 5e-5)):
+        B, H = model.field(P["M"], SLIVER_OBS, precision=prec)
+        assert np.isfinite(B).all() and np.isfinite(H).all()
+        assert np.abs(H - H50).max() <= tol * np.abs(H50).max()
+    model.close()
+    r = rad.Solve(top, 1e-5, 1000)
+    assert rad.UtiAsmLastBackend() in ("gpu", "gpu_streaming")
+    Bs = np.array([rad.Fld(top, "b", list(p)) for p in SLIVER_OBS])
+    assert r[0] <= 1e-5 and np.isfinite(Bs).all()
+
+
+def test_repeated_solve_and_field_are_bit_identical(oracle_off):
+    """Every reduction runs in a fixed order: the same solve and the same
+    field sweep twice give identical bits (the reference repeats a GPU solve
+    and compares %.17e strings, tests/test_gpu_asm_cpu_solve.py:187-200)."""
+    print("case: repeated solve and field", flush=True)
+    ids = models.tet_mesh((4, 3, 2), 4.0, jitter=0.1, seed=11, M=[0, 0, 0])
+    iron = rad.ObjCnt(ids)
+    rad.MatApl(iron, rad.MatSatIsoFrm([2118, 1.362], [63.06, 0.2605], [17.138, 0.4917]))
+    src = rad.ObjRecCur([-8, 6, 4], [4, 12, 8], [0, 0, 2.0])
+    top = rad.ObjCnt([iron, src])
+    k = rad.RlxPre(top)
+    it = rad._reg[k]
+    n = len(it.items)
+    pts = np.random.default_rng(4).uniform(-6, 20, (5000, 3))
+    runs = []
+    for _ in range(2):
+        r, M, H = it.matrix.relax(it.mats, it.hext, np.zeros((n, 3)), 1e-10, 2000)
+        B32 = it.model.field(M, pts, precision=N.RH_MIXED)[0]
+        B64 = it.model.field(M, pts, precision=N.RH_FP64)[0]
+        runs.append((r.iters, M, H, B32, B64))
+    assert runs[0][0] == runs[1][0]
+    for a, b in zip(runs[0][1:], runs[1][1:]):
+        assert np.array_equal(a, b)
+    rad.UtiDel(k)

@@ -182,3 +182,62 @@ def test_kernel_math_matches_50_digit_values_where_reference_form_loses_digits(L
     # documents the reference forms' loss on these elements
     assert worst_o > 1e4 * worst_k
     assert worst_oz > 1e-5
+
+
+SLIVER_THICKNESS = [1.0, 1e-2, 1e-4, 1e-6, 1e-8]
+SLIVER_OBS = np.array([[20.0, 5.0, 3.0], [-15.0, 8.0, -4.0], [5.0, 5.0, 12.0], [0.3, 0.2, 0.05]])
+SLIVER_FACES = [[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]]
+BLOCK_FACES = [[0, 1, 3, 2], [4, 5, 7, 6], [0, 1, 5, 4], [2, 3, 7, 6], [0, 2, 6, 4], [1, 3, 7, 5]]
+
+
+def sliver_verts(t):
+    return [[0, 0, 0], [10, 0, 0], [0, 10, 0], [3, 3, t]]
+
+
+def sliver_model(t):
+    """A sliver tetrahedron (apex t mm above its base) and a solid block, as
+    the reference's flatness sweep builds them (tests/test_sliver_tet_asm.py:39-53)."""
+    sliver = rad.ObjPolyhdr(sliver_verts(t), [[1, 2, 3], [1, 2, 4], [1, 3, 4], [2, 3, 4]], [0, 0, 1])
+    rad.MatApl(sliver, rad.MatLin([0.06, 0.17], 1.2))
+    block = rad.ObjRecMag([6, 6, 12], [8, 8, 6], [0, 0, 1])
+    rad.MatApl(block, rad.MatLin([0.06, 0.17], 1.2))
+    return rad.ObjCnt([sliver, block])
+
+
+def sliver_mp50(t, M_sliver, M_block, pts):
+    """H of the sliver and the block at pts (50-digit face-charge closed form),
+    and the sliver's self block (columns = H at its centroid for M = e_k)."""
+    sys_path = os.path.join(HERE, "golden")
+    import sys
+    if sys_path not in sys.path:
+        sys.path.insert(0, sys_path)
+    from make_mp_golden import H_mp
+    bv = [[6 + sx * 4, 6 + sy * 4, 12 + sz * 3] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)]
+    H = np.array([[float(a) + float(b) for a, b in zip(H_mp(sliver_verts(t), SLIVER_FACES, M_sliver, p),
+                                                       H_mp(bv, BLOCK_FACES, M_block, p))] for p in pts])
+    c = np.mean(np.array(sliver_verts(t), dtype=float), axis=0)
+    Qs = np.array([[float(x) for x in H_mp(sliver_verts(t), SLIVER_FACES, e, c)] for e in np.eye(3)]).T
+    return H, Qs
+
+
+@pytest.mark.parametrize("t", SLIVER_THICKNESS)
+def test_sliver_tetrahedron_stays_finite_and_exact(L, t):
+    """A sliver down to 1e-8 mm (the reference's flatness sweep): interaction
+    blocks and fields finite; the sliver's self block and H at the sweep's
+    points within 1e-10 of 50-digit values in FP64 (5e-5, the reference's
+    tolerance, in FP32); the oracle agrees while it can (t >= 1e-4).  Below
+    that the oracle -- the reference's face-frame form, randomization off --
+    loses the self block (Q_zz -> 0 instead of -1 at t = 1e-6)."""
+    top = sliver_model(t)
+    P, _ = models.packed(top)
+    Q = _assemble(L, P)
+    assert np.isfinite(Q).all()
+    H50, Q50 = sliver_mp50(t, P["M"][0], P["M"][1], SLIVER_OBS)
+    assert np.abs(Q[0:3, 0:3] - Q50).max() <= 1e-10
+    for fp32, tol in ((0, 1e-10), (1, 5e-5)):
+        B, H = _field(L, P, P["M"], SLIVER_OBS, fp32, kinds=bool(fp32))
+        assert np.isfinite(B).all() and np.isfinite(H).all()
+        assert np.abs(H - H50).max() <= tol * np.abs(H50).max()
+    if t >= 1e-4:
+        Qo, _jit = models.oracle_assemble(P)
+        assert np.abs(Q - Qo).max() <= 5e-5 * np.abs(Qo).max()
