@@ -153,6 +153,10 @@ constexpr int kTetTile = 32;
 // occupancy floor for the tetrahedron sweep (waves per SIMD): bounds the VGPRs
 // the scheduler may use for hoisted LDS reads
 #ifndef RH_TET_WAVES
+// edge and face records staged per LDS tile of the mesh sweep
+#ifndef RH_MESH_STAGE
+#define RH_MESH_STAGE 128
+#endif
 #define RH_TET_WAVES 3
 #endif
 
@@ -294,7 +298,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH_TET_WAVE
 k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, int64_t slice0,
              double* __restrict__ B, double* __restrict__ H, double* __restrict__ part)
 {
-    __shared__ __align__(16) double tile[kMeshTile * kMeshIn];
+    __shared__ __align__(16) double tile[RH_MESH_STAGE * kMeshFace > kMeshTile * kMeshIn ? RH_MESH_STAGE * kMeshFace
+                                                                                     : kMeshTile * kMeshIn];
     const MeshDev& md = m.mesh;
     const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
     const bool live = p < np;
@@ -322,8 +327,8 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
             h[1] += L * rec[11];
             h[2] += L * rec[12];
         };
-        for (int64_t base = e0; base < e1; base += kMeshTile) {
-            const int cnt = int((e1 - base < kMeshTile) ? (e1 - base) : kMeshTile);
+        for (int64_t base = e0; base < e1; base += RH_MESH_STAGE) {
+            const int cnt = int((e1 - base < RH_MESH_STAGE) ? (e1 - base) : RH_MESH_STAGE);
             const double2* src = reinterpret_cast<const double2*>(md.ch_edge + kMeshEdge * base);
             __syncthreads();
             for (int i = threadIdx.x; i < cnt * (kMeshEdge / 2); i += 256) dst[i] = src[i];
@@ -368,8 +373,8 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
             h[1] -= om * rec[19];
             h[2] -= om * rec[20];
         };
-        for (int64_t base = f0; base < f1; base += kMeshTile) {
-            const int cnt = int((f1 - base < kMeshTile) ? (f1 - base) : kMeshTile);
+        for (int64_t base = f0; base < f1; base += RH_MESH_STAGE) {
+            const int cnt = int((f1 - base < RH_MESH_STAGE) ? (f1 - base) : RH_MESH_STAGE);
             const double2* src = reinterpret_cast<const double2*>(md.ch_face + kMeshFace * base);
             __syncthreads();
             for (int i = threadIdx.x; i < cnt * (kMeshFace / 2); i += 256) dst[i] = src[i];

@@ -145,19 +145,20 @@ RH_HD float rh_log1p(float x)
 //  * reciprocal: the hardware seed refined by two Newton-Raphson steps;
 //  * square root: the hardware reciprocal-square-root seed, one Goldschmidt
 //    step and one Newton correction;
-//  * ln(num/den): exponents split off, the mantissa ratio q brought into
-//    [1/sqrt2, sqrt2), ln q = 2 atanh(s) with s = (a - b)/(a + b) exact in
-//    the numerator, |s| <= 3 - 2 sqrt2, and a degree-6 polynomial in s^2;
+//  * ln(num/den): exponents split off, ln(a/b) = 2 atanh(s) for the
+//    mantissas a, b in [1/2, 1) with s = (a - b)/(a + b) exact in the
+//    numerator, |s| < 1/3, and a degree-9 polynomial in s^2;
 //  * atan2: the octant reduced to |t| <= tan(pi/8) (atan r = pi/4 +
 //    atan((r - 1)/(r + 1)) above it, one division either way) and a
 //    degree-10 polynomial in t^2.
 // The polynomials are Chebyshev interpolants of (2 atanh(s)/s - 2)/s^2 and
 // (atan(t)/t - 1)/t^2 on those ranges, fitted at 60 digits; evaluated in
-// double they are within 3e-17 absolute of ln q and atan t.  The host build
+// double they are within 6e-17 absolute of ln(a/b) and 3e-17 of atan t.  The host build
 // runs the same code with exactly rounded seeds.
-constexpr double kLogP[7] = {0.666666666666667,   0.39999999999899505, 0.28571428625975487,
-                             0.2222221113479508,  0.18182889125261723, 0.15331721600556042,
-                             0.14616449685043406};
+constexpr double kLogP[10] = {0.6666666666666666,  0.40000000000014585, 0.2857142856710876,
+                              0.22222222717427864, 0.18181789436132859, 0.15385570064958623,
+                              0.13314144209654935, 0.12001072809590713, 0.08800738486144674,
+                              0.1616404561194353};
 constexpr double kAtanP[11] = {-0.3333333333333333,  0.1999999999999552,   -0.14285714284666542,
                                0.11111111015256361,  -0.09090904578123903, 0.07692183190826087,
                                -0.06664511447381948, 0.0585814891280221,   -0.0508544973794026,
@@ -166,9 +167,10 @@ constexpr double kAtanP[11] = {-0.3333333333333333,  0.1999999999999552,   -0.14
 // The same coefficients in constant memory: the kernels then hold them in
 // scalar registers and the Horner steps take them as the scalar operand of
 // v_fma_f64 (as immediates they are copied into vector registers per step).
-static __constant__ double d_log_p[7] = {0.666666666666667,   0.39999999999899505, 0.28571428625975487,
-                                         0.2222221113479508,  0.18182889125261723, 0.15331721600556042,
-                                         0.14616449685043406};
+static __constant__ double d_log_p[10] = {0.6666666666666666,  0.40000000000014585, 0.2857142856710876,
+                                          0.22222222717427864, 0.18181789436132859, 0.15385570064958623,
+                                          0.13314144209654935, 0.12001072809590713, 0.08800738486144674,
+                                          0.1616404561194353};
 static __constant__ double d_atan_p[11] = {
     -0.3333333333333333,  0.1999999999999552,   -0.14285714284666542, 0.11111111015256361,
     -0.09090904578123903, 0.07692183190826087,  -0.06664511447381948, 0.0585814891280221,
@@ -176,7 +178,6 @@ static __constant__ double d_atan_p[11] = {
 #endif
 constexpr double kLn2Hi = 0.6931471803691238;  // ln 2 to 32 bits: k * kLn2Hi is exact
 constexpr double kLn2Lo = 1.9082149292705877e-10;
-constexpr double kSqrt2 = 1.4142135623730951;
 constexpr double kTanPi8 = 0.41421356237309503;
 constexpr double kPi4 = 0.7853981633974483, kPi2 = 1.5707963267948966, kPi = 3.141592653589793;
 
@@ -220,17 +221,15 @@ RH_HD void rh_frexp(double x, double& m, int& e)
 #endif
 }
 
-// ln(num/den), num and den positive, finite and normal.
+// ln(num/den), num and den positive, finite and normal: with the mantissas
+// a, b in [1/2, 1), s = (a - b)/(a + b) (numerator exact), |s| < 1/3.
 RH_HD double rh_log_ratio(double num, double den)
 {
     double a, b;
     int ea, eb;
     rh_frexp(num, a, ea);
     rh_frexp(den, b, eb);
-    const bool hi = a > kSqrt2 * b, lo = b > kSqrt2 * a;
-    b = hi ? 2.0 * b : b;
-    a = lo ? 2.0 * a : a;
-    const double k = double(ea - eb + (hi ? 1 : 0) - (lo ? 1 : 0));
+    const double k = double(ea - eb);
     const double s = (a - b) * rh_rcp(a + b);
     const double v = s * s;
 #ifdef __HIP_DEVICE_COMPILE__
@@ -238,9 +237,9 @@ RH_HD double rh_log_ratio(double num, double den)
 #else
     const double* c = kLogP;
 #endif
-    double p = c[6];
+    double p = c[9];
 #pragma unroll
-    for (int i = 5; i >= 0; i--) p = fma(p, v, c[i]);
+    for (int i = 8; i >= 0; i--) p = fma(p, v, c[i]);
     const double r = fma(s * v, p, 2.0 * s);
     return fma(k, kLn2Hi, fma(k, kLn2Lo, r));
 }
