@@ -1260,7 +1260,13 @@ rh_status rh_relax_tab(rh_matrix* q, const double* mats, const double* tab, int6
                        const double* hext, double* M_inout, double* H_out, const rh_relax_opts* opts,
                        rh_relax_result* res)
 {
-    if (!q || !mats || !hext || !M_inout || !H_out) return fail(RH_ERR_ARG, "null argument");
+    if (!q || !opts || !res) return fail(RH_ERR_ARG, "null argument");
+    if (q->m->host.n == 0) {
+        std::memset(res, 0, sizeof(*res));
+        res->converged = 1;
+        return RH_OK;
+    }
+    if (!mats || !hext || !M_inout || !H_out) return fail(RH_ERR_ARG, "null argument");
     if (ntab < 0 || (ntab > 0 && !tab)) return fail(RH_ERR_ARG, "bad table");
     rh_model* m = q->m;
     RH_HIP(enter_device(m->device));
@@ -1334,8 +1340,9 @@ rh_status rh_field(rh_model* m, const double* M, const double* pts, int64_t np, 
 {
     if (prec != RH_FP64 && prec != RH_MIXED) return fail(RH_ERR_ARG, "unknown precision");
     if (!m || !pts || np < 0 || (!B_out && !H_out)) return fail(RH_ERR_ARG, "bad argument");
-    RH_HIP(enter_device(m->device));
     const int64_t n = m->host.n;
+    if (n > 0 && !M) return fail(RH_ERR_ARG, "null magnetization argument");
+    RH_HIP(enter_device(m->device));
     if (np == 0) return RH_OK;
     double *dM = nullptr, *dP = nullptr, *dB = nullptr, *dH = nullptr;
     auto cleanup = [&]() {
