@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void k_adj_scale(int64_t n, const double* __re
 // Block-Jacobi update of the transposed system (I - Q^T K^T) lam = rhs:
 // lam_new_i = (I - Qii^T K_i^T)^{-1} (rhs_i + y_i), y = offdiag(Q)^T K^T lam;
 // F = lam_new - lam, per-block [sum |F|^2, max |lam_new|, 0].
-__global__ __launch_bounds__(256) void k_adj_update(int64_t n, const double* __restrict__ mats,
+__global__ __launch_bounds__(kRelaxLanes) void k_adj_update(int64_t n, const double* __restrict__ mats,
                                                     const double* __restrict__ rhs,
                                                     const double* __restrict__ qdiag,
                                                     const double* __restrict__ y,
@@ -251,8 +251,8 @@ __global__ __launch_bounds__(256) void k_adj_update(int64_t n, const double* __r
                                                     double* __restrict__ F,
                                                     double* __restrict__ partials)
 {
-    __shared__ double red[2][256];
-    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    __shared__ double red[2][kRelaxLanes];
+    const int64_t i = int64_t(blockIdx.x) * kRelaxLanes + threadIdx.x;
     double f2 = 0.0, mm = 0.0;
     if (i < n) {
         double K[9];
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(256) void k_adj_update(int64_t n, const double* __r
     red[0][threadIdx.x] = f2;
     red[1][threadIdx.x] = mm;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = kRelaxLanes / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             red[0][threadIdx.x] += red[0][threadIdx.x + s];
             red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
@@ -409,7 +409,7 @@ hipError_t launch_adj_update(int64_t n, const double* mats, const double* rhs, c
                              const double* y, const double* lam, double* F, double* partials,
                              hipStream_t s)
 {
-    hipLaunchKernelGGL(k_adj_update, dim3(unsigned(relax_blocks(n))), dim3(256), 0, s, n, mats, rhs,
+    hipLaunchKernelGGL(k_adj_update, dim3(unsigned(relax_blocks(n))), dim3(kRelaxLanes), 0, s, n, mats, rhs,
                        qdiag, y, lam, F, partials);
     return hipGetLastError();
 }

@@ -1186,6 +1186,9 @@ rh_status rh_relax_tab_dev(rh_matrix* q, const double* d_mats, const double* d_t
     RH_HIP(dalloc(&w.part, size_t(3 * nb)));
     RH_HIP(dalloc(&w.dots, size_t(17 * db)));
     RH_HIP(hipMemsetAsync(w.x, 0, size_t(q->ld) * sizeof(double), s));
+    // d_H starts the per-element nonlinear solves (previous local field); a
+    // zero start keeps repeated solves bit-identical
+    RH_HIP(hipMemsetAsync(d_H, 0, size_t(n3) * sizeof(double), s));
     if (opts->keep_m) {
         RH_HIP(hipMemcpyAsync(w.x, d_M, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
     } else {

@@ -618,161 +618,6 @@ __device__ __forceinline__ double law_chi(const double* mat, const double* tab, 
     return frm_chi(mat, h);
 }
 
-__device__ void local_solve(const double* mat, const double* q, const double* heff, double* Mp,
-                            double* Hl, const double* tab)
-{
-    const int kind = int(mat[0]);
-    const double* mr = mat + 13;
-    double qmr[3];
-    mat3_vec(q, mr, qmr);
-    const double b[3] = {heff[0] + qmr[0], heff[1] + qmr[1], heff[2] + qmr[2]};
-    if (kind == 0) {
-        double A[9];
-        for (int k = 0; k < 9; k++) A[k] = mat[1] * q[k];
-        solve_IminusA(A, b, Hl);
-        for (int k = 0; k < 3; k++) Mp[k] = mat[1] * Hl[k] + mr[k];
-        return;
-    }
-    if (kind == 1) {
-        const double* K = mat + 4;
-        double A[9];
-        for (int a = 0; a < 3; a++)
-            for (int c = 0; c < 3; c++)
-                A[3 * a + c] = q[3 * a] * K[c] + q[3 * a + 1] * K[3 + c] + q[3 * a + 2] * K[6 + c];
-        solve_IminusA(A, b, Hl);
-        double kh[3];
-        mat3_vec(K, Hl, kh);
-        for (int k = 0; k < 3; k++) Mp[k] = kh[k] + mr[k];
-        return;
-    }
-    // nonlinear isotropic: find h = |H(h)|, H(h) = (I - chi_s(h) Q)^{-1} b,
-    // bracketed in [0, |b|] (Illinois false position).
-    const double bn = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
-    if (!(bn > 0.0)) {
-        Hl[0] = Hl[1] = Hl[2] = 0.0;
-        for (int k = 0; k < 3; k++) Mp[k] = mr[k];
-        return;
-    }
-    auto phi = [&](double h, double* Hv) -> double {
-        const double chi = law_chi(mat, tab, h);
-        double A[9];
-        for (int k = 0; k < 9; k++) A[k] = chi * q[k];
-        solve_IminusA(A, b, Hv);
-        return sqrt(Hv[0] * Hv[0] + Hv[1] * Hv[1] + Hv[2] * Hv[2]);
-    };
-    double Hv[3];
-    double lo = 0.0, glo = phi(0.0, Hv) - 0.0;
-    double hi = bn, ghi = phi(hi, Hv) - hi;
-    for (int k = 0; k < 60 && ghi > 0.0; k++) {
-        lo = hi;
-        glo = ghi;
-        hi *= 2.0;
-        ghi = phi(hi, Hv) - hi;
-    }
-    double h = hi;
-    int side = 0;
-    for (int it = 0; it < 100; it++) {
-        if (!(glo > 0.0) || !(ghi < 0.0)) {
-            h = (glo <= 0.0) ? lo : hi;
-            break;
-        }
-        h = (lo * ghi - hi * glo) / (ghi - glo);
-        const double g = phi(h, Hv) - h;
-        if (fabs(hi - lo) <= 1e-15 * bn || g == 0.0) break;
-        if (g > 0.0) {
-            lo = h;
-            glo = g;
-            if (side == 1) ghi *= 0.5;
-            side = 1;
-        } else {
-            hi = h;
-            ghi = g;
-            if (side == -1) glo *= 0.5;
-            side = -1;
-        }
-    }
-    phi(h, Hv);
-    const double hn = sqrt(Hv[0] * Hv[0] + Hv[1] * Hv[1] + Hv[2] * Hv[2]);
-    const double chi = law_chi(mat, tab, hn);
-    for (int k = 0; k < 3; k++) {
-        Hl[k] = Hv[k];
-        Mp[k] = chi * Hv[k] + mr[k];
-    }
-}
-
-// FP64 defect of the stored-matrix field: out = hext + rowT*H64 - y - Qii M,
-// with H64 the FP64 field of all elements at the collocation points (lab
-// frame) and y the stored-matrix product (diagonal blocks excluded).
-__global__ __launch_bounds__(256) void k_refine_defect(int64_t n, const double* __restrict__ H64,
-                                                       const double* __restrict__ rowt,
-                                                       const double* __restrict__ y,
-                                                       const double* __restrict__ qdiag,
-                                                       const double* __restrict__ M,
-                                                       const double* __restrict__ hext,
-                                                       double* __restrict__ out)
-{
-    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= n) return;
-    double h[3] = {H64[3 * i], H64[3 * i + 1], H64[3 * i + 2]};
-    if (rowt) {
-        const double* t = rowt + 9 * i;
-        const double g[3] = {h[0], h[1], h[2]};
-        for (int k = 0; k < 3; k++) h[k] = t[3 * k] * g[0] + t[3 * k + 1] * g[1] + t[3 * k + 2] * g[2];
-    }
-    const double* D = qdiag + 9 * i;
-    const double m[3] = {M[3 * i], M[3 * i + 1], M[3 * i + 2]};
-    for (int k = 0; k < 3; k++)
-        out[3 * i + k] = hext[3 * i + k] + (h[k] - y[3 * i + k]) -
-                         (D[3 * k] * m[0] + D[3 * k + 1] * m[1] + D[3 * k + 2] * m[2]);
-}
-
-__global__ __launch_bounds__(256) void k_relax_update(int64_t n, const double* __restrict__ mats,
-                                                      const double* __restrict__ tab,
-                                                      const double* __restrict__ hext,
-                                                      const double* __restrict__ qdiag,
-                                                      const double* __restrict__ y,
-                                                      const double* __restrict__ M,
-                                                      double* __restrict__ F,
-                                                      double* __restrict__ Hout,
-                                                      double* __restrict__ partials)
-{
-    __shared__ double red[3][256];
-    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    double f2 = 0.0, mm = 0.0, hh = 0.0;
-    if (i < n) {
-        const double heff[3] = {y[3 * i] + hext[3 * i], y[3 * i + 1] + hext[3 * i + 1],
-                                y[3 * i + 2] + hext[3 * i + 2]};
-        double mp[3], hl[3];
-        local_solve(mats + 16 * i, qdiag + 9 * i, heff, mp, hl, tab);
-        for (int k = 0; k < 3; k++) {
-            const double d = mp[k] - M[3 * i + k];
-            F[3 * i + k] = d;
-            Hout[3 * i + k] = hl[k];
-            f2 += d * d;
-        }
-        mm = sqrt(mp[0] * mp[0] + mp[1] * mp[1] + mp[2] * mp[2]);
-        hh = sqrt(hl[0] * hl[0] + hl[1] * hl[1] + hl[2] * hl[2]);
-    }
-    red[0][threadIdx.x] = f2;
-    red[1][threadIdx.x] = mm;
-    red[2][threadIdx.x] = hh;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + s];
-            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + s]);
-            red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][threadIdx.x + s]);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        partials[3 * blockIdx.x] = red[0][0];
-        partials[3 * blockIdx.x + 1] = red[1][0];
-        partials[3 * blockIdx.x + 2] = red[2][0];
-    }
-}
-
-// ---------------------------------------------------------------- Newton-Krylov
 // |M|(h) and d|M|/dh of an isotropic nonlinear law (kind 2 tanh sum, kind 3
 // table), h > 0.
 __device__ __forceinline__ void law_f_df(const double* mat, const double* tab, double h, double& f,
@@ -819,12 +664,202 @@ __device__ __forceinline__ void law_f_df(const double* mat, const double* tab, d
     }
 }
 
+__device__ void local_solve(const double* mat, const double* q, const double* heff, double* Mp,
+                            double* Hl, const double* tab, double warm)
+{
+    const int kind = int(mat[0]);
+    const double* mr = mat + 13;
+    double qmr[3];
+    mat3_vec(q, mr, qmr);
+    const double b[3] = {heff[0] + qmr[0], heff[1] + qmr[1], heff[2] + qmr[2]};
+    if (kind == 0) {
+        double A[9];
+        for (int k = 0; k < 9; k++) A[k] = mat[1] * q[k];
+        solve_IminusA(A, b, Hl);
+        for (int k = 0; k < 3; k++) Mp[k] = mat[1] * Hl[k] + mr[k];
+        return;
+    }
+    if (kind == 1) {
+        const double* K = mat + 4;
+        double A[9];
+        for (int a = 0; a < 3; a++)
+            for (int c = 0; c < 3; c++)
+                A[3 * a + c] = q[3 * a] * K[c] + q[3 * a + 1] * K[3 + c] + q[3 * a + 2] * K[6 + c];
+        solve_IminusA(A, b, Hl);
+        double kh[3];
+        mat3_vec(K, Hl, kh);
+        for (int k = 0; k < 3; k++) Mp[k] = kh[k] + mr[k];
+        return;
+    }
+    // nonlinear isotropic: find h = |H(h)|, H(h) = (I - chi(h) Q)^{-1} b with
+    // the secant susceptibility chi(h) = |M|(h)/h.  Safeguarded Newton on
+    // g(h) = |H(h)| - h inside a bracket g(lo) > 0 > g(hi), started from the
+    // previous iteration's |H| (warm) when it lies in the bracket; where a
+    // Newton step leaves the bracket, false position, and bisection after two
+    // false-position steps in a row.
+    const double bn = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    if (!(bn > 0.0)) {
+        Hl[0] = Hl[1] = Hl[2] = 0.0;
+        for (int k = 0; k < 3; k++) Mp[k] = mr[k];
+        return;
+    }
+    const double chi0 = law_chi(mat, tab, 0.0);
+    // |H(h)| into the return value, H(h) into Hv, dg/dh into *dg (if given)
+    auto phi = [&](double h, double* Hv, double* dg) -> double {
+        double chi = chi0, dchi = 0.0;
+        if (h > 0.0) {
+            double f, df;
+            law_f_df(mat, tab, h, f, df);
+            chi = f / h;
+            dchi = (df - chi) / h;
+        }
+        double A[9];
+        for (int k = 0; k < 9; k++) A[k] = -chi * q[k];
+        A[0] += 1.0;
+        A[4] += 1.0;
+        A[8] += 1.0;
+        double Ai[9];
+        inv3d(A, Ai);
+        mat3_vec(Ai, b, Hv);
+        const double hv = sqrt(Hv[0] * Hv[0] + Hv[1] * Hv[1] + Hv[2] * Hv[2]);
+        if (dg) {
+            double qh[3], dH[3];
+            mat3_vec(q, Hv, qh);
+            mat3_vec(Ai, qh, dH);
+            *dg = (hv > 0.0 ? dchi * (Hv[0] * dH[0] + Hv[1] * dH[1] + Hv[2] * dH[2]) / hv : 0.0) - 1.0;
+        }
+        return hv;
+    };
+    double Hv[3];
+    double lo = 0.0, glo = phi(0.0, Hv, nullptr);
+    double hi = bn, ghi = phi(hi, Hv, nullptr) - hi;
+    for (int k = 0; k < 60 && ghi > 0.0; k++) {
+        lo = hi;
+        glo = ghi;
+        hi *= 2.0;
+        ghi = phi(hi, Hv, nullptr) - hi;
+    }
+    double h;
+    if (!(glo > 0.0) || !(ghi < 0.0)) {
+        h = (glo <= 0.0) ? lo : hi;
+    } else {
+        h = (warm > lo && warm < hi) ? warm : (lo * ghi - hi * glo) / (ghi - glo);
+        int fp = 0;
+        for (int it = 0; it < 60; it++) {
+            double dg;
+            const double g = phi(h, Hv, &dg) - h;
+            if (g == 0.0) break;
+            if (g > 0.0) {
+                lo = h;
+                glo = g;
+            } else {
+                hi = h;
+                ghi = g;
+            }
+            double hn = (dg < 0.0) ? h - g / dg : -1.0;
+            if (hn > lo && hn < hi) {
+                fp = 0;
+            } else {
+                hn = (fp < 2) ? (lo * ghi - hi * glo) / (ghi - glo) : 0.5 * (lo + hi);
+                fp = (fp < 2) ? fp + 1 : 0;
+                if (!(hn > lo && hn < hi)) hn = 0.5 * (lo + hi);
+            }
+            const bool done = fabs(hn - h) <= 1e-15 * bn || hi - lo <= 1e-15 * bn;
+            h = hn;
+            if (done) break;
+        }
+    }
+    phi(h, Hv, nullptr);
+    const double hn = sqrt(Hv[0] * Hv[0] + Hv[1] * Hv[1] + Hv[2] * Hv[2]);
+    const double chi = law_chi(mat, tab, hn);
+    for (int k = 0; k < 3; k++) {
+        Hl[k] = Hv[k];
+        Mp[k] = chi * Hv[k] + mr[k];
+    }
+}
+
+// FP64 defect of the stored-matrix field: out = hext + rowT*H64 - y - Qii M,
+// with H64 the FP64 field of all elements at the collocation points (lab
+// frame) and y the stored-matrix product (diagonal blocks excluded).
+__global__ __launch_bounds__(256) void k_refine_defect(int64_t n, const double* __restrict__ H64,
+                                                       const double* __restrict__ rowt,
+                                                       const double* __restrict__ y,
+                                                       const double* __restrict__ qdiag,
+                                                       const double* __restrict__ M,
+                                                       const double* __restrict__ hext,
+                                                       double* __restrict__ out)
+{
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    double h[3] = {H64[3 * i], H64[3 * i + 1], H64[3 * i + 2]};
+    if (rowt) {
+        const double* t = rowt + 9 * i;
+        const double g[3] = {h[0], h[1], h[2]};
+        for (int k = 0; k < 3; k++) h[k] = t[3 * k] * g[0] + t[3 * k + 1] * g[1] + t[3 * k + 2] * g[2];
+    }
+    const double* D = qdiag + 9 * i;
+    const double m[3] = {M[3 * i], M[3 * i + 1], M[3 * i + 2]};
+    for (int k = 0; k < 3; k++)
+        out[3 * i + k] = hext[3 * i + k] + (h[k] - y[3 * i + k]) -
+                         (D[3 * k] * m[0] + D[3 * k + 1] * m[1] + D[3 * k + 2] * m[2]);
+}
+
+__global__ __launch_bounds__(kRelaxLanes) void k_relax_update(int64_t n, const double* __restrict__ mats,
+                                                              const double* __restrict__ tab,
+                                                              const double* __restrict__ hext,
+                                                              const double* __restrict__ qdiag,
+                                                              const double* __restrict__ y,
+                                                              const double* __restrict__ M,
+                                                              double* __restrict__ F,
+                                                              double* __restrict__ Hout,
+                                                              double* __restrict__ partials)
+{
+    __shared__ double red[3][kRelaxLanes];
+    const int64_t i = int64_t(blockIdx.x) * kRelaxLanes + threadIdx.x;
+    double f2 = 0.0, mm = 0.0, hh = 0.0;
+    if (i < n) {
+        const double heff[3] = {y[3 * i] + hext[3 * i], y[3 * i + 1] + hext[3 * i + 1],
+                                y[3 * i + 2] + hext[3 * i + 2]};
+        // the previous local field's magnitude starts the nonlinear solve
+        const double warm = sqrt(Hout[3 * i] * Hout[3 * i] + Hout[3 * i + 1] * Hout[3 * i + 1] +
+                                 Hout[3 * i + 2] * Hout[3 * i + 2]);
+        double mp[3], hl[3];
+        local_solve(mats + 16 * i, qdiag + 9 * i, heff, mp, hl, tab, warm);
+        for (int k = 0; k < 3; k++) {
+            const double d = mp[k] - M[3 * i + k];
+            F[3 * i + k] = d;
+            Hout[3 * i + k] = hl[k];
+            f2 += d * d;
+        }
+        mm = sqrt(mp[0] * mp[0] + mp[1] * mp[1] + mp[2] * mp[2]);
+        hh = sqrt(hl[0] * hl[0] + hl[1] * hl[1] + hl[2] * hl[2]);
+    }
+    red[0][threadIdx.x] = f2;
+    red[1][threadIdx.x] = mm;
+    red[2][threadIdx.x] = hh;
+    __syncthreads();
+    for (int st = kRelaxLanes / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + st];
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + st]);
+            red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x] = red[0][0];
+        partials[3 * blockIdx.x + 1] = red[1][0];
+        partials[3 * blockIdx.x + 2] = red[2][0];
+    }
+}
+
+// ---------------------------------------------------------------- Newton-Krylov
 // Material law at the total field of the current magnetization: H = y + Qii M
 // + hext, F = M_mat(H) - M, tangent D = dM_mat/dH (D = mp hh^T + sc (I - hh^T)
 // for isotropic laws, hh = H/|H|, sc = |M|/|H|, mp = d|M|/d|H|; chi0 I at
 // H = 0; chi I or K for linear laws) and the block preconditioner
 // P = (I - D Qii)^{-1}.  partials per block: [sum |F|^2, max |M_mat|, max |H|].
-__global__ __launch_bounds__(256) void k_nk_material(int64_t n, const double* __restrict__ mats,
+__global__ __launch_bounds__(kRelaxLanes) void k_nk_material(int64_t n, const double* __restrict__ mats,
                                                      const double* __restrict__ tab,
                                                      const double* __restrict__ hext,
                                                      const double* __restrict__ qdiag,
@@ -834,8 +869,8 @@ __global__ __launch_bounds__(256) void k_nk_material(int64_t n, const double* __
                                                      double* __restrict__ P, double* __restrict__ Hout,
                                                      double* __restrict__ partials)
 {
-    __shared__ double red[3][256];
-    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    __shared__ double red[3][kRelaxLanes];
+    const int64_t i = int64_t(blockIdx.x) * kRelaxLanes + threadIdx.x;
     double f2 = 0.0, mm = 0.0, hh = 0.0;
     if (i < n) {
         const double* mat = mats + 16 * i;
@@ -895,7 +930,7 @@ __global__ __launch_bounds__(256) void k_nk_material(int64_t n, const double* __
     red[1][threadIdx.x] = mm;
     red[2][threadIdx.x] = hh;
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
+    for (int st = kRelaxLanes / 2; st > 0; st >>= 1) {
         if (threadIdx.x < st) {
             red[0][threadIdx.x] += red[0][threadIdx.x + st];
             red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + st]);
@@ -1185,13 +1220,13 @@ hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t l
     return hipGetLastError();
 }
 
-int64_t relax_blocks(int64_t n) { return (n + 255) / 256; }
+int64_t relax_blocks(int64_t n) { return (n + kRelaxLanes - 1) / kRelaxLanes; }
 
 hipError_t launch_relax_update(int64_t n, const double* mats, const double* tab, const double* hext,
                                const double* qdiag, const double* y, const double* M, double* F,
                                double* Hout, double* partials, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_relax_update, dim3(unsigned(relax_blocks(n))), dim3(256), 0, s, n, mats,
+    hipLaunchKernelGGL(k_relax_update, dim3(unsigned(relax_blocks(n))), dim3(kRelaxLanes), 0, s, n, mats,
                        tab, hext, qdiag, y, M, F, Hout, partials);
     return hipGetLastError();
 }
@@ -1209,21 +1244,21 @@ hipError_t launch_nk_material(int64_t n, const double* mats, const double* tab, 
                               const double* qdiag, const double* y, const double* M, double* F, double* D,
                               double* P, double* Hout, double* partials, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_nk_material, dim3(unsigned(relax_blocks(n))), dim3(256), 0, s, n, mats, tab, hext,
+    hipLaunchKernelGGL(k_nk_material, dim3(unsigned(relax_blocks(n))), dim3(kRelaxLanes), 0, s, n, mats, tab, hext,
                        qdiag, y, M, F, D, P, Hout, partials);
     return hipGetLastError();
 }
 
 hipError_t launch_nk_prec(int64_t n, const double* P, const double* v, double* z, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_nk_prec, dim3(unsigned(relax_blocks(n))), dim3(256), 0, s, n, P, v, z);
+    hipLaunchKernelGGL(k_nk_prec, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, P, v, z);
     return hipGetLastError();
 }
 
 hipError_t launch_nk_apply(int64_t n, const double* D, const double* qdiag, const double* z, const double* y,
                            double* w, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_nk_apply, dim3(unsigned(relax_blocks(n))), dim3(256), 0, s, n, D, qdiag, z, y, w);
+    hipLaunchKernelGGL(k_nk_apply, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, D, qdiag, z, y, w);
     return hipGetLastError();
 }
 

@@ -73,6 +73,7 @@ hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t l
 // Per-element implicit update: proposal M_prop = M_mat(H) with
 // H = y + hext + Qii M_prop solved locally; F = M_prop - M.  partials gets
 // per-block [sum |F|^2, max |M_prop|, max |H|].
+constexpr int kRelaxLanes = 64;  // lanes per workgroup of the per-element relaxation kernels
 int64_t relax_blocks(int64_t n);
 // tab: node triples of tabulated laws (material kind 3), may be null otherwise.
 hipError_t launch_relax_update(int64_t n, const double* mats, const double* tab, const double* hext,
