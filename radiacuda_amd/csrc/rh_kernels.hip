@@ -150,14 +150,14 @@ __global__ __launch_bounds__(256) void k_field(DevModel m, const double* __restr
 // conflicts) while it evaluates its own point.  Staging keeps the record out of
 // scalar registers, whose pressure otherwise spills through VGPR lanes.
 constexpr int kTetTile = 32;
-// occupancy floor for the tetrahedron sweep (waves per SIMD): bounds the VGPRs
-// the scheduler may use for hoisted LDS reads
+// occupancy floor for the tetrahedron and mesh sweeps (waves per SIMD): bounds
+// the VGPRs the scheduler may use for hoisted LDS reads
 #ifndef RH_TET_WAVES
+#define RH_TET_WAVES 3
+#endif
 // edge and face records staged per LDS tile of the mesh sweep
 #ifndef RH_MESH_STAGE
 #define RH_MESH_STAGE 128
-#endif
-#define RH_TET_WAVES 3
 #endif
 
 template <class R, bool SYM>
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH_TET_WAVE
 
 // ---------------------------------------------------------------- mesh form
 // Charged records of the mesh form for magnetization M (one lane per entity).
-__global__ __launch_bounds__(256) void k_mesh_charge_edges(MeshDev md, const double* __restrict__ M)
+__global__ __launch_bounds__(256) void k_mesh_charge_edges(MeshDev md, const double* __restrict__ M, bool pack32)
 {
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (i >= md.ne) return;
@@ -254,9 +254,17 @@ __global__ __launch_bounds__(256) void k_mesh_charge_edges(MeshDev md, const dou
     o[11] = w[1];
     o[12] = w[2];
     o[13] = md.edge_cont[i];
+    if (pack32) {
+        float* f = reinterpret_cast<float*>(o + 6);
+        const float t0 = float(g[6]), t1 = float(g[7]), t2 = float(g[8]), len = float(g[9]);
+        f[0] = t0;
+        f[1] = t1;
+        f[2] = t2;
+        f[3] = len;
+    }
 }
 
-__global__ __launch_bounds__(256) void k_mesh_charge_faces(MeshDev md, const double* __restrict__ M)
+__global__ __launch_bounds__(256) void k_mesh_charge_faces(MeshDev md, const double* __restrict__ M, bool pack32)
 {
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     if (i >= md.nf) return;
@@ -273,6 +281,16 @@ __global__ __launch_bounds__(256) void k_mesh_charge_faces(MeshDev md, const dou
     o[19] = s * n1;
     o[20] = s * n2;
     o[21] = md.face_cont[i];
+    if (pack32) {
+        float* f = reinterpret_cast<float*>(o + 12);
+        const float a1 = float(g[15]), a2 = float(g[16]);
+        f[0] = float(n0);
+        f[1] = float(n1);
+        f[2] = float(n2);
+        f[3] = a1;
+        f[4] = a2;
+        f[5] = 0.0f;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_mesh_charge_in(MeshDev md, const double* __restrict__ M)
@@ -322,10 +340,18 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
         auto edge = [&](int64_t g, const double* rec, MeshVtx<R>& a, MeshVtx<R>& b) {
             if (!(mesh_code(rec + 13) == 1 && g != e0)) a = mesh_vtx<R>(rec, x);
             b = mesh_vtx<R>(rec + 3, x);
-            const double L = mesh_edge_L_from<R>(a.r, a.d, b.r, b.d, rec);
-            h[0] += L * rec[10];
-            h[1] += L * rec[11];
-            h[2] += L * rec[12];
+            if constexpr (sizeof(R) == 4) {
+                const float* f = mesh32(rec, 6);
+                const double L = double(edge_log1p(a.r, a.d, b.r, b.d, v3(f[0], f[1], f[2]), f[3]));
+                h[0] += L * rec[10];
+                h[1] += L * rec[11];
+                h[2] += L * rec[12];
+            } else {
+                const double L = mesh_edge_L_from<R>(a.r, a.d, b.r, b.d, rec);
+                h[0] += L * rec[10];
+                h[1] += L * rec[11];
+                h[2] += L * rec[12];
+            }
         };
         for (int64_t base = e0; base < e1; base += RH_MESH_STAGE) {
             const int cnt = int((e1 - base < RH_MESH_STAGE) ? (e1 - base) : RH_MESH_STAGE);
@@ -356,22 +382,42 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
         auto face = [&](int64_t g, const double* rec, MeshVtx<R>& a, MeshVtx<R>& b, MeshVtx<R>& c, R& dab,
                         R& dbc) {
             const int code = mesh_code(rec + 21);
-            double om;
-            if (code == 2) {
-                om = mesh_face_omega<R>(rec, x);
-            } else {
-                if (!(code == 1 && g != f0)) {
-                    a = mesh_vtx<R>(rec, x);
-                    b = mesh_vtx<R>(rec + 3, x);
-                    dab = dot(a.r, b.r);
+            if constexpr (sizeof(R) == 4) {
+                float om;
+                if (code == 2) {
+                    om = mesh32_quad_omega(rec, x);
+                } else {
+                    if (!(code == 1 && g != f0)) {
+                        a = mesh_vtx<R>(rec, x);
+                        b = mesh_vtx<R>(rec + 3, x);
+                        dab = dot(a.r, b.r);
+                    }
+                    c = mesh_vtx<R>(rec + 6, x);
+                    dbc = dot(b.r, c.r);
+                    om = mesh32_tri_omega(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, rec);
                 }
-                c = mesh_vtx<R>(rec + 6, x);
-                dbc = dot(b.r, c.r);
-                om = mesh_tri_omega_from<R>(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, rec);
+                const double omd = double(om);
+                h[0] -= omd * rec[18];
+                h[1] -= omd * rec[19];
+                h[2] -= omd * rec[20];
+            } else {
+                double om;
+                if (code == 2) {
+                    om = mesh_face_omega<R>(rec, x);
+                } else {
+                    if (!(code == 1 && g != f0)) {
+                        a = mesh_vtx<R>(rec, x);
+                        b = mesh_vtx<R>(rec + 3, x);
+                        dab = dot(a.r, b.r);
+                    }
+                    c = mesh_vtx<R>(rec + 6, x);
+                    dbc = dot(b.r, c.r);
+                    om = mesh_tri_omega_from<R>(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, rec);
+                }
+                h[0] -= om * rec[18];
+                h[1] -= om * rec[19];
+                h[2] -= om * rec[20];
             }
-            h[0] -= om * rec[18];
-            h[1] -= om * rec[19];
-            h[2] -= om * rec[20];
         };
         for (int64_t base = f0; base < f1; base += RH_MESH_STAGE) {
             const int cnt = int((f1 - base < RH_MESH_STAGE) ? (f1 - base) : RH_MESH_STAGE);
@@ -1186,10 +1232,10 @@ hipError_t launch_field(const DevModel& m0, const double* M, const double* pts, 
         const MeshDev& md = m.mesh;
         if (md.ne)
             hipLaunchKernelGGL(k_mesh_charge_edges, dim3(unsigned((md.ne + 255) / 256)), dim3(256), 0, s,
-                               md, M);
+                               md, M, fp32);
         if (md.nf)
             hipLaunchKernelGGL(k_mesh_charge_faces, dim3(unsigned((md.nf + 255) / 256)), dim3(256), 0, s,
-                               md, M);
+                               md, M, fp32);
         if (md.ni)
             hipLaunchKernelGGL(k_mesh_charge_in, dim3(unsigned((md.ni + 255) / 256)), dim3(256), 0, s, md,
                                M);

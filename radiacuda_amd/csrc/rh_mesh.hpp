@@ -119,6 +119,39 @@ RH_HD double mesh_tri_omega_from(const V3<R>& r0, R R0, const V3<R>& r1, R R1, c
     return 2.0 * double(rh_atan2(im, re));
 }
 
+// FP32 sweep records: the geometry the lanes use in FP32 is stored as floats
+// when the records are charged (edge slots 6-7: t(3), len; face slots 12-14:
+// n(3), the two doubled areas), so the sweep converts none of it per record;
+// vertices (offsets are formed in FP64) and the weights w and g (the sums are
+// accumulated in FP64) stay FP64.
+RH_HD const float* mesh32(const double* __restrict__ rec, int slot)
+{
+    return reinterpret_cast<const float*>(rec + slot);
+}
+
+RH_HD float mesh32_tri_omega(const V3<float>& r0, float R0, const V3<float>& r1, float R1, const V3<float>& r2,
+                             float R2, float d01, float d12, const double* __restrict__ rec)
+{
+    const float* f = mesh32(rec, 12);
+    const float im = f[3] * dot(r0, v3(f[0], f[1], f[2]));
+    const float re = R0 * R1 * R2 + d01 * R2 + dot(r0, r2) * R1 + d12 * R0;
+    return 2.0f * rh_atan2(im, re);
+}
+
+RH_HD float mesh32_quad_omega(const double* __restrict__ rec, const double p[3])
+{
+    const V3<float> r0 = rel<float>(rec, p), r1 = rel<float>(rec + 3, p), r2 = rel<float>(rec + 6, p),
+                    r3 = rel<float>(rec + 9, p);
+    const float R0 = rh_sqrt(dot(r0, r0)), R1 = rh_sqrt(dot(r1, r1)), R2 = rh_sqrt(dot(r2, r2)),
+                R3 = rh_sqrt(dot(r3, r3));
+    const float* f = mesh32(rec, 12);
+    const float h = dot(r0, v3(f[0], f[1], f[2]));
+    const float im = f[3] * h, im2 = f[4] * h;
+    const float re = R0 * R1 * R2 + dot(r0, r1) * R2 + dot(r0, r2) * R1 + dot(r1, r2) * R0;
+    const float re2 = R0 * R2 * R3 + dot(r0, r2) * R3 + dot(r0, r3) * R2 + dot(r2, r3) * R0;
+    return 2.0f * rh_atan2(re * im2 + im * re2, re * re2 - im * im2);
+}
+
 // Plane test of one inside record.
 RH_HD bool mesh_inside(const double* __restrict__ rec, const double p[3])
 {
