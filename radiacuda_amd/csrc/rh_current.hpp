@@ -11,7 +11,9 @@
 //
 // Arc current: trapezoid rule over nseg sectors of the azimuthal integral,
 // the numerical scheme of radTArcCur::B_compWithTrapeth
-// (cpp/src/core/radarccu.cpp:30-234), written here for the device.
+// (cpp/src/core/radarccu.cpp:30-234), written here for the device: the
+// cosines and sines of the sector angles advance by rotation, and the logs
+// and arctangents use the short forms of rh_math.hpp.
 #pragma once
 
 #include "rh_math.hpp"
@@ -27,7 +29,7 @@ RH_HD void block_current_B(const double* par, const double p[3], double B[3])
     double Rc[2][2][2];
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++)
-            for (int k = 0; k < 2; k++) Rc[i][j][k] = sqrt(X[i] * X[i] + Y[j] * Y[j] + Z[k] * Z[k]);
+            for (int k = 0; k < 2; k++) Rc[i][j][k] = rh_sqrt(X[i] * X[i] + Y[j] * Y[j] + Z[k] * Z[k]);
     // edge logs
     double Lx[2][2], Ly[2][2], Lz[2][2];
     for (int j = 0; j < 2; j++)
@@ -35,21 +37,21 @@ RH_HD void block_current_B(const double* par, const double p[3], double B[3])
             double num, den;
             edge_ratio(v3(X[0], Y[j], Z[k]), Rc[0][j][k], v3(X[1], Y[j], Z[k]), Rc[1][j][k],
                        v3(1.0, 0.0, 0.0), num, den);
-            Lx[j][k] = log(num / den);
+            Lx[j][k] = rh_log_ratio(num, den);
         }
     for (int i = 0; i < 2; i++)
         for (int k = 0; k < 2; k++) {
             double num, den;
             edge_ratio(v3(X[i], Y[0], Z[k]), Rc[i][0][k], v3(X[i], Y[1], Z[k]), Rc[i][1][k],
                        v3(0.0, 1.0, 0.0), num, den);
-            Ly[i][k] = log(num / den);
+            Ly[i][k] = rh_log_ratio(num, den);
         }
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++) {
             double num, den;
             edge_ratio(v3(X[i], Y[j], Z[0]), Rc[i][j][0], v3(X[i], Y[j], Z[1]), Rc[i][j][1],
                        v3(0.0, 0.0, 1.0), num, den);
-            Lz[i][j] = log(num / den);
+            Lz[i][j] = rh_log_ratio(num, den);
         }
     // solid angle of a rectangle face at index `side` of axis a (outward normal
     // s*e_a, s = 2*side-1), corners counter-clockwise about the normal
@@ -79,7 +81,7 @@ RH_HD void block_current_B(const double* par, const double p[3], double B[3])
         double re1, im1, re2, im2;
         tri_factor(r[0], rr[0], r[1], rr[1], r[2], rr[2], re1, im1);
         tri_factor(r[0], rr[0], r[2], rr[2], r[3], rr[3], re2, im2);
-        return 2.0 * atan2(re1 * im2 + im1 * re2, re1 * re2 - im1 * im2);
+        return 2.0 * rh_atan2(re1 * im2 + im1 * re2, re1 * re2 - im1 * im2);
     };
     // face potentials; for face (a, side) the edges run along the other two axes
     double G[3] = {0, 0, 0};
@@ -118,7 +120,7 @@ RH_HD double atan_pair(double x, double y)
     double d = 1.0 - x * y;
     if (d == 0.0) d = 1e-50;
     double k = (d > 0) ? 0.0 : ((x < 0) ? -1.0 : 1.0);
-    return atan((x + y) / d) + 3.141592653589793238 * k;
+    return rh_atan2((x + y) / d, 1.0) + 3.141592653589793238 * k;
 }
 
 RH_HD void arc_current_B(const double* par, const double P[3], double B[3])
@@ -140,10 +142,14 @@ RH_HD void arc_current_B(const double* par, const double P[3], double B[3])
     const double tr1 = 2.0 * r * rmin, tr2 = 2.0 * r * rmax;
     const int npt = nseg + 1;
     const double step = (phimax - phimin + small) / (npt - 1);
-    double dphi = phimin - phi + small;
+    const double dphi0 = phimin - phi + small;
+    // cos and sin of dphi = dphi0 + i step and of dphi + phi by rotation
+    const double cs = cos(step), ss = sin(step);
+    double cd = cos(dphi0), sd = sin(dphi0);
+    double cp = cos(dphi0 + phi), sp = sin(dphi0 + phi);
     double sx = 0, sy = 0, sz = 0;
     for (int i = 0; i < npt; i++) {
-        const double cd = cos(dphi), as = fabs(sin(dphi));
+        const double as = fabs(sd);
         double b11 = fabs(a11 - tr1 * cd);
         if (b11 < zero_tol) b11 = zero_tol;
         double b21 = fabs(a21 - tr2 * cd);
@@ -158,7 +164,7 @@ RH_HD void arc_current_B(const double* par, const double P[3], double B[3])
         if (fabs(q12) < D12 * rel && q < u1 * u1 * rel) q12 = 0.5 * q / fabs(u1);
         if (fabs(q21) < D21 * rel && q < u2 * u2 * rel) q21 = 0.5 * q / fabs(u2);
         if (fabs(q22) < D22 * rel && q < u2 * u2 * rel) q22 = 0.5 * q / fabs(u2);
-        const double l1 = log(q11 / q21), l2 = log(q22 / q12);
+        const double l1 = rh_log_ratio(q11, q21), l2 = rh_log_ratio(q22, q12);
         const double ras = r * as;
         const double w1 = fabs(u1) / ras, w2 = fabs(u2) / ras;
         const double s1 = (u1 < 0) ? -1.0 : 1.0, s2 = (u2 < 0) ? -1.0 : 1.0;
@@ -171,15 +177,18 @@ RH_HD void arc_current_B(const double* par, const double P[3], double B[3])
         if (fabs(z1p21) < D21 * rel && e2 < z1z1 * rel) z1p21 = 0.5 * e2 / fabs(z1);
         if (fabs(z2p12) < D12 * rel && e1 < z2z2 * rel) z2p12 = 0.5 * e1 / fabs(z2);
         if (fabs(z2p22) < D22 * rel && e2 < z2z2 * rel) z2p22 = 0.5 * e2 / fabs(z2);
-        const double php = dphi + phi;
-        const double cp = cos(php), sp = sin(php);
         const double fxy = D11 - D21 + D22 - D12 + rc * (l1 + l2);
-        const double fz = buf - rc * log((z1p11 / z2p12) * (z2p22 / z1p21));
+        const double fz = buf - rc * rh_log_ratio(z1p11 * z2p22, z2p12 * z1p21);
         const double w = (i == 0 || i == npt - 1) ? 0.5 : 1.0;
         sx += w * cp * fxy;
         sy += w * sp * fxy;
         sz += w * fz;
-        dphi += step;
+        const double cd1 = cd * cs - sd * ss, sd1 = sd * cs + cd * ss;
+        const double cp1 = cp * cs - sp * ss, sp1 = sp * cs + cp * ss;
+        cd = cd1;
+        sd = sd1;
+        cp = cp1;
+        sp = sp1;
     }
     const double f = 1e-4 * jaz * step;
     B[0] = f * sx;
