@@ -268,6 +268,42 @@ RH_HD double rh_atan2(double y, double x)
     return copysign(a, y);
 }
 
+// atan2 for sweeps in which most angles are small: when every active lane of
+// the wave has x > 0 and |y| < tan(pi/8) x, the octant reduction of rh_atan2
+// is the identity for every lane, and the wave evaluates only the polynomial
+// on t = y/x.  Those are the operations rh_atan2 performs for such a lane (the
+// polynomial is odd, so the sign of y passes through unchanged), so both
+// branches give identical results; any other wave takes rh_atan2.
+RH_HD double rh_atan2_wave(double y, double x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    if (__all(fabs(y) < kTanPi8 * x)) {
+        const double t = y * rh_rcp(x);
+        const double u = t * t;
+        const double* c = d_atan_p;
+        double p = c[10];
+#pragma unroll
+        for (int i = 9; i >= 0; i--) p = fma(p, u, c[i]);
+        return fma(t * u, p, t);
+    }
+#endif
+    return rh_atan2(y, x);
+}
+RH_HD float rh_atan2_wave(float y, float x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    if (__all(fabsf(y) < 0.41421356f * x)) {
+        const float t = y * rh_rcp(x);
+        const float u = t * t;
+        const float p = -0.3333333134651184f +
+                        u * (0.19999539852142334f +
+                             u * (-0.14263956248760223f + u * (0.10743731260299683f + u * -0.06451927870512009f)));
+        return fmaf(t * u, p, t);
+    }
+#endif
+    return rh_atan2(y, x);
+}
+
 // Vertex relative to the observation point, formed in FP64 then rounded to R.
 template <class R>
 RH_HD V3<R> rel(const double* v, const double p[3])

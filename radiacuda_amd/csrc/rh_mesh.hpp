@@ -116,7 +116,7 @@ RH_HD double mesh_tri_omega_from(const V3<R>& r0, R R0, const V3<R>& r1, R R1, c
     const V3<R> n = v3(R(rec[12]), R(rec[13]), R(rec[14]));
     const R im = R(rec[15]) * dot(r0, n);
     const R re = R0 * R1 * R2 + d01 * R2 + dot(r0, r2) * R1 + d12 * R0;
-    return 2.0 * double(rh_atan2(im, re));
+    return 2.0 * double(rh_atan2_wave(im, re));
 }
 
 // FP32 sweep records: the geometry the lanes use in FP32 is stored as floats
@@ -135,7 +135,7 @@ RH_HD float mesh32_tri_omega(const V3<float>& r0, float R0, const V3<float>& r1,
     const float* f = mesh32(rec, 12);
     const float im = f[3] * dot(r0, v3(f[0], f[1], f[2]));
     const float re = R0 * R1 * R2 + d01 * R2 + dot(r0, r2) * R1 + d12 * R0;
-    return 2.0f * rh_atan2(im, re);
+    return 2.0f * rh_atan2_wave(im, re);
 }
 
 RH_HD float mesh32_quad_omega(const double* __restrict__ rec, const double p[3])
