@@ -340,18 +340,13 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
         auto edge = [&](int64_t g, const double* rec, MeshVtx<R>& a, MeshVtx<R>& b) {
             if (!(mesh_code(rec + 13) == 1 && g != e0)) a = mesh_vtx<R>(rec, x);
             b = mesh_vtx<R>(rec + 3, x);
-            if constexpr (sizeof(R) == 4) {
-                const float* f = mesh32(rec, 6);
-                const double L = double(edge_log1p(a.r, a.d, b.r, b.d, v3(f[0], f[1], f[2]), f[3]));
-                h[0] += L * rec[10];
-                h[1] += L * rec[11];
-                h[2] += L * rec[12];
-            } else {
-                const double L = mesh_edge_L_from<R>(a.r, a.d, b.r, b.d, rec);
-                h[0] += L * rec[10];
-                h[1] += L * rec[11];
-                h[2] += L * rec[12];
-            }
+            // L_e from the end offsets, distances and the length (the
+            // FP32 records carry the length as a float in slot 7)
+            const R len = (sizeof(R) == 4) ? R(mesh32(rec, 6)[3]) : R(rec[9]);
+            const double L = double(edge_L(a.r, a.d, b.r, b.d, len));
+            h[0] += L * rec[10];
+            h[1] += L * rec[11];
+            h[2] += L * rec[12];
         };
         for (int64_t base = e0; base < e1; base += RH_MESH_STAGE) {
             const int cnt = int((e1 - base < RH_MESH_STAGE) ? (e1 - base) : RH_MESH_STAGE);

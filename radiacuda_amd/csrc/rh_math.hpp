@@ -348,6 +348,63 @@ RH_HD R edge_log1p(const V3<R>& ra, R Ra, const V3<R>& rb, R Rb, const V3<R>& t,
     return rh_log1p(num * rh_rcp(den));
 }
 
+// An empty volatile asm statement: no instruction, but the compiler may not
+// execute the block it sits in speculatively, so a wave-uniform branch around
+// rarely needed work stays a branch instead of becoming selects.
+#ifdef __HIP_DEVICE_COMPILE__
+#define RH_KEEP_BRANCH() asm volatile("")
+#else
+#define RH_KEEP_BRANCH() ((void)0)
+#endif
+
+// The edge integral of 1/|x-p| from the end distances and the length alone:
+// with s = Ra + Rb and l = |rb - ra|, (s + l)(s - l) = 2 (Ra Rb + ra . rb),
+// so
+//     L = ln((s + l)/(s - l)) = ln(1 + l (l + s) / (Ra Rb + ra . rb)),
+// whose terms are all positive when ra . rb >= 0 (the point sees the edge
+// under an angle below pi/2).  For ra . rb < 0 the denominator is
+// |ra x rb|^2 / (Ra Rb - ra . rb), again without cancellation.  A wave whose
+// lanes all have ra . rb >= 0 skips the cross product; each lane's result
+// does not depend on the branch its wave takes.
+RH_HD double edge_L(const V3<double>& ra, double Ra, const V3<double>& rb, double Rb, double len)
+{
+    const double q = len * (len + (Ra + Rb));
+    const double dab = dot(ra, rb);
+    double den = fma(Ra, Rb, dab);
+    double num = den + q;
+#ifdef __HIP_DEVICE_COMPILE__
+    if (!__all(dab >= 0.0))
+#endif
+    {
+        RH_KEEP_BRANCH();
+        if (dab < 0.0) {
+            const V3<double> c = cross(ra, rb);
+            den = dot(c, c);
+            num = fma(q, fma(Ra, Rb, -dab), den);
+        }
+    }
+    return rh_log_ratio(num, den);
+}
+
+// FP32: the same ratio as ln(1 + delta), delta = l (l + s) / (Ra Rb + ra . rb).
+RH_HD float edge_L(const V3<float>& ra, float Ra, const V3<float>& rb, float Rb, float len)
+{
+    const float q = len * (len + (Ra + Rb));
+    const float dab = dot(ra, rb);
+    float delta = q * rh_rcp(fmaf(Ra, Rb, dab));
+#ifdef __HIP_DEVICE_COMPILE__
+    if (!__all(dab >= 0.0f))
+#endif
+    {
+        RH_KEEP_BRANCH();
+        if (dab < 0.0f) {
+            const V3<float> c = cross(ra, rb);
+            delta = (q * fmaf(Ra, Rb, -dab)) * rh_rcp(dot(c, c));
+        }
+    }
+    return rh_log1p(delta);
+}
+
 // Solid-angle factor of triangle (r1, r2, r3) as a complex number whose
 // argument is half the signed solid angle (Van Oosterom & Strackee 1983).
 template <class R>

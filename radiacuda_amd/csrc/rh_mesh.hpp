@@ -40,11 +40,7 @@ RH_HD double mesh_edge_L(const double* __restrict__ rec, const double p[3])
 {
     const V3<R> ra = rel<R>(rec, p), rb = rel<R>(rec + 3, p);
     const R Ra = rh_sqrt(dot(ra, ra)), Rb = rh_sqrt(dot(rb, rb));
-    const V3<R> t = v3(R(rec[6]), R(rec[7]), R(rec[8]));
-    if (sizeof(R) == 4) return double(edge_log1p(ra, Ra, rb, Rb, t, R(rec[9])));
-    R num, den;
-    edge_ratio(ra, Ra, rb, Rb, t, num, den);
-    return double(rh_log_ratio(num, den));
+    return double(edge_L(ra, Ra, rb, Rb, R(rec[9])));
 }
 
 // Signed solid angle of one charged face at p: the fan triangles (v0,v1,v2)
@@ -94,17 +90,6 @@ RH_HD MeshVtx<R> mesh_vtx(const double* __restrict__ v, const double p[3])
 {
     const V3<R> r = rel<R>(v, p);
     return MeshVtx<R>{r, rh_sqrt(dot(r, r))};
-}
-
-// L_e of a charged edge from the offsets and distances of its ends.
-template <class R>
-RH_HD double mesh_edge_L_from(const V3<R>& ra, R Ra, const V3<R>& rb, R Rb, const double* __restrict__ rec)
-{
-    const V3<R> t = v3(R(rec[6]), R(rec[7]), R(rec[8]));
-    if (sizeof(R) == 4) return double(edge_log1p(ra, Ra, rb, Rb, t, R(rec[9])));
-    R num, den;
-    edge_ratio(ra, Ra, rb, Rb, t, num, den);
-    return double(rh_log_ratio(num, den));
 }
 
 // Signed solid angle of a charged triangle from the offsets and distances of
