@@ -655,15 +655,7 @@ RH_HD bool field_tet(const double* __restrict__ rec, const double p[3], double H
     for (int e = 0; e < 6; e++) {
         const double* E = rec + 12 + 7 * e;
         const int a = kTetEdgeA[e], b = kTetEdgeB[e];
-        const V3<R> t = v3(R(E[0]), R(E[1]), R(E[2]));
-        double L;
-        if (sizeof(R) == 4) {
-            L = double(edge_log1p(r[a], Rv[a], r[b], Rv[b], t, R(E[3])));
-        } else {
-            R num, den;
-            edge_ratio(r[a], Rv[a], r[b], Rv[b], t, num, den);
-            L = double(rh_log_ratio(num, den));
-        }
+        const double L = double(edge_L(r[a], Rv[a], r[b], Rv[b], R(E[3])));
         h0 += L * E[4];
         h1 += L * E[5];
         h2 += L * E[6];
@@ -687,7 +679,7 @@ RH_HD bool field_tet(const double* __restrict__ rec, const double p[3], double H
             const R im = R(F[3]) * hd[f];
             const R re = Rv[a] * Rv[b] * Rv[c] + dot(r[a], r[b]) * Rv[c] + dot(r[a], r[c]) * Rv[b] +
                          dot(r[b], r[c]) * Rv[a];
-            om = 2.0 * double(rh_atan2(im, re));
+            om = 2.0 * double(rh_atan2_wave(im, re));
             om_sum += om;
         } else {
             om = (inside ? 4.0 * 3.14159265358979323846 : 0.0) - om_sum;

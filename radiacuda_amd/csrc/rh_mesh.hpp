@@ -64,7 +64,7 @@ RH_HD double mesh_face_omega(const double* __restrict__ rec, const double p[3])
         re = pr;
         im = pi;
     }
-    return 2.0 * double(rh_atan2(im, re));
+    return 2.0 * double(rh_atan2_wave(im, re));
 }
 
 // Wave-uniform integer of a 0/1/2 code stored as a double: its high word
@@ -134,7 +134,7 @@ RH_HD float mesh32_quad_omega(const double* __restrict__ rec, const double p[3])
     const float im = f[3] * h, im2 = f[4] * h;
     const float re = R0 * R1 * R2 + dot(r0, r1) * R2 + dot(r0, r2) * R1 + dot(r1, r2) * R0;
     const float re2 = R0 * R2 * R3 + dot(r0, r2) * R3 + dot(r0, r3) * R2 + dot(r2, r3) * R0;
-    return 2.0f * rh_atan2(re * im2 + im * re2, re * re2 - im * im2);
+    return 2.0f * rh_atan2_wave(re * im2 + im * re2, re * re2 - im * im2);
 }
 
 // Plane test of one inside record.
