@@ -45,45 +45,126 @@ __device__ __forceinline__ void store_out(T* p, double v) { *p = T(v); }
 
 // ---------------------------------------------------------------- assembly
 // Rows of elements [r0, r1) into Q (Q holds row 3*r0 first); block columns
-// are the elements kidx[k0 .. k1) of one kind slot.
+// are the elements kidx[k0 .. k1) of one kind slot.  Each lane owns one row
+// element and the wave sweeps the sources with uniform records.  A lane's
+// 3x3 blocks for S consecutive sources are staged in LDS and the wave then
+// writes the 3*64 matrix rows of the tile as contiguous row segments of 96
+// bytes (4 sources in FP64, 8 in FP32), so that a cache line is written by
+// at most two consecutive tiles of one wave instead of by 64 rows' worth of
+// interleaved 24-byte stores; the tile (20 KB) still allows two waves per
+// SIMD.  Measured on the 8,000-tetrahedron yoke: 2.95 ms against 4.15 ms
+// unstaged (FP64), 2.41 against 3.03 ms (FP32); wider tiles cost occupancy.
+// Sources whose columns are not consecutive (mixed kinds) are stored
+// directly.
+#ifndef RH_ASM_S64
+#define RH_ASM_S64 4
+#endif
+#ifndef RH_ASM_S32
+#define RH_ASM_S32 8
+#endif
+template <class OUT>
+struct AsmTile {
+    static constexpr int S = (sizeof(OUT) == 8) ? RH_ASM_S64 : RH_ASM_S32;  // sources per tile
+    static constexpr int U = 3 * S * int(sizeof(OUT)) / 8;  // 8-byte units per row segment
+    static constexpr int W = (U + 1) * 8 / int(sizeof(OUT));  // LDS row stride (elements)
+};
+
 template <class R, bool SYM, bool ROWT, class OUT, int KIND>
 __global__ __launch_bounds__(64) void k_assemble(DevModel m, OUT* __restrict__ Q, int64_t ld,
                                                  double* __restrict__ qdiag, int64_t jt, int64_t r0,
                                                  int64_t r1)
 {
-    const int64_t i = r0 + int64_t(blockIdx.x) * 64 + threadIdx.x;
+    constexpr int S = AsmTile<OUT>::S, W = AsmTile<OUT>::W;
+    __shared__ OUT tile[3 * 64 * W];
+    const int lane = threadIdx.x;
+    const int64_t i0 = r0 + int64_t(blockIdx.x) * 64;
+    const int64_t i = i0 + lane;
+    const bool live = i < r1;
+    const int nrow = int((r1 - i0 < 64) ? (r1 - i0) : 64);
     const int64_t kend = m.koff[KIND + 1];
     const int64_t k0 = m.koff[KIND] + int64_t(blockIdx.y) * jt;
     const int64_t k1 = (k0 + jt < kend) ? (k0 + jt) : kend;
-    if (i >= r1) return;
     const ModelView mv = view(m);
-    const double x[3] = {m.colloc[3 * i], m.colloc[3 * i + 1], m.colloc[3 * i + 2]};
+    const int64_t ic = live ? i : i0;  // idle lanes evaluate row i0 and store nothing
+    const double x[3] = {m.colloc[3 * ic], m.colloc[3 * ic + 1], m.colloc[3 * ic + 2]};
     double rt[9];
     if (ROWT)
-        for (int k = 0; k < 9; k++) rt[k] = m.rowt[9 * i + k];
-    OUT* row = Q + (3 * (i - r0)) * ld;
-    for (int64_t k = k0; k < k1; k++) {
-        const int64_t j = m.kidx[k];
-        double q[9];
-        block_Q<R, SYM, KIND>(mv, j, x, q);
-        if (ROWT) {
-            double t[9];
-            for (int a = 0; a < 3; a++)
-                for (int b = 0; b < 3; b++)
-                    t[3 * a + b] = rt[3 * a] * q[b] + rt[3 * a + 1] * q[3 + b] + rt[3 * a + 2] * q[6 + b];
-            for (int k = 0; k < 9; k++) q[k] = t[k];
-        }
-        if (j == i) {
-            for (int k = 0; k < 9; k++) {
-                qdiag[9 * i + k] = q[k];
-                q[k] = 0.0;
+        for (int k = 0; k < 9; k++) rt[k] = m.rowt[9 * ic + k];
+    OUT* row = Q + (3 * (ic - r0)) * ld;
+    for (int64_t kb = k0; kb < k1; kb += S) {
+        const int cnt = int((k1 - kb < S) ? (k1 - kb) : S);
+        const int64_t jb = m.kidx[kb];
+        const bool run = (m.kidx[kb + cnt - 1] - jb == cnt - 1);  // consecutive columns
+        for (int s = 0; s < cnt; s++) {
+            const int64_t j = m.kidx[kb + s];
+            double q[9];
+            block_Q<R, SYM, KIND>(mv, j, x, q);
+            if (ROWT) {
+                double t[9];
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++)
+                        t[3 * a + b] = rt[3 * a] * q[b] + rt[3 * a + 1] * q[3 + b] + rt[3 * a + 2] * q[6 + b];
+                for (int k = 0; k < 9; k++) q[k] = t[k];
+            }
+            if (live && j == i) {
+                for (int k = 0; k < 9; k++) {
+                    qdiag[9 * i + k] = q[k];
+                    q[k] = 0.0;
+                }
+            }
+            if (run) {
+                OUT* t = tile + 3 * lane * W + 3 * s;
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) t[a * W + b] = OUT(q[3 * a + b]);
+            } else if (live) {
+                OUT* o = row + 3 * j;
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) store_out(o + a * ld + b, q[3 * a + b]);
             }
         }
-        OUT* o = row + 3 * j;
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-            for (int b = 0; b < 3; b++) store_out(o + a * ld + b, q[3 * a + b]);
+        if (run) {
+            __syncthreads();
+            OUT* base = Q + (3 * (i0 - r0)) * ld + 3 * jb;
+            constexpr int U = AsmTile<OUT>::U, RP = 64 / U;  // rows per pass
+            if (cnt == S && nrow == 64 && ((3 * jb * int64_t(sizeof(OUT))) & 7) == 0) {
+                // full tile: 8-byte units, RP rows of U units per pass, offsets
+                // in 32 bits from a uniform base (the tile spans < 2^32 bytes)
+                const int lr = lane / U, lc = lane - U * lr;
+                if (lr < RP) {
+                    constexpr int WU = W * int(sizeof(OUT)) / 8;
+                    const uint64_t* src = reinterpret_cast<const uint64_t*>(tile) + lr * WU + lc;
+                    char* dst = reinterpret_cast<char*>(base);  // uniform, advanced per pass
+                    const int64_t step = int64_t(RP) * ld * int64_t(sizeof(OUT));
+                    const uint32_t off = uint32_t(lr) * uint32_t(ld * sizeof(OUT)) + uint32_t(8 * lc);
+                    constexpr int full = 192 / RP;  // passes with every lane's row in the tile
+#pragma unroll 4
+                    for (int it = 0; it < full; it++) {
+                        *reinterpret_cast<uint64_t*>(dst + off) = src[RP * it * WU];
+                        dst += step;
+                    }
+                    if (RP * full + lr < 192) *reinterpret_cast<uint64_t*>(dst + off) = src[RP * full * WU];
+                }
+            } else {
+                // partial tile: the 3*nrow matrix rows, 3*cnt values each
+                const int cols = 3 * cnt, total = 3 * nrow * cols;
+                const int step_r = 64 / cols, step_c = 64 - step_r * cols;
+                int r = lane / cols, c = lane - r * cols;
+                for (int e = lane; e < total; e += 64) {
+                    base[int64_t(r) * ld + c] = tile[r * W + c];
+                    c += step_c;
+                    r += step_r;
+                    if (c >= cols) {
+                        c -= cols;
+                        r++;
+                    }
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 
