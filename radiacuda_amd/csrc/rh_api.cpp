@@ -321,6 +321,8 @@ rh_status rh_model_create(const rh_model_desc* desc, int device, void* stream, r
         if (e == hipSuccess) e = up(&md.tile_box, F.tile_box);
         if (e == hipSuccess) e = up(&md.edge_cont, F.edge_cont);
         if (e == hipSuccess) e = up(&md.face_cont, F.face_cont);
+        if (e == hipSuccess) e = up(&md.face_edge, F.face_edge);
+        md.ne_free = F.ne_free;
         double* ch = nullptr;
         if (e == hipSuccess) e = dalloc(&ch, size_t(rh::kMeshEdge * md.ne + rh::kMeshFace * md.nf +
                                                     rh::kMeshIn * md.ni) + 2);

@@ -531,39 +531,6 @@ void build_mesh_form(PackedModel& P)
     // triangle's last two (the solid angle of a triangle does not depend on
     // the order of its vertices once the normal is fixed).  The sweeps then
     // compute one vertex distance per continued edge or triangle.
-    std::vector<std::pair<int32_t, int32_t>> ekeys(einc.size());
-    for (const auto& kv : eid) ekeys[size_t(kv.second)] = kv.first;
-    std::vector<int32_t> eorder;
-    std::vector<char> eflip, econt;
-    {
-        std::vector<std::vector<int32_t>> vadj(vpos.size());
-        for (size_t e = 0; e < ekeys.size(); e++) {
-            vadj[size_t(ekeys[e].first)].push_back(int32_t(e));
-            vadj[size_t(ekeys[e].second)].push_back(int32_t(e));
-        }
-        std::vector<char> used(ekeys.size(), 0);
-        std::vector<size_t> next(vpos.size(), 0);
-        auto take = [&](int32_t v) -> int32_t {
-            std::vector<int32_t>& a = vadj[size_t(v)];
-            while (next[size_t(v)] < a.size() && used[size_t(a[next[size_t(v)]])]) next[size_t(v)]++;
-            return next[size_t(v)] < a.size() ? a[next[size_t(v)]] : -1;
-        };
-        for (size_t e0 = 0; e0 < ekeys.size(); e0++) {
-            if (used[e0]) continue;
-            int32_t cur = ekeys[e0].first, e = int32_t(e0);
-            bool first = true;
-            while (e >= 0) {
-                used[size_t(e)] = 1;
-                const bool flip = ekeys[size_t(e)].first != cur;
-                eorder.push_back(e);
-                eflip.push_back(flip ? 1 : 0);
-                econt.push_back(first ? 0 : 1);
-                first = false;
-                cur = flip ? ekeys[size_t(e)].first : ekeys[size_t(e)].second;
-                e = take(cur);
-            }
-        }
-    }
     std::vector<int32_t> forder;
     std::vector<std::vector<int32_t>> fverts;
     std::vector<char> fcont;
@@ -616,6 +583,61 @@ void build_mesh_form(PackedModel& P)
                 fcont.push_back(2);  // quadrilateral
             }
     }
+    // Each unique edge that is a side of a triangle is owned by the first
+    // triangle in sweep order that has it: that triangle's sweep step already
+    // holds both vertex offsets, distances and their dot product.
+    std::vector<std::pair<int32_t, int32_t>> ekeys(einc.size());
+    for (const auto& kv : eid) ekeys[size_t(kv.second)] = kv.first;
+    std::vector<int64_t> owner(einc.size(), -1);  // 3 * face position + side
+    for (size_t o = 0; o < forder.size(); o++) {
+        if (fcont[o] == 2) continue;
+        for (int k = 0; k < 3; k++) {
+            const int32_t a = fverts[o][size_t(k)], b = fverts[o][size_t((k + 1) % 3)];
+            const auto it = eid.find({std::min(a, b), std::max(a, b)});
+            if (it == eid.end()) continue;  // a diagonal of a fanned polygon
+            if (owner[size_t(it->second)] < 0) owner[size_t(it->second)] = int64_t(3 * o + k);
+        }
+    }
+    std::vector<int32_t> eorder;
+    std::vector<char> eflip, econt;
+    {
+        std::vector<std::vector<int32_t>> vadj(vpos.size());
+        for (size_t e = 0; e < ekeys.size(); e++) {
+            vadj[size_t(ekeys[e].first)].push_back(int32_t(e));
+            vadj[size_t(ekeys[e].second)].push_back(int32_t(e));
+        }
+        std::vector<char> used(ekeys.size(), 0);
+        std::vector<size_t> next(vpos.size(), 0);
+        auto take = [&](int32_t v) -> int32_t {
+            std::vector<int32_t>& a = vadj[size_t(v)];
+            while (next[size_t(v)] < a.size() && used[size_t(a[next[size_t(v)]])]) next[size_t(v)]++;
+            return next[size_t(v)] < a.size() ? a[next[size_t(v)]] : -1;
+        };
+        for (size_t e = 0; e < ekeys.size(); e++)
+            if (owner[e] >= 0) used[e] = 1;  // evaluated by its triangle
+        for (size_t e0 = 0; e0 < ekeys.size(); e0++) {
+            if (used[e0]) continue;
+            int32_t cur = ekeys[e0].first, e = int32_t(e0);
+            bool first = true;
+            while (e >= 0) {
+                used[size_t(e)] = 1;
+                const bool flip = ekeys[size_t(e)].first != cur;
+                eorder.push_back(e);
+                eflip.push_back(flip ? 1 : 0);
+                econt.push_back(first ? 0 : 1);
+                first = false;
+                cur = flip ? ekeys[size_t(e)].first : ekeys[size_t(e)].second;
+                e = take(cur);
+            }
+        }
+        M.ne_free = int64_t(eorder.size());
+        for (size_t e = 0; e < ekeys.size(); e++)
+            if (owner[e] >= 0) {
+                eorder.push_back(int32_t(e));
+                eflip.push_back(0);
+                econt.push_back(0);
+            }
+    }
     // edges
     M.edge_inc_off.push_back(0);
     for (size_t o = 0; o < eorder.size(); o++) {
@@ -635,6 +657,12 @@ void build_mesh_form(PackedModel& P)
             for (int k = 0; k < 6; k++) M.edge_inc_w.push_back(inc.second[k]);
         }
         M.edge_inc_off.push_back(int64_t(M.edge_inc_elem.size()));
+    }
+    // owned edges: record index per triangle side
+    M.face_edge.assign(3 * forder.size(), -1);
+    for (size_t o = 0; o < eorder.size(); o++) {
+        const int64_t w = owner[size_t(eorder[o])];
+        if (w >= 0) M.face_edge[size_t(w)] = int64_t(o);
     }
     // faces
     M.face_inc_off.push_back(0);

@@ -237,6 +237,9 @@ constexpr int kTetTile = 32;
 #define RH_TET_WAVES 3
 #endif
 // edge and face records staged per LDS tile of the mesh sweep
+#ifndef RH_MESH_STAGE32
+#define RH_MESH_STAGE32 64
+#endif
 #ifndef RH_MESH_STAGE
 #define RH_MESH_STAGE 128
 #endif
@@ -362,6 +365,27 @@ __global__ __launch_bounds__(256) void k_mesh_charge_faces(MeshDev md, const dou
     o[19] = s * n1;
     o[20] = s * n2;
     o[21] = md.face_cont[i];
+    // owned edges: length and the net weight of the charged edge record
+    // (charged before the faces, on the same stream)
+    int32_t mask = 0;
+    for (int k = 0; k < 3; k++) {
+        const int64_t e = md.face_edge[3 * i + k];
+        double* d = o + kMeshFaceEdges + 4 * k;
+        if (e >= 0) {
+            const double* ce = md.ch_edge + kMeshEdge * e;
+            d[0] = md.edge_geo[kMeshEdgeGeo * e + 9];
+            d[1] = ce[10];
+            d[2] = ce[11];
+            d[3] = ce[12];
+            mask |= 1 << k;
+        } else {
+            d[0] = d[1] = d[2] = d[3] = 0.0;
+        }
+    }
+    int32_t* mk = reinterpret_cast<int32_t*>(o + kMeshFaceEdges + 12);
+    mk[0] = mask;
+    mk[1] = 0;
+    o[kMeshFaceEdges + 13] = 0.0;
     if (pack32) {
         float* f = reinterpret_cast<float*>(o + 12);
         const float a1 = float(g[15]), a2 = float(g[16]);
@@ -397,7 +421,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH_TET_WAVE
 k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, int64_t slice0,
              double* __restrict__ B, double* __restrict__ H, double* __restrict__ part)
 {
-    __shared__ __align__(16) double tile[RH_MESH_STAGE * kMeshFace > kMeshTile * kMeshIn ? RH_MESH_STAGE * kMeshFace
+    // records per staging tile: FP64 sweeps are held to 4 waves per SIMD by
+    // their registers, FP32 sweeps fit 7 and take a smaller tile to keep them
+    constexpr int kStage = (sizeof(R) == 4) ? RH_MESH_STAGE32 : RH_MESH_STAGE;
+    __shared__ __align__(16) double tile[kStage * kMeshFace > kMeshTile * kMeshIn ? kStage * kMeshFace
                                                                                      : kMeshTile * kMeshIn];
     const MeshDev& md = m.mesh;
     const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
@@ -411,13 +438,14 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
     }
     double h[3] = {0.0, 0.0, 0.0}, min[3] = {0.0, 0.0, 0.0};
     double2* dst = reinterpret_cast<double2*>(tile);
-    // unique edges: + L_e w_e, along paths: a continued edge takes its first
-    // vertex's offset and distance from the previous edge.  The loop is
-    // unrolled by two with the vertex slots P, Q swapping roles, so the
-    // carried values stay in place (no register copies per edge).
+    // unique edges that no triangle owns (edges of quadrilaterals only):
+    // + L_e w_e, along paths: a continued edge takes its first vertex's
+    // offset and distance from the previous edge.  The loop is unrolled by
+    // two with the vertex slots P, Q swapping roles, so the carried values
+    // stay in place (no register copies per edge).
     {
         MeshVtx<R> P{}, Q{};
-        const int64_t e0 = md.ne * y / S, e1 = md.ne * (y + 1) / S;
+        const int64_t e0 = md.ne_free * y / S, e1 = md.ne_free * (y + 1) / S;
         auto edge = [&](int64_t g, const double* rec, MeshVtx<R>& a, MeshVtx<R>& b) {
             if (!(mesh_code(rec + 13) == 1 && g != e0)) a = mesh_vtx<R>(rec, x);
             b = mesh_vtx<R>(rec + 3, x);
@@ -429,8 +457,8 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
             h[1] += L * rec[11];
             h[2] += L * rec[12];
         };
-        for (int64_t base = e0; base < e1; base += RH_MESH_STAGE) {
-            const int cnt = int((e1 - base < RH_MESH_STAGE) ? (e1 - base) : RH_MESH_STAGE);
+        for (int64_t base = e0; base < e1; base += kStage) {
+            const int cnt = int((e1 - base < kStage) ? (e1 - base) : kStage);
             const double2* src = reinterpret_cast<const double2*>(md.ch_edge + kMeshEdge * base);
             __syncthreads();
             for (int i = threadIdx.x; i < cnt * (kMeshEdge / 2); i += 256) dst[i] = src[i];
@@ -450,53 +478,54 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
     }
     // unique faces: - Omega_f g_f, triangles along strips: a continued
     // triangle takes its first two vertices (and their dot product) from the
-    // previous one.  Unrolled by three with the slots A, B, C rotating.
+    // previous one.  Unrolled by three with the slots A, B, C rotating.  A
+    // triangle also evaluates the edges it owns (+ L_e w_e) from the
+    // offsets, distances and dot products of its vertices.
     {
         MeshVtx<R> A{}, Bv{}, C{};
         R dAB = R(0), dBC = R(0), dCA = R(0);
         const int64_t f0 = md.nf * y / S, f1 = md.nf * (y + 1) / S;
+        // an edge owned by the current triangle: the offsets and distances of
+        // its ends and their dot product are already at hand
+        auto owned = [&](const MeshVtx<R>& u, const MeshVtx<R>& v, R duv, const double* e) {
+            const double L = double(edge_L(u.r, u.d, v.r, v.d, duv, R(e[0])));
+            h[0] += L * e[1];
+            h[1] += L * e[2];
+            h[2] += L * e[3];
+        };
         auto face = [&](int64_t g, const double* rec, MeshVtx<R>& a, MeshVtx<R>& b, MeshVtx<R>& c, R& dab,
                         R& dbc) {
             const int code = mesh_code(rec + 21);
-            if constexpr (sizeof(R) == 4) {
-                float om;
-                if (code == 2) {
-                    om = mesh32_quad_omega(rec, x);
-                } else {
-                    if (!(code == 1 && g != f0)) {
-                        a = mesh_vtx<R>(rec, x);
-                        b = mesh_vtx<R>(rec + 3, x);
-                        dab = dot(a.r, b.r);
-                    }
-                    c = mesh_vtx<R>(rec + 6, x);
-                    dbc = dot(b.r, c.r);
-                    om = mesh32_tri_omega(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, rec);
-                }
-                const double omd = double(om);
-                h[0] -= omd * rec[18];
-                h[1] -= omd * rec[19];
-                h[2] -= omd * rec[20];
-            } else {
-                double om;
-                if (code == 2) {
+            double om;
+            if (code == 2) {
+                if constexpr (sizeof(R) == 4)
+                    om = double(mesh32_quad_omega(rec, x));
+                else
                     om = mesh_face_omega<R>(rec, x);
-                } else {
-                    if (!(code == 1 && g != f0)) {
-                        a = mesh_vtx<R>(rec, x);
-                        b = mesh_vtx<R>(rec + 3, x);
-                        dab = dot(a.r, b.r);
-                    }
-                    c = mesh_vtx<R>(rec + 6, x);
-                    dbc = dot(b.r, c.r);
-                    om = mesh_tri_omega_from<R>(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, rec);
+            } else {
+                if (!(code == 1 && g != f0)) {
+                    a = mesh_vtx<R>(rec, x);
+                    b = mesh_vtx<R>(rec + 3, x);
+                    dab = dot(a.r, b.r);
                 }
-                h[0] -= om * rec[18];
-                h[1] -= om * rec[19];
-                h[2] -= om * rec[20];
+                c = mesh_vtx<R>(rec + 6, x);
+                dbc = dot(b.r, c.r);
+                const R dca = dot(c.r, a.r);
+                if constexpr (sizeof(R) == 4)
+                    om = double(mesh32_tri_omega(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, dca, rec));
+                else
+                    om = mesh_tri_omega_from<R>(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, dca, rec);
+                const int mask = mesh_mask(rec + kMeshFaceEdges + 12);
+                if (mask & 1) owned(a, b, dab, rec + kMeshFaceEdges);
+                if (mask & 2) owned(b, c, dbc, rec + kMeshFaceEdges + 4);
+                if (mask & 4) owned(c, a, dca, rec + kMeshFaceEdges + 8);
             }
+            h[0] -= om * rec[18];
+            h[1] -= om * rec[19];
+            h[2] -= om * rec[20];
         };
-        for (int64_t base = f0; base < f1; base += RH_MESH_STAGE) {
-            const int cnt = int((f1 - base < RH_MESH_STAGE) ? (f1 - base) : RH_MESH_STAGE);
+        for (int64_t base = f0; base < f1; base += kStage) {
+            const int cnt = int((f1 - base < kStage) ? (f1 - base) : kStage);
             const double2* src = reinterpret_cast<const double2*>(md.ch_face + kMeshFace * base);
             __syncthreads();
             for (int i = threadIdx.x; i < cnt * (kMeshFace / 2); i += 256) dst[i] = src[i];

@@ -27,6 +27,8 @@ struct MeshDev {
     const double* tile_box = nullptr;
     const double* edge_cont = nullptr;
     const double* face_cont = nullptr;
+    int64_t ne_free = 0;                // edges [0, ne_free) are swept on their own
+    const int64_t* face_edge = nullptr;  // 3 per face: owned edge record or -1
     double* ch_edge = nullptr;
     double* ch_face = nullptr;
     double* ch_in = nullptr;

@@ -366,10 +366,9 @@ RH_HD R edge_log1p(const V3<R>& ra, R Ra, const V3<R>& rb, R Rb, const V3<R>& t,
 // |ra x rb|^2 / (Ra Rb - ra . rb), again without cancellation.  A wave whose
 // lanes all have ra . rb >= 0 skips the cross product; each lane's result
 // does not depend on the branch its wave takes.
-RH_HD double edge_L(const V3<double>& ra, double Ra, const V3<double>& rb, double Rb, double len)
+RH_HD double edge_L(const V3<double>& ra, double Ra, const V3<double>& rb, double Rb, double dab, double len)
 {
     const double q = len * (len + (Ra + Rb));
-    const double dab = dot(ra, rb);
     double den = fma(Ra, Rb, dab);
     double num = den + q;
 #ifdef __HIP_DEVICE_COMPILE__
@@ -387,10 +386,9 @@ RH_HD double edge_L(const V3<double>& ra, double Ra, const V3<double>& rb, doubl
 }
 
 // FP32: the same ratio as ln(1 + delta), delta = l (l + s) / (Ra Rb + ra . rb).
-RH_HD float edge_L(const V3<float>& ra, float Ra, const V3<float>& rb, float Rb, float len)
+RH_HD float edge_L(const V3<float>& ra, float Ra, const V3<float>& rb, float Rb, float dab, float len)
 {
     const float q = len * (len + (Ra + Rb));
-    const float dab = dot(ra, rb);
     float delta = q * rh_rcp(fmaf(Ra, Rb, dab));
 #ifdef __HIP_DEVICE_COMPILE__
     if (!__all(dab >= 0.0f))
@@ -403,6 +401,12 @@ RH_HD float edge_L(const V3<float>& ra, float Ra, const V3<float>& rb, float Rb,
         }
     }
     return rh_log1p(delta);
+}
+
+template <class R>
+RH_HD R edge_L(const V3<R>& ra, R Ra, const V3<R>& rb, R Rb, R len)
+{
+    return edge_L(ra, Ra, rb, Rb, dot(ra, rb), len);
 }
 
 // Solid-angle factor of triangle (r1, r2, r3) as a complex number whose

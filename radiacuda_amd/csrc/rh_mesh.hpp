@@ -22,10 +22,13 @@ namespace rh {
 constexpr int kMeshEdge = 14;
 // Charged face record: v0..v3 (12), n(3), 2*area(v0,v1,v2), 2*area(v0,v2,v3),
 // nv, g(3), code (0 triangle, 1 triangle whose v0, v1 are the previous
-// record's v1, v2, 2 quadrilateral).  A
+// record's v1, v2, 2 quadrilateral), then the edges the triangle owns: for
+// the sides (v0,v1), (v1,v2), (v2,v0) in turn len and w(3) (slots 22-33), and
+// the mask of owned sides as an integer in the low word of slot 34.  A
 // quadrilateral is CCW about n; a triangle's vertices may come in either
 // order (its solid angle below uses n and the area, not the order).
-constexpr int kMeshFace = 22;
+constexpr int kMeshFace = 36;
+constexpr int kMeshFaceEdges = 22;
 // Inside-test record: nplanes, M(3), up to 8 planes (n(3), d) with outward n:
 // inside iff n.p < d for every plane.
 constexpr int kMeshPlanes = 8;
@@ -79,6 +82,17 @@ RH_HD int mesh_code(const double* __restrict__ slot)
     return hi == 0 ? 0 : (hi == 0x3ff00000 ? 1 : 2);
 }
 
+// The owned-side mask of a face record (an integer in the slot's low word),
+// wave-uniform on the device.
+RH_HD int mesh_mask(const double* __restrict__ slot)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_readfirstlane(reinterpret_cast<const int*>(slot)[0]);
+#else
+    return int(uint32_t(__builtin_bit_cast(uint64_t, *slot) & 0xffffffffu));
+#endif
+}
+
 // A vertex relative to the observation point and its distance.
 template <class R>
 struct MeshVtx {
@@ -96,11 +110,11 @@ RH_HD MeshVtx<R> mesh_vtx(const double* __restrict__ v, const double p[3])
 // its vertices and r0 . r1.
 template <class R>
 RH_HD double mesh_tri_omega_from(const V3<R>& r0, R R0, const V3<R>& r1, R R1, const V3<R>& r2, R R2, R d01,
-                                 R d12, const double* __restrict__ rec)
+                                 R d12, R d20, const double* __restrict__ rec)
 {
     const V3<R> n = v3(R(rec[12]), R(rec[13]), R(rec[14]));
     const R im = R(rec[15]) * dot(r0, n);
-    const R re = R0 * R1 * R2 + d01 * R2 + dot(r0, r2) * R1 + d12 * R0;
+    const R re = R0 * R1 * R2 + d01 * R2 + d20 * R1 + d12 * R0;
     return 2.0 * double(rh_atan2_wave(im, re));
 }
 
@@ -115,11 +129,11 @@ RH_HD const float* mesh32(const double* __restrict__ rec, int slot)
 }
 
 RH_HD float mesh32_tri_omega(const V3<float>& r0, float R0, const V3<float>& r1, float R1, const V3<float>& r2,
-                             float R2, float d01, float d12, const double* __restrict__ rec)
+                             float R2, float d01, float d12, float d20, const double* __restrict__ rec)
 {
     const float* f = mesh32(rec, 12);
     const float im = f[3] * dot(r0, v3(f[0], f[1], f[2]));
-    const float re = R0 * R1 * R2 + d01 * R2 + dot(r0, r2) * R1 + d12 * R0;
+    const float re = R0 * R1 * R2 + d01 * R2 + d20 * R1 + d12 * R0;
     return 2.0f * rh_atan2_wave(im, re);
 }
 
