@@ -563,12 +563,41 @@ void build_mesh_form(PackedModel& P)
             forder.push_back(int32_t(f0));
             fverts.push_back(v);
             fcont.push_back(0);
-            int32_t p = v[1], q = v[2];
-            for (int32_t g = other(p, q); g >= 0; g = other(p, q)) {
-                used[size_t(g)] = 1;
+            // among the triangles on (p, q), take one whose far edge (q, r)
+            // still continues the strip, preferring the one with the fewest
+            // continuations (strip starts on the C-yoke: 13 % -> 9 %)
+            auto third = [&](int32_t g, int32_t a, int32_t b) {
                 int32_t r = -1;
                 for (int32_t x : floop[size_t(g)])
-                    if (x != p && x != q) r = x;
+                    if (x != a && x != b) r = x;
+                return r;
+            };
+            auto next = [&](int32_t a, int32_t b) -> int32_t {
+                auto it = tri_at.find({std::min(a, b), std::max(a, b)});
+                if (it == tri_at.end()) return -1;
+                int32_t best = -1;
+                int best_n = 0;
+                for (int32_t g : it->second) {
+                    if (used[size_t(g)]) continue;
+                    const int32_t r = third(g, a, b);
+                    int n = 0;
+                    auto jt = tri_at.find({std::min(b, r), std::max(b, r)});
+                    if (jt != tri_at.end())
+                        for (int32_t h : jt->second)
+                            if (h != g && !used[size_t(h)]) n++;
+                    // score: continuing candidates first, then fewer continuations
+                    const bool better = best < 0 || (n > 0 && (best_n == 0 || n < best_n));
+                    if (better) {
+                        best = g;
+                        best_n = n;
+                    }
+                }
+                return best;
+            };
+            int32_t p = v[1], q = v[2];
+            for (int32_t g = next(p, q); g >= 0; g = next(p, q)) {
+                used[size_t(g)] = 1;
+                const int32_t r = third(g, p, q);
                 forder.push_back(g);
                 fverts.push_back({p, q, r});
                 fcont.push_back(1);
