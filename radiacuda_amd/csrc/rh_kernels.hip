@@ -231,17 +231,23 @@ __global__ __launch_bounds__(256) void k_field(DevModel m, const double* __restr
 // conflicts) while it evaluates its own point.  Staging keeps the record out of
 // scalar registers, whose pressure otherwise spills through VGPR lanes.
 constexpr int kTetTile = 32;
-// occupancy floor for the tetrahedron and mesh sweeps (waves per SIMD): bounds
+// occupancy floor for the tetrahedron sweep (waves per SIMD): bounds
 // the VGPRs the scheduler may use for hoisted LDS reads
 #ifndef RH_TET_WAVES
 #define RH_TET_WAVES 3
 #endif
-// edge and face records staged per LDS tile of the mesh sweep
+// mesh sweep: occupancy floor (waves per SIMD) and edge and face records
+// staged per LDS tile.  FP64: 5 waves (96 VGPRs, 16 bytes of spill) with
+// 64-record tiles measured 53.3 ms against 54.5 ms at 4 waves with 128-record
+// tiles on the C-yoke; FP32 runs 7 waves at 67 VGPRs.
+#ifndef RH_MESH_WAVES
+#define RH_MESH_WAVES 5
+#endif
 #ifndef RH_MESH_STAGE32
 #define RH_MESH_STAGE32 64
 #endif
 #ifndef RH_MESH_STAGE
-#define RH_MESH_STAGE 128
+#define RH_MESH_STAGE 64
 #endif
 
 template <class R, bool SYM>
@@ -417,12 +423,10 @@ __global__ __launch_bounds__(256) void k_mesh_charge_in(MeshDev md, const double
 // records read at wave-uniform addresses.  Inside tiles whose bounding box
 // holds no point of the wave are skipped (wave-uniform vote).
 template <class R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH_TET_WAVES))) void
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH_MESH_WAVES))) void
 k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, int64_t slice0,
              double* __restrict__ B, double* __restrict__ H, double* __restrict__ part)
 {
-    // records per staging tile: FP64 sweeps are held to 4 waves per SIMD by
-    // their registers, FP32 sweeps fit 7 and take a smaller tile to keep them
     constexpr int kStage = (sizeof(R) == 4) ? RH_MESH_STAGE32 : RH_MESH_STAGE;
     __shared__ __align__(16) double tile[kStage * kMeshFace > kMeshTile * kMeshIn ? kStage * kMeshFace
                                                                                      : kMeshTile * kMeshIn];
