@@ -118,6 +118,95 @@ class Polyhedron(Obj):
                 raise ValueError("face vertex index out of range")
 
 
+def cut_convex(verts, faces, normal, offset, tol):
+    """Splits a convex polyhedron (vertices, face loops) by the plane
+    normal . x = offset.  Returns (below, above): each a (vertices, faces)
+    pair, or None when that side holds no volume.  Vertices within tol of the
+    plane count as on it; the cut adds one vertex per crossed edge and a cap
+    face on each side (the outward orientation of every face is settled by
+    the packer).  The pieces of radTPolyhedron::CutItself
+    (cpp/src/core/radvlpgn.cpp:1330), computed here by clipping each face
+    loop against the half-spaces."""
+    V = np.asarray(verts, dtype=np.float64)
+    nrm = np.asarray(normal, dtype=np.float64)
+    sd = V @ nrm - offset
+    sd[np.abs(sd) <= tol] = 0.0
+    if (sd >= 0.0).all():
+        return None, (V, [list(f) for f in faces])
+    if (sd <= 0.0).all():
+        return (V, [list(f) for f in faces]), None
+    out = list(V)
+    side = list(np.sign(sd))
+    made = {}
+
+    def crossing(i, j):
+        key = (min(i, j), max(i, j))
+        if key not in made:
+            t = sd[i] / (sd[i] - sd[j])
+            out.append(V[i] + t * (V[j] - V[i]))
+            side.append(0.0)
+            made[key] = len(out) - 1
+        return made[key]
+
+    lower, upper, cap = [], [], set()
+    for f in faces:
+        lo, hi = [], []
+        m = len(f)
+        for k in range(m):
+            i, j = f[k], f[(k + 1) % m]
+            if sd[i] <= 0.0:
+                lo.append(i)
+            if sd[i] >= 0.0:
+                hi.append(i)
+            if sd[i] == 0.0:
+                cap.add(i)
+            if sd[i] * sd[j] < 0.0:
+                x = crossing(i, j)
+                lo.append(x)
+                hi.append(x)
+                cap.add(x)
+        if len(lo) >= 3 and any(side[v] < 0 for v in lo):
+            lower.append(lo)
+        if len(hi) >= 3 and any(side[v] > 0 for v in hi):
+            upper.append(hi)
+    cap = sorted(cap)
+    if len(cap) >= 3:
+        P = np.array([out[i] for i in cap])
+        c = P.mean(0)
+        u = np.cross(nrm, [1.0, 0.0, 0.0] if abs(nrm[0]) < 0.9 else [0.0, 1.0, 0.0])
+        u /= np.linalg.norm(u)
+        w = np.cross(nrm, u)
+        ang = np.arctan2((P - c) @ w, (P - c) @ u)
+        loop = [cap[i] for i in np.argsort(ang)]
+        lower.append(loop)
+        upper.append(loop[::-1])
+
+    def compact(fs):
+        used = sorted({v for f in fs for v in f})
+        remap = {v: k for k, v in enumerate(used)}
+        return np.array([out[v] for v in used]), [[remap[v] for v in f] for f in fs]
+
+    return compact(lower), compact(upper)
+
+
+def division_planes(lo, hi, count, ratio):
+    """Offsets of the count-1 cutting planes between the extreme vertex
+    projections lo < hi, the piece widths in geometric progression with
+    last/first = ratio (radTPolyhedron::DeterminePointsOnCuttingPlanes,
+    cpp/src/core/radvlpgn.cpp:2386-2420)."""
+    if count <= 1:
+        return []
+    q0 = ratio ** (1.0 / (count - 1))
+    den = ratio * q0 - 1.0
+    a1 = (q0 - 1.0) / den if abs(den) > 1e-13 else 1.0 / count
+    out, tau, dt = [], a1, a1
+    for _ in range(count - 1):
+        out.append(lo + tau * (hi - lo))
+        dt *= q0
+        tau += dt
+    return out
+
+
 class ArcCur(Obj):
     """Arc of rectangular section carrying azimuthal current density (radarccu.h)."""
 

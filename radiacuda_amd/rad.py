@@ -169,21 +169,29 @@ def ObjCntSize(obj):
 
 
 def ObjDivMag(obj, k, *args):
-    """Subdivide a block into kx*ky*kz blocks ([k1,k2,k3] or [[k1,q1],...]); the
-    object becomes a container of the pieces (radrec.cpp:971-1080)."""
+    """Subdivide a block into kx*ky*kz blocks, or a polyhedron by planes normal
+    to the axes of its own frame ([k1,k2,k3] or [[k1,q1],...]); the object
+    becomes a container of the pieces (radrec.cpp:971-1080 for blocks;
+    radvlpgn.cpp:2176-2300 for polyhedra: the planes of each direction are
+    placed over the extent of the whole polyhedron, the pieces ordered by x
+    slice, then y, then z, empty pieces dropped)."""
     o = _obj(obj)
     if isinstance(o, O.Container):
         for m in list(o.members):
             ObjDivMag(m, k, *args)
         return obj
-    if not isinstance(o, O.RecMag):
-        raise NotImplementedError("ObjDivMag is supported for rectangular blocks only")
     spec = []
     for kk in k:
         if isinstance(kk, (list, tuple)):
             spec.append((int(round(kk[0])), float(kk[1])))
         else:
             spec.append((int(round(kk)), 1.0))
+    if any(cnt < 1 for cnt, _q in spec):
+        raise RuntimeError("subdivision numbers must be >= 1")
+    if isinstance(o, O.Polyhedron):
+        return _div_polyhedron(obj, o, spec)
+    if not isinstance(o, O.RecMag):
+        raise NotImplementedError("ObjDivMag is supported for blocks and polyhedra")
     edges = []
     for ax in range(3):
         cnt, q = spec[ax]
@@ -212,6 +220,86 @@ def ObjDivMag(obj, k, *args):
     _reg[int(obj)] = c
     _bump()
     return obj
+
+
+def _div_polyhedron(obj, o, spec):
+    V = o.verts
+    size = float(np.abs(V.max(0) - V.min(0)).max())
+    tol = 1e-10 * size
+    pieces = [(V, [list(f) for f in o.faces])]
+    for ax in range(3):
+        cnt, q = spec[ax]
+        if cnt <= 1:
+            continue
+        nrm = np.zeros(3)
+        nrm[ax] = 1.0
+        planes = O.division_planes(float(V[:, ax].min()), float(V[:, ax].max()), cnt, q)
+        nxt = []
+        for pv, pf in pieces:
+            rest = (pv, pf)
+            for c in planes:
+                below, rest = O.cut_convex(rest[0], rest[1], nrm, c, tol)
+                if below is not None:
+                    nxt.append(below)
+                if rest is None:
+                    break
+            if rest is not None:
+                nxt.append(rest)
+        pieces = nxt
+    kids = []
+    for pv, pf in pieces:
+        b = O.Polyhedron(pv, pf, o.M)
+        b.mat = o.mat
+        kids.append(_add(b))
+    c = O.Container(kids)
+    c.trfs = o.trfs
+    _reg[int(obj)] = c
+    _bump()
+    return obj
+
+
+def ObjThckPgn(xc, lx, pgn, a="x", M=(0, 0, 0)):
+    """Prism of thickness lx centred at xc along axis a over the polygon pgn,
+    whose 2D points map to (y, z) for a = 'x', (z, x) for 'y' and (x, y) for
+    'z' (radpy.cpp:179-226, radapl1.cpp:318-470).  An axis-aligned rectangle
+    becomes a block, as in the reference; other polygons must be convex."""
+    if isinstance(a, (list, tuple, np.ndarray)):
+        a, M = "x", a
+    a = str(a).lower()
+    if a not in ("x", "y", "z") or float(lx) <= 0.0:
+        raise RuntimeError("Incorrect argument: ObjThckPgn")
+    P = np.asarray(pgn, dtype=np.float64).reshape(-1, 2)
+    if len(P) < 3:
+        raise RuntimeError("Incorrect argument: ObjThckPgn, incorrect polygon definition")
+    ax = "xyz".index(a)
+    other = {0: (1, 2), 1: (2, 0), 2: (0, 1)}[ax]
+    M = _vec3(M)
+    lo, hi = float(xc) - 0.5 * float(lx), float(xc) + 0.5 * float(lx)
+    xs, ys = np.unique(P[:, 0]), np.unique(P[:, 1])
+    if len(P) == 4 and len(xs) == 2 and len(ys) == 2 and \
+            all(((p[0] == xs).any() and (p[1] == ys).any()) for p in P) and len({tuple(p) for p in P}) == 4:
+        ctr = np.zeros(3)
+        dims = np.zeros(3)
+        ctr[ax], dims[ax] = float(xc), float(lx)
+        ctr[other[0]], dims[other[0]] = 0.5 * (xs[0] + xs[1]), xs[1] - xs[0]
+        ctr[other[1]], dims[other[1]] = 0.5 * (ys[0] + ys[1]), ys[1] - ys[0]
+        return _add(O.RecMag(ctr, dims, M))
+    # convexity: every turn of the loop has the same sign
+    d = np.roll(P, -1, 0) - P
+    turn = d[:, 0] * np.roll(d, -1, 0)[:, 1] - d[:, 1] * np.roll(d, -1, 0)[:, 0]
+    if not ((turn > 0).all() or (turn < 0).all()):
+        raise NotImplementedError("ObjThckPgn: only convex polygons are supported")
+    n = len(P)
+    verts = np.zeros((2 * n, 3))
+    for k, (u, v) in enumerate(P):
+        for layer, t in ((0, lo), (1, hi)):
+            w = verts[layer * n + k]
+            w[ax], w[other[0]], w[other[1]] = t, u, v
+    faces = [list(range(n)), list(range(n, 2 * n))]
+    for i in range(n):
+        j = (i + 1) % n
+        faces.append([i, j, n + j, n + i])
+    return _add(O.Polyhedron(verts, faces, M))
 
 
 def ObjM(obj):

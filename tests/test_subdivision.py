@@ -1,0 +1,129 @@
+"""Subdivision of polyhedra (ObjDivMag) and extruded polygons (ObjThckPgn) in
+the object model, on the CPU.
+
+Reference behaviour pinned here (radvlpgn.cpp:2176-2420, radapl1.cpp:318-470,
+radpy.cpp:179-226):
+
+- the planes of each direction span the extreme vertex projections of the
+  whole polyhedron, with piece widths in geometric progression (last/first =
+  q), pieces ordered by x slice, then y, then z, and empty pieces dropped;
+- a thick polygon maps its 2D points to (y, z), (z, x) or (x, y) for the axes
+  x, y, z, and an axis-aligned rectangle becomes a block.
+
+The mathematics fixes the rest: the pieces tile the polyhedron (their volumes
+sum to its volume, each lies in its cell), and with a uniform magnetization
+the pieces' field equals the whole polyhedron's (superposition), checked with
+the oracle."""
+import numpy as np
+import pytest
+
+from oracle import oracle as OR
+from radiacuda_amd import objects as O
+from radiacuda_amd import rad
+from tests import models
+
+TET_V = [[0, 0, 0], [10, 0, 0], [0, 10, 0], [0, 0, 10]]
+TET_F = [[1, 2, 3], [1, 2, 4], [1, 3, 4], [2, 3, 4]]
+
+
+def _volume(V, F):
+    c = V.mean(0)
+    vol = 0.0
+    for f in F:
+        for k in range(1, len(f) - 1):
+            vol += abs(np.dot(V[f[0]] - c, np.cross(V[f[k]] - c, V[f[k + 1]] - c))) / 6.0
+    return vol
+
+
+def _pieces(obj):
+    return [o for (o, _k, _s) in O.walk(rad._reg, obj)]
+
+
+@pytest.fixture(autouse=True)
+def _fresh():
+    rad.UtiDelAll()
+    OR.set_randomization(False)
+    yield
+    rad.UtiDelAll()
+
+
+def test_tetrahedron_divided_3x3x3_tiles_the_volume():
+    t = rad.ObjPolyhdr(TET_V, TET_F, [0, 0, 1])
+    rad.ObjDivMag(t, [3, 3, 3])
+    ps = _pieces(t)
+    # the cells (i, j, k) of size 10/3 meeting x + y + z < 10: i + j + k <= 2
+    assert len(ps) == 10
+    assert sum(_volume(p.verts, p.faces) for p in ps) == pytest.approx(1000.0 / 6.0, rel=1e-13)
+    h = 10.0 / 3.0
+    prev = None
+    for p in ps:
+        cell = np.floor(p.verts.mean(0) / h)
+        assert (p.verts >= cell * h - 1e-9).all() and (p.verts <= (cell + 1) * h + 1e-9).all()
+        # x slice, then y, then z
+        if prev is not None:
+            assert tuple(prev) < tuple(cell)
+        prev = cell
+        assert np.array_equal(p.M, [0, 0, 1])
+
+
+def test_ratio_planes_follow_the_geometric_progression():
+    """[[4, q]]: widths w, w r, w r^2, w r^3 with r^3 = q over [0, 10]."""
+    q = 3.0
+    planes = O.division_planes(0.0, 10.0, 4, q)
+    r = q ** (1.0 / 3.0)
+    w = 10.0 / (1 + r + r * r + r ** 3)
+    assert np.allclose(planes, np.cumsum([w, w * r, w * r * r]), rtol=0, atol=1e-12)
+    assert O.division_planes(0.0, 10.0, 3, 1.0) == pytest.approx([10 / 3, 20 / 3], abs=1e-12)
+    b = rad.ObjPolyhdr([[0, 0, 0], [6, 0, 0], [6, 2, 0], [0, 2, 0], [0, 0, 2], [6, 0, 2], [6, 2, 2], [0, 2, 2]],
+                       [[1, 2, 3, 4], [5, 6, 7, 8], [1, 2, 6, 5], [2, 3, 7, 6], [3, 4, 8, 7], [4, 1, 5, 8]])
+    rad.ObjDivMag(b, [[4, q], 1, 1])
+    xs = sorted({round(float(x), 9) for p in _pieces(b) for x in p.verts[:, 0]})
+    assert np.allclose(xs[1:-1], np.array(O.division_planes(0.0, 6.0, 4, q)), atol=1e-9)
+
+
+def test_divided_polyhedron_field_equals_whole_by_superposition():
+    rng = np.random.default_rng(3)
+    V = np.array(TET_V, dtype=float) @ np.array([[1, 0.2, 0], [0, 1, 0.3], [0.1, 0, 1]]) + [1, -2, 0.5]
+    M = [0.3, -0.7, 1.1]
+    whole = rad.ObjPolyhdr(V.tolist(), TET_F, M)
+    P1, _ = models.packed(whole)
+    div = rad.ObjPolyhdr(V.tolist(), TET_F, M)
+    rad.ObjDivMag(div, [[3, 0.5], 2, [2, 2.0]])
+    P2, items = models.packed(div)
+    assert len(items) > 6
+    pts = rng.uniform(-15, 25, (200, 3))
+    B1, H1, _ = models.oracle_field(P1, np.array([M]), pts)
+    B2, H2, _ = models.oracle_field(P2, np.tile(M, (len(items), 1)), pts)
+    assert np.abs(H2 - H1).max() <= 1e-11 * np.abs(H1).max()
+    assert np.abs(B2 - B1).max() <= 1e-11 * np.abs(B1).max()
+
+
+def test_thick_polygon_axes_and_rectangle():
+    # rectangle -> block, centre and dimensions per axis mapping
+    for a, ctr, dims in (("x", [1.0, 2.5, -1.0], [4.0, 3.0, 2.0]),
+                         ("y", [-1.0, 1.0, 2.5], [2.0, 4.0, 3.0]),
+                         ("z", [2.5, -1.0, 1.0], [3.0, 2.0, 4.0])):
+        k = rad.ObjThckPgn(1.0, 4.0, [[1, -2], [4, -2], [4, 0], [1, 0]], a, [0, 0, 1])
+        o = rad._reg[k]
+        assert isinstance(o, O.RecMag)
+        assert np.allclose(o.center, ctr) and np.allclose(o.dims, dims), a
+    # triangle along y: 2D (u, v) -> (z, x)
+    k = rad.ObjThckPgn(2.0, 2.0, [[0, 0], [3, 0], [0, 4]], "y", [0.1, 0.2, 0.9])
+    o = rad._reg[k]
+    assert isinstance(o, O.Polyhedron)
+    want = {(0, 1, 0), (0, 1, 3), (4, 1, 0), (0, 3, 0), (0, 3, 3), (4, 3, 0)}
+    assert {tuple(np.round(v, 12)) for v in o.verts} == want
+    assert _volume(o.verts, o.faces) == pytest.approx(12.0, rel=1e-14)
+    # its field equals the same prism given as ObjPolyhdr
+    P1, _ = models.packed(k)
+    q = rad.ObjPolyhdr([[0, 1, 0], [0, 1, 3], [4, 1, 0], [0, 3, 0], [0, 3, 3], [4, 3, 0]],
+                       [[1, 2, 3], [4, 5, 6], [1, 2, 5, 4], [2, 3, 6, 5], [3, 1, 4, 6]], [0.1, 0.2, 0.9])
+    P2, _ = models.packed(q)
+    pts = np.random.default_rng(1).uniform(-6, 9, (60, 3))
+    B1, _h, _ = models.oracle_field(P1, np.array([[0.1, 0.2, 0.9]]), pts)
+    B2, _h, _ = models.oracle_field(P2, np.array([[0.1, 0.2, 0.9]]), pts)
+    assert np.abs(B1 - B2).max() <= 1e-12 * np.abs(B2).max()
+    with pytest.raises(NotImplementedError):
+        rad.ObjThckPgn(0.0, 1.0, [[0, 0], [4, 0], [1, 1], [0, 4]], "z")
+    with pytest.raises(RuntimeError):
+        rad.ObjThckPgn(0.0, -1.0, [[0, 0], [1, 0], [0, 1]])
