@@ -222,6 +222,128 @@ def ObjDivMag(obj, k, *args):
     return obj
 
 
+def ObjFullMag(P, L, M, K, grp, mat, rgb=(0, 0, 0)):
+    """Block with centre P, dimensions L and magnetization M, subdivided by K,
+    given material mat (0: none) and added to container grp (radpy.cpp:612)."""
+    k = ObjRecMag(P, L, M)
+    ObjDivMag(k, K)
+    if int(mat) != 0:
+        MatApl(k, mat)
+    g = _obj(grp)
+    if not isinstance(g, O.Container):
+        raise RuntimeError("Radia::Error016: not a container")
+    g.members.append(int(k))
+    _bump()
+    return k
+
+
+def ObjMltExtRtg(slices, M=(0, 0, 0)):
+    """Polyhedra between consecutive rectangles [[centre(3), [wx, wy]], ...]
+    in planes normal to z (radpy.cpp:428-500): ObjMltExtPgn with the
+    rectangles as slices."""
+    if len(slices) < 2:
+        raise RuntimeError("Incorrect argument: ObjMltExtRtg, number of slices / layers should be >= 2")
+    pg = []
+    for (c, d) in slices:
+        c = _vec3(c)
+        hx, hy = 0.5 * float(d[0]), 0.5 * float(d[1])
+        pg.append([[[c[0] - hx, c[1] - hy], [c[0] + hx, c[1] - hy], [c[0] + hx, c[1] + hy],
+                    [c[0] - hx, c[1] + hy]], c[2]])
+    return ObjMltExtPgn(pg, M)
+
+
+def ObjDegFre(obj):
+    """Degrees of freedom of the relaxation: 3 per magnetizable element
+    (radpy.cpp:1396)."""
+    n = 0
+    for o, _k, _s in O.walk(_reg, int(obj)):
+        if isinstance(o, (O.RecMag, O.Polyhedron)) and not getattr(o, "is_current", False):
+            n += 3
+    return n
+
+
+def ObjDrwAtr(obj, rgb=(0, 0, 0), thcn=0.001):
+    """Drawing attributes: graphics are not part of this build, so this only
+    checks the object and returns it."""
+    _obj(obj)
+    return obj
+
+
+def ObjCutMag(obj, P, N, opt=""):
+    """Cuts obj by the plane through P with normal N into new objects on its
+    lower and upper side; returns their keys (the object itself when the
+    plane misses it) and leaves obj as it was (radapl3.cpp:1347-1420).  In
+    a container, members wholly on one side join that side's container as
+    they are.  The
+    plane is taken in the frame of the object's own coordinates
+    ('Frame->Loc'); blocks are cut as polyhedra."""
+    if opt and "loc" not in str(opt).lower():
+        raise NotImplementedError("ObjCutMag: only 'Frame->Loc' is supported")
+    nrm = np.asarray(_vec3(N), dtype=np.float64)
+    if not np.linalg.norm(nrm) > 0.0:
+        raise RuntimeError("Incorrect argument: ObjCutMag")
+    nrm = nrm / np.linalg.norm(nrm)
+    off = float(nrm @ np.asarray(_vec3(P)))
+    lo, hi = _cut(int(obj), nrm, off)
+    if lo is None or hi is None:
+        return [int(obj)]
+    return [lo, hi]
+
+
+def _as_polyhedron(o):
+    if isinstance(o, O.Polyhedron):
+        return o.verts, o.faces
+    c, h = np.asarray(o.center), 0.5 * np.asarray(o.dims)
+    V = np.array([[c[0] + sx * h[0], c[1] + sy * h[1], c[2] + sz * h[2]]
+                  for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)])
+    return V, HEX_FACES0
+
+
+HEX_FACES0 = [[0, 1, 3, 2], [4, 5, 7, 6], [0, 1, 5, 4], [2, 3, 7, 6], [0, 2, 6, 4], [1, 3, 7, 5]]
+
+
+def _cut(key, nrm, off):
+    o = _obj(key)
+    if isinstance(o, O.Container):
+        los, his = [], []
+        for m in o.members:
+            a, b = _cut(int(m), nrm, off)
+            if a is not None:
+                los.append(a)
+            if b is not None:
+                his.append(b)
+        if not his:
+            return key, None  # wholly on one side: the container itself
+        if not los:
+            return None, key
+        out = []
+        for part in (los, his):
+            c = O.Container(part)
+            c.trfs = list(o.trfs)
+            out.append(_add(c))
+        return out[0], out[1]
+    if not isinstance(o, (O.RecMag, O.Polyhedron)) or getattr(o, "is_current", False):
+        raise NotImplementedError("ObjCutMag: only magnetized blocks and polyhedra are cut")
+    V, F = _as_polyhedron(o)
+    size = float(np.abs(V.max(0) - V.min(0)).max())
+    below, above = O.cut_convex(V, F, nrm, off, 1e-10 * size)
+    if above is None:
+        return key, None  # wholly on one side: the object itself
+    if below is None:
+        return None, key
+    keys = []
+    for piece in (below, above):
+        if piece is None:
+            keys.append(None)
+            continue
+        b = O.Polyhedron(piece[0], piece[1], o.M)
+        b.mat = o.mat
+        b.trfs = list(o.trfs)
+        keys.append(_add(b))
+    _bump()
+    return keys[0], keys[1]
+
+
 def _div_polyhedron(obj, o, spec):
     V = o.verts
     size = float(np.abs(V.max(0) - V.min(0)).max())
@@ -414,6 +536,35 @@ def MatSatIsoTab(points):
     return _add(O.Material(O.MAT_NONLIN_TAB, x=x, y=y, d=d))
 
 
+# Radia's standard materials (radmater.cpp:714-731): linear anisotropic
+# (ksi_par, ksi_perp, Mr) and tanh-sum saturating laws (ms_1..3, ksi_1..3).
+_STD_MATERIALS = {
+    "NDFEB": ("lin", (0.06, 0.17, 1.2)),
+    "SMCO5": ("lin", (0.005, 0.04, 0.85)),
+    "SM2CO17": ("lin", (0.005, 0.04, 1.05)),
+    "FERRITE": ("lin", (0.07, 0.2, 0.35)),
+    "XC06": ("sat", ((1.362, 0.2605, 0.4917), (2118.0, 63.06, 17.138))),
+    "STEEL37": ("sat", ((1.1488, 0.4268, 0.4759), (1596.3, 133.11, 18.713))),
+    "STEEL42": ("sat", ((1.441, 0.2912, 0.3316), (968.66, 24.65, 8.3))),
+    "AFK502": ("sat", ((1.788, 0.437, 0.115), (10485.0, 241.5, 7.43))),
+    "AFK1": ("sat", ((1.704, 0.493, 0.152), (2001.0, 38.56, 1.24))),
+}
+
+
+def MatStd(name, mr=0.0):
+    """Standard material by name (case-insensitive); for the permanent
+    magnets mr > 0 replaces the remanence (radpy.cpp:1900, radmater.cpp:733-770)."""
+    ent = _STD_MATERIALS.get(str(name).upper())
+    if ent is None:
+        raise RuntimeError("Radia::Error008: unknown material name")
+    kind, par = ent
+    if kind == "lin":
+        kpar, kper, br = par
+        return MatLin([kpar, kper], float(mr) if float(mr) > 0.0 else br)
+    ms, ks = par
+    return MatSatIsoFrm(*[[ks[i], ms[i]] for i in range(3)])
+
+
 def MatApl(obj, mat):
     m = _obj(mat)
     if not isinstance(m, O.Material):
@@ -558,6 +709,23 @@ def RlxAuto(intrc, prec, maxiter, meth=4, *opts):
     return [res.misfit, res.max_m, res.max_h, res.iters]
 
 
+def RlxMan(intrc, meth, iters, rlx=0.0):
+    """Runs iters iterations of the relaxation from the current
+    magnetization and returns [misfit, max|M|, max|H|]
+    (radpy.cpp:2271-2296).  Every method number runs the device
+    Anderson-accelerated block Jacobi (its first step under-relaxed by rlx
+    when 0 < rlx <= 1), so the state after a given number of iterations is
+    not the reference's method-by-method trajectory; the fixed point is the
+    same.  The iterations work on the stored matrix without the FP64
+    refinement pass."""
+    if int(iters) < 1:
+        raise RuntimeError("Incorrect argument: RlxMan")
+    w = float(rlx)
+    opts = ("ZeroM->False", "Refine->-1") + (("omega->%r" % w,) if 0.0 < w <= 1.0 else ())
+    r = RlxAuto(intrc, 0.0, int(iters), 4, *opts)
+    return r[:3]
+
+
 def Solve(obj, prec, maxiter, meth=4):
     """RlxPre + RlxAuto (radentry.cpp:1842-1855)."""
     k = RlxPre(obj)
@@ -632,6 +800,42 @@ def Fld(obj, comp, P, use_gpu=True, precision="double"):
     src = B if comp[0] == "b" else H
     res = [_component(v, comp) for v in src]
     return res[0] if single else res
+
+
+def FldLst(obj, comp, P1, P2, np_, arg="arg", start=0.0):
+    """Field along the segment P1 -> P2 at np_ equally spaced points (np_ = 1
+    with a finite P2 means 101), each preceded by its longitudinal position
+    start + i*step when arg is 'arg' (radapl3.cpp:34-100); returned as the
+    reference's Python layer shapes it (a flat list when each point has one
+    value)."""
+    a = str(arg).lower()
+    if a not in ("arg", "noarg"):
+        raise RuntimeError("Radia::Error034")
+    p1 = np.asarray(_vec3(P1), dtype=np.float64)
+    p2 = np.asarray(_vec3(P2), dtype=np.float64)
+    n = int(np_)
+    if n <= 0:
+        raise RuntimeError("Incorrect argument: FldLst")
+    if n == 1 and (p2 < 1e22).all():
+        n = 101
+    pts = [p1.copy()]
+    argv = [float(start)]
+    if n > 1:
+        step = (1.0 / float(n - 1)) * (p2 - p1)
+        ds = float(np.sqrt(step @ step))
+        for _ in range(1, n):
+            pts.append(pts[-1] + step)
+            argv.append(argv[-1] + ds)
+    vals = Fld(obj, comp, np.array(pts))
+    rows = []
+    for i, v in enumerate(vals):
+        row = [argv[i]] if a == "arg" else []
+        row += list(v) if isinstance(v, list) else [v]
+        rows.append(row)
+    if n == 1:
+        flat = rows[0]
+        return flat[0] if len(flat) == 1 else flat
+    return [r[0] if len(r) == 1 else r for r in rows]
 
 
 # ------------------------------------------------------------------ utilities

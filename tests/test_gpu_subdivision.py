@@ -52,3 +52,45 @@ def test_divided_tetrahedron_and_thick_polygon_relax():
     Bo, _Ho, _bad = models.oracle_field(P, M, pts)
     assert np.abs(B - Bo).max() <= 1e-10 * np.abs(Bo).max()
     rad.UtiDel(k)
+
+
+def test_field_list_and_manual_relaxation():
+    print("case: FldLst, RlxMan, MatStd on an iron cube", flush=True)
+    rad.UtiDelAll()
+    OR.set_randomization(False)
+    g = rad.ObjCnt([])
+    rad.ObjFullMag([0, 0, 0], [20, 20, 20], [0, 0, 0], [2, 2, 2], g, rad.MatStd("Steel37"), [0, 0, 1])
+    coil = rad.ObjRecCur([0, 0, 25], [30, 30, 6], [0, 0, 3.0])
+    top = rad.ObjCnt([g, coil])
+    # manual relaxation, a few iterations at a time, reaches RlxAuto's state
+    k = rad.RlxPre(top)
+    for _ in range(100):
+        r = rad.RlxMan(k, 4, 10, 0.5)
+        assert len(r) == 3
+        if r[0] <= 1e-11:
+            break
+    Mman = np.array([m for (_c, m) in rad.ObjM(g)])
+    rad.UtiDel(k)
+    k2 = rad.RlxPre(top)
+    ra = rad.RlxAuto(k2, 1e-11, 2000)
+    Maut = np.array([m for (_c, m) in rad.ObjM(g)])
+    rad.UtiDel(k2)
+    print("RlxMan misfit %.2e, RlxAuto %s, |dM| %.2e" % (r[0], ra, np.abs(Mman - Maut).max()), flush=True)
+    assert r[0] <= 1e-11
+    # RlxMan iterates on the FP32-stored matrix without the FP64 refinement
+    # that RlxAuto applies: the two fixed points differ at FP32 storage level
+    assert np.abs(Mman - Maut).max() <= 1e-6 * np.abs(Maut).max()
+    # field lists
+    p1, p2 = [-30.0, 1.0, 2.0], [30.0, -1.0, 4.0]
+    pts = [np.array(p1) + i * (np.array(p2) - np.array(p1)) / 10.0 for i in range(11)]
+    bz = rad.Fld(top, "bz", [p.tolist() for p in pts])
+    rows = rad.FldLst(top, "bz", p1, p2, 11, "arg", 2.0)
+    step = np.linalg.norm(np.array(p2) - np.array(p1)) / 10.0
+    assert len(rows) == 11 and all(len(rw) == 2 for rw in rows)
+    assert np.allclose([rw[0] for rw in rows], 2.0 + step * np.arange(11), rtol=0, atol=1e-12)
+    assert np.allclose([rw[1] for rw in rows], bz, rtol=1e-13, atol=1e-15)
+    flat = rad.FldLst(top, "bz", p1, p2, 11, "noarg")
+    assert len(flat) == 11 and all(isinstance(v, float) for v in flat)
+    b3 = rad.FldLst(top, "b", p1, p2, 11, "noarg")
+    assert len(b3) == 11 and all(len(v) == 3 for v in b3)
+    assert len(rad.FldLst(top, "bz", p1, p2, 1, "noarg")) == 101

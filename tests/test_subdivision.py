@@ -127,3 +127,57 @@ def test_thick_polygon_axes_and_rectangle():
         rad.ObjThckPgn(0.0, 1.0, [[0, 0], [4, 0], [1, 1], [0, 4]], "z")
     with pytest.raises(RuntimeError):
         rad.ObjThckPgn(0.0, -1.0, [[0, 0], [1, 0], [0, 1]])
+
+
+def test_cut_by_a_slanted_plane_keeps_the_object_and_tiles_it():
+    V = np.array(TET_V, dtype=float)
+    t = rad.ObjPolyhdr(TET_V, TET_F, [0.2, 0.1, 1.0])
+    b = rad.ObjRecMag([20, 0, 0], [4, 6, 8], [0, 0, 1])
+    top = rad.ObjCnt([t, b])
+    lo, hi = rad.ObjCutMag(top, [0, 1, 0], [-0.1, 1, 0.05])
+    assert isinstance(rad._reg[t], O.Polyhedron)  # the object itself is unchanged
+    pl, ph = _pieces(lo), _pieces(hi)
+    assert len(pl) == 2 and len(ph) == 2
+    n = np.array([-0.1, 1, 0.05]) / np.linalg.norm([-0.1, 1, 0.05])
+    off = n @ np.array([0.0, 1.0, 0.0])
+    for p in pl:
+        assert (p.verts @ n <= off + 1e-9).all()
+    for p in ph:
+        assert (p.verts @ n >= off - 1e-9).all()
+    vol = sum(_volume(p.verts, p.faces) for p in pl + ph)
+    assert vol == pytest.approx(1000.0 / 6.0 + 4 * 6 * 8, rel=1e-13)
+    # superposition: the two halves give the whole field
+    P0, _ = models.packed(top)
+    Pc, items = models.packed(rad.ObjCnt([lo, hi]))
+    pts = np.random.default_rng(2).uniform(-10, 30, (100, 3))
+    M0 = np.array([[0.2, 0.1, 1.0], [0, 0, 1]])
+    Mc = np.stack([o.M for (o, _k, _s) in items])
+    B0, _h, _ = models.oracle_field(P0, M0, pts)
+    Bc, _h, _ = models.oracle_field(Pc, Mc, pts)
+    assert np.abs(Bc - B0).max() <= 1e-11 * np.abs(B0).max()
+    # a plane that misses the object returns the object
+    assert rad.ObjCutMag(t, [50, 0, 0], [1, 0, 0]) == [t]
+
+
+def test_standard_materials_and_block_helpers():
+    m = rad._reg[rad.MatStd("NdFeB")]
+    m2 = rad._reg[rad.MatLin([0.06, 0.17], 1.2)]
+    assert m.record([0, 0, 1]).tolist() == m2.record([0, 0, 1]).tolist()
+    m3 = rad._reg[rad.MatStd("ndfeb", 1.3)]
+    assert m3.record([0, 0, 1]).tolist() == rad._reg[rad.MatLin([0.06, 0.17], 1.3)].record([0, 0, 1]).tolist()
+    x = rad._reg[rad.MatStd("Xc06")]
+    y = rad._reg[rad.MatSatIsoFrm([2118, 1.362], [63.06, 0.2605], [17.138, 0.4917])]
+    assert x.record([0, 0, 0]).tolist() == y.record([0, 0, 0]).tolist()
+    with pytest.raises(RuntimeError):
+        rad.MatStd("unobtainium")
+    g = rad.ObjCnt([])
+    k = rad.ObjFullMag([0, 0, 0], [10, 20, 30], [0, 0, 1], [2, 1, 3], g, rad.MatStd("Steel37"), [1, 0, 0])
+    assert rad.ObjCntStuf(g) == [k]
+    assert len(_pieces(k)) == 6 and all(p.mat is not None for p in _pieces(k))
+    assert rad.ObjDegFre(g) == 18
+    r = rad.ObjMltExtRtg([[[0, 0, 0], [4, 2]], [[1, 0, 5], [2, 2]], [[1, 1, 9], [2, 4]]], [0, 1, 0])
+    ps = _pieces(r)
+    assert len(ps) == 2
+    # prismatoid between 4x2 and 2x2 rectangles 5 apart: h/6 (A1 + A2 + 4 A_mid), A_mid = 3 x 2
+    assert _volume(ps[0].verts, ps[0].faces) == pytest.approx(5.0 / 6.0 * (8 + 4 + 4 * 6), rel=1e-12)
+    assert rad.ObjDrwAtr(r, [1, 0, 0], 0.001) == r
