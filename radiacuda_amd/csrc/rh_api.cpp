@@ -620,6 +620,7 @@ namespace {
 // field of all other elements (the fixed point of radGPU_RelaxAuto,
 // cpp/src/core/radgpurlx.cu:799-1349, with an acceleration of our own).
 constexpr int kHist = 8;
+static_assert(kHist <= rh::kHistMax, "the fused relaxation step holds at most kHistMax differences");
 
 struct RelaxWork {
     double* x = nullptr;   // [ld] current iterate (padded)
@@ -727,54 +728,90 @@ rh_status relax_run(rh_matrix* q, const double* d_mats, const double* d_tab, con
             RH_HIP(matrix_pass(q, w.x, w.y, s));
         }
         tm.end();
-        if (adj)
-            RH_HIP(rh::launch_adj_update(n, d_mats, d_hext, q->d_qdiag, w.y, w.x, w.F, w.part, s));
-        else
-            RH_HIP(rh::launch_relax_update(n, d_mats, d_tab, d_hext, q->d_qdiag, w.y, w.x, w.F, d_H, w.part,
-                                           s));
-        // tentatively extend the history with the newest differences
-        if (it > 1) {
-            const double* v1[2] = {w.F, w.Fp};
-            const double c1[2] = {1.0, -1.0};
-            RH_HIP(rh::launch_lincomb(2, v1, c1, n3, w.dF[head], s));
-            const double* v2[2] = {w.x, w.xp};
-            RH_HIP(rh::launch_lincomb(2, v2, c1, n3, w.dX[head], s));
-            head = (head + 1) % kHist;
-            hist = std::min(hist + 1, kHist);
-        }
-        RH_HIP(hipMemcpyAsync(w.Fp, w.F, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
-        RH_HIP(hipMemcpyAsync(w.xp, w.x, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        // dv: the dot products of the newest difference with every stored
+        // one, then of every stored difference with F (oldest first)
         int slot[kHist];
-        for (int k = 0; k < hist; k++) slot[k] = (head - hist + k + kHist) % kHist;
-        int nd = 0;
-        if (hist > 0) {
-            // dots: newest dF with every dF, and every dF with F
-            const double* a[17];
-            const double* b[17];
-            const int newest = slot[hist - 1];
-            for (int k = 0; k < hist; k++) {
-                a[nd] = w.dF[newest];
-                b[nd] = w.dF[slot[k]];
-                nd++;
-            }
-            for (int k = 0; k < hist; k++) {
-                a[nd] = w.dF[slot[k]];
-                b[nd] = w.F;
-                nd++;
-            }
-            RH_HIP(rh::launch_dots(nd, a, b, n3, w.dots, s));
-            RH_HIP(hipMemcpyAsync(hbuf + 3 * nb, w.dots, size_t(nd * db) * sizeof(double),
-                                  hipMemcpyDeviceToHost, s));
-        }
-        RH_HIP(hipMemcpyAsync(hbuf, w.part, size_t(3 * nb) * sizeof(double), hipMemcpyDeviceToHost, s));
-        RH_HIP(hipStreamSynchronize(s));
+        std::vector<double> dv;
         double f2 = 0.0;
         maxM = 1e-17;
         maxH = 1e-17;
-        for (int64_t bb = 0; bb < nb; bb++) {
-            f2 += hbuf[3 * bb];
-            maxM = std::max(maxM, hbuf[3 * bb + 1]);
-            maxH = std::max(maxH, hbuf[3 * bb + 2]);
+        if (adj) {
+            RH_HIP(rh::launch_adj_update(n, d_mats, d_hext, q->d_qdiag, w.y, w.x, w.F, w.part, s));
+            // tentatively extend the history with the newest differences
+            if (it > 1) {
+                const double* v1[2] = {w.F, w.Fp};
+                const double c1[2] = {1.0, -1.0};
+                RH_HIP(rh::launch_lincomb(2, v1, c1, n3, w.dF[head], s));
+                const double* v2[2] = {w.x, w.xp};
+                RH_HIP(rh::launch_lincomb(2, v2, c1, n3, w.dX[head], s));
+                head = (head + 1) % kHist;
+                hist = std::min(hist + 1, kHist);
+            }
+            RH_HIP(hipMemcpyAsync(w.Fp, w.F, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+            RH_HIP(hipMemcpyAsync(w.xp, w.x, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
+            for (int k = 0; k < hist; k++) slot[k] = (head - hist + k + kHist) % kHist;
+            int nd = 0;
+            if (hist > 0) {
+                const double* a[17];
+                const double* b[17];
+                const int newest = slot[hist - 1];
+                for (int k = 0; k < hist; k++) {
+                    a[nd] = w.dF[newest];
+                    b[nd] = w.dF[slot[k]];
+                    nd++;
+                }
+                for (int k = 0; k < hist; k++) {
+                    a[nd] = w.dF[slot[k]];
+                    b[nd] = w.F;
+                    nd++;
+                }
+                RH_HIP(rh::launch_dots(nd, a, b, n3, w.dots, s));
+                RH_HIP(hipMemcpyAsync(hbuf + 3 * nb, w.dots, size_t(nd * db) * sizeof(double),
+                                      hipMemcpyDeviceToHost, s));
+            }
+            RH_HIP(hipMemcpyAsync(hbuf, w.part, size_t(3 * nb) * sizeof(double), hipMemcpyDeviceToHost, s));
+            RH_HIP(hipStreamSynchronize(s));
+            for (int64_t bb = 0; bb < nb; bb++) {
+                f2 += hbuf[3 * bb];
+                maxM = std::max(maxM, hbuf[3 * bb + 1]);
+                maxH = std::max(maxH, hbuf[3 * bb + 2]);
+            }
+            dv.assign(size_t(nd), 0.0);
+            for (int d = 0; d < nd; d++)
+                for (int64_t bb = 0; bb < db; bb++) dv[size_t(d)] += hbuf[3 * nb + d * db + bb];
+        } else {
+            // the update, the history differences, the copies and the dot
+            // products in one pass
+            rh::HistPack hp;
+            hp.Fp = w.Fp;
+            hp.xp = w.xp;
+            if (it > 1) {
+                hp.dFw = w.dF[head];
+                hp.dXw = w.dX[head];
+                head = (head + 1) % kHist;
+                hist = std::min(hist + 1, kHist);
+            }
+            for (int k = 0; k < hist; k++) {
+                slot[k] = (head - hist + k + kHist) % kHist;
+                hp.dF[k] = w.dF[slot[k]];
+            }
+            hp.nh = hist;
+            RH_HIP(rh::launch_relax_step(n, d_mats, d_tab, d_hext, q->d_qdiag, w.y, w.x, w.F, d_H, w.part, hp,
+                                         s));
+            RH_HIP(hipMemcpyAsync(hbuf, w.part, size_t(rh::kRelaxPart * nb) * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+            RH_HIP(hipStreamSynchronize(s));
+            dv.assign(size_t(2 * hist), 0.0);
+            for (int64_t bb = 0; bb < nb; bb++) {
+                const double* o = hbuf + rh::kRelaxPart * bb;
+                f2 += o[0];
+                maxM = std::max(maxM, o[1]);
+                maxH = std::max(maxH, o[2]);
+                for (int k = 0; k < hist; k++) {
+                    dv[size_t(k)] += o[3 + k];
+                    dv[size_t(hist + k)] += o[3 + rh::kHistMax + k];
+                }
+            }
         }
         misfit = std::sqrt(f2 / double(n));
         if (!std::isfinite(misfit)) return fail(RH_ERR_NOTCONV, "relaxation produced a non-finite residual");
@@ -797,9 +834,6 @@ rh_status relax_run(rh_matrix* q, const double* d_mats, const double* d_tab, con
         best = std::min(best, misfit);
         std::vector<double> gam;
         if (!plain) {
-            std::vector<double> dv(size_t(nd), 0.0);
-            for (int d = 0; d < nd; d++)
-                for (int64_t bb = 0; bb < db; bb++) dv[size_t(d)] += hbuf[3 * nb + d * db + bb];
             const int newest = slot[hist - 1];
             for (int k = 0; k < hist; k++) {
                 G[size_t(newest * kHist + slot[k])] = dv[size_t(k)];
@@ -1185,7 +1219,7 @@ rh_status rh_relax_tab_dev(rh_matrix* q, const double* d_mats, const double* d_t
     }
     const int64_t nb = rh::relax_blocks(n);
     const int64_t db = rh::dot_blocks(n3);
-    RH_HIP(dalloc(&w.part, size_t(3 * nb)));
+    RH_HIP(dalloc(&w.part, size_t(rh::kRelaxPart * nb)));
     RH_HIP(dalloc(&w.dots, size_t(17 * db)));
     RH_HIP(hipMemsetAsync(w.x, 0, size_t(q->ld) * sizeof(double), s));
     // d_H starts the per-element nonlinear solves (previous local field); a
@@ -1201,7 +1235,7 @@ rh_status rh_relax_tab_dev(rh_matrix* q, const double* d_mats, const double* d_t
     }
     // one pinned read-back per iteration: residual partials, then dot partials
     double* hbuf = nullptr;
-    RH_HIP(hipHostMalloc(reinterpret_cast<void**>(&hbuf), size_t(3 * nb + 17 * db) * sizeof(double)));
+    RH_HIP(hipHostMalloc(reinterpret_cast<void**>(&hbuf), size_t(rh::kRelaxPart * nb + 17 * db) * sizeof(double)));
     struct PinnedFree {
         double* p;
         ~PinnedFree() { (void)hipHostFree(p); }
@@ -1534,7 +1568,7 @@ rh_status rh_relax_adjoint_dev(rh_matrix* q, const double* d_mats, const double*
     }
     const int64_t nb = rh::relax_blocks(n);
     const int64_t db = rh::dot_blocks(n3);
-    RH_HIP(dalloc(&w.part, size_t(3 * nb)));
+    RH_HIP(dalloc(&w.part, size_t(rh::kRelaxPart * nb)));
     RH_HIP(dalloc(&w.dots, size_t(17 * db)));
     AdjointPass ap;
     ap.chunks = rh::matvec_t_chunks(n3, q->ld);
@@ -1553,7 +1587,7 @@ rh_status rh_relax_adjoint_dev(rh_matrix* q, const double* d_mats, const double*
     if (opts->keep_m)
         RH_HIP(hipMemcpyAsync(w.x, d_lam, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
     double* hbuf = nullptr;
-    RH_HIP(hipHostMalloc(reinterpret_cast<void**>(&hbuf), size_t(3 * nb + 17 * db) * sizeof(double)));
+    RH_HIP(hipHostMalloc(reinterpret_cast<void**>(&hbuf), size_t(rh::kRelaxPart * nb + 17 * db) * sizeof(double)));
     struct PinnedFree {
         double* p;
         ~PinnedFree() { (void)hipHostFree(p); }

@@ -1211,6 +1211,86 @@ __global__ __launch_bounds__(256) void k_lincomb(int nv, CombPack pk, int64_t le
     }
 }
 
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// One relaxation step with its Anderson bookkeeping (one wave per block).
+static_assert(kRelaxLanes == 64, "k_relax_step reduces over one wave per block");
+__global__ __launch_bounds__(kRelaxLanes) void k_relax_step(int64_t n, const double* __restrict__ mats,
+                                                            const double* __restrict__ tab,
+                                                            const double* __restrict__ hext,
+                                                            const double* __restrict__ qdiag,
+                                                            const double* __restrict__ y,
+                                                            const double* __restrict__ M,
+                                                            double* __restrict__ F,
+                                                            double* __restrict__ Hout,
+                                                            double* __restrict__ partials, HistPack hp)
+{
+    const int64_t i = int64_t(blockIdx.x) * kRelaxLanes + threadIdx.x;
+    double f2 = 0.0, mm = 0.0, hh = 0.0;
+    double dd[2 * kHistMax];
+#pragma unroll
+    for (int k = 0; k < 2 * kHistMax; k++) dd[k] = 0.0;
+    if (i < n) {
+        const double heff[3] = {y[3 * i] + hext[3 * i], y[3 * i + 1] + hext[3 * i + 1],
+                                y[3 * i + 2] + hext[3 * i + 2]};
+        const double warm = sqrt(Hout[3 * i] * Hout[3 * i] + Hout[3 * i + 1] * Hout[3 * i + 1] +
+                                 Hout[3 * i + 2] * Hout[3 * i + 2]);
+        double mp[3], hl[3];
+        local_solve(mats + 16 * i, qdiag + 9 * i, heff, mp, hl, tab, warm);
+        double fi[3], dfn[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < 3; k++) {
+            const int64_t j = 3 * i + k;
+            const double xk = M[j];
+            const double d = mp[k] - xk;
+            fi[k] = d;
+            F[j] = d;
+            Hout[j] = hl[k];
+            f2 += d * d;
+            if (hp.dFw) {
+                dfn[k] = d - hp.Fp[j];
+                hp.dFw[j] = dfn[k];
+                hp.dXw[j] = xk - hp.xp[j];
+            }
+            hp.Fp[j] = d;
+            hp.xp[j] = xk;
+        }
+        mm = sqrt(mp[0] * mp[0] + mp[1] * mp[1] + mp[2] * mp[2]);
+        hh = sqrt(hl[0] * hl[0] + hl[1] * hl[1] + hl[2] * hl[2]);
+#pragma unroll
+        for (int h = 0; h < kHistMax; h++) {
+            if (h < hp.nh) {
+                double a = 0.0, b = 0.0;
+                for (int k = 0; k < 3; k++) {
+                    const double v = (h == hp.nh - 1) ? dfn[k] : hp.dF[h][3 * i + k];
+                    a += dfn[k] * v;
+                    b += v * fi[k];
+                }
+                dd[h] = a;
+                dd[kHistMax + h] = b;
+            }
+        }
+    }
+    f2 = wave_sum(f2);
+    mm = wave_max(mm);
+    hh = wave_max(hh);
+#pragma unroll
+    for (int k = 0; k < 2 * kHistMax; k++)
+        if ((k % kHistMax) < hp.nh) dd[k] = wave_sum(dd[k]);
+    if (threadIdx.x == 0) {
+        double* o = partials + int64_t(kRelaxPart) * blockIdx.x;
+        o[0] = f2;
+        o[1] = mm;
+        o[2] = hh;
+#pragma unroll
+        for (int k = 0; k < 2 * kHistMax; k++) o[3 + k] = dd[k];
+    }
+}
+
 // ---------------------------------------------------------------- currents
 __global__ __launch_bounds__(256) void k_current(int64_t ns, const int32_t* __restrict__ kind,
                                                  const double* __restrict__ par,
@@ -1383,6 +1463,16 @@ hipError_t launch_relax_update(int64_t n, const double* mats, const double* tab,
 {
     hipLaunchKernelGGL(k_relax_update, dim3(unsigned(relax_blocks(n))), dim3(kRelaxLanes), 0, s, n, mats,
                        tab, hext, qdiag, y, M, F, Hout, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_relax_step(int64_t n, const double* mats, const double* tab, const double* hext,
+                             const double* qdiag, const double* y, const double* M, double* F,
+                             double* Hout, double* partials, const HistPack& hp, hipStream_t s)
+{
+    if (hp.nh < 0 || hp.nh > kHistMax || (hp.nh > 0 && !hp.dFw)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_relax_step, dim3(unsigned(relax_blocks(n))), dim3(kRelaxLanes), 0, s, n, mats, tab,
+                       hext, qdiag, y, M, F, Hout, partials, hp);
     return hipGetLastError();
 }
 

@@ -82,6 +82,26 @@ hipError_t launch_relax_update(int64_t n, const double* mats, const double* tab,
                                const double* qdiag, const double* y, const double* M, double* F,
                                double* Hout, double* partials, hipStream_t s);
 
+// The relaxation step with the Anderson bookkeeping in the same pass: the
+// update above, then (when dFw is set) dFw = F - Fp and dXw = M - xp, then
+// Fp = F and xp = M, and the dot products of the newest difference (dF[nh-1],
+// which is dFw) with every dF[k] and of every dF[k] with F.  partials holds
+// kRelaxPart doubles per block: [sum |F|^2, max |M_prop|, max |H|, nh dots
+// with the newest difference, padding to kHistMax, nh dots with F, padding].
+constexpr int kHistMax = 8;
+constexpr int kRelaxPart = 3 + 2 * kHistMax;
+struct HistPack {
+    double* Fp = nullptr;
+    double* xp = nullptr;
+    double* dFw = nullptr;  // difference slot written in this pass (null: none)
+    double* dXw = nullptr;
+    const double* dF[kHistMax] = {nullptr};  // oldest first; dF[nh-1] == dFw
+    int nh = 0;
+};
+hipError_t launch_relax_step(int64_t n, const double* mats, const double* tab, const double* hext,
+                             const double* qdiag, const double* y, const double* M, double* F,
+                             double* Hout, double* partials, const HistPack& hp, hipStream_t s);
+
 // out = hext + rowT*H64 - y - Qii*M per element (rowt may be null = identity).
 hipError_t launch_refine_defect(int64_t n, const double* H64, const double* rowt, const double* y,
                                 const double* qdiag, const double* M, const double* hext,
