@@ -120,7 +120,7 @@ RH_HD double atan_pair(double x, double y)
     double d = 1.0 - x * y;
     if (d == 0.0) d = 1e-50;
     double k = (d > 0) ? 0.0 : ((x < 0) ? -1.0 : 1.0);
-    return rh_atan2((x + y) / d, 1.0) + 3.141592653589793238 * k;
+    return rh_atan2((x + y) * rh_rcp(d), 1.0) + 3.141592653589793238 * k;
 }
 
 RH_HD void arc_current_B(const double* par, const double P[3], double B[3])
@@ -154,8 +154,8 @@ RH_HD void arc_current_B(const double* par, const double P[3], double B[3])
         if (b11 < zero_tol) b11 = zero_tol;
         double b21 = fabs(a21 - tr2 * cd);
         if (b21 < zero_tol) b21 = zero_tol;
-        const double D11 = sqrt(b11), D12 = sqrt(fabs(a12 - tr1 * cd));
-        const double D21 = sqrt(b21), D22 = sqrt(fabs(a22 - tr2 * cd));
+        const double D11 = rh_sqrt(b11), D12 = rh_sqrt(fabs(a12 - tr1 * cd));
+        const double D21 = rh_sqrt(b21), D22 = rh_sqrt(fabs(a22 - tr2 * cd));
         const double rc = r * cd;
         const double u1 = rmin - rc, u2 = rmax - rc;
         const double q = z1z1 + rr * as * as;
@@ -166,10 +166,11 @@ RH_HD void arc_current_B(const double* par, const double P[3], double B[3])
         if (fabs(q22) < D22 * rel && q < u2 * u2 * rel) q22 = 0.5 * q / fabs(u2);
         const double l1 = rh_log_ratio(q11, q21), l2 = rh_log_ratio(q22, q12);
         const double ras = r * as;
-        const double w1 = fabs(u1) / ras, w2 = fabs(u2) / ras;
+        const double rr_as = (ras > 0.0) ? rh_rcp(ras) : INFINITY;  // as x/0 would give
+        const double w1 = fabs(u1) * rr_as, w2 = fabs(u2) * rr_as;
         const double s1 = (u1 < 0) ? -1.0 : 1.0, s2 = (u2 < 0) ? -1.0 : 1.0;
-        const double buf = ras * (s1 * atan_pair(z2 * w1 / D12, -z1 * w1 / D11) +
-                                  s2 * atan_pair(z1 * w2 / D21, -z2 * w2 / D22)) +
+        const double buf = ras * (s1 * atan_pair(z2 * w1 * rh_rcp(D12), -z1 * w1 * rh_rcp(D11)) +
+                                  s2 * atan_pair(z1 * w2 * rh_rcp(D21), -z2 * w2 * rh_rcp(D22))) +
                            z1 * l1 + z2 * l2;
         const double e1 = rr + r1r1 - tr1 * cd, e2 = rr + r2r2 - tr2 * cd;
         double z1p11 = z1 + D11, z1p21 = z1 + D21, z2p12 = z2 + D12, z2p22 = z2 + D22;
