@@ -838,6 +838,231 @@ def FldLst(obj, comp, P1, P2, np_, arg="arg", start=0.0):
     return [r[0] if len(r) == 1 else r for r in rows]
 
 
+def _face_planes(o):
+    """Outward face planes (n, d: n.x <= d inside) of a block or convex
+    polyhedron in its own coordinates."""
+    if isinstance(o, O.RecMag):
+        c, h = np.asarray(o.center), 0.5 * np.asarray(o.dims)
+        out = []
+        for a in range(3):
+            n = np.zeros(3)
+            n[a] = 1.0
+            out += [(n, c[a] + h[a]), (-n, -(c[a] - h[a]))]
+        return out
+    V = o.verts
+    c = V.mean(0)
+    out = []
+    for f in o.faces:
+        p0, p1, p2 = V[f[0]], V[f[1]], V[f[2]]
+        n = np.cross(p1 - p0, p2 - p0)
+        k = 2
+        while np.linalg.norm(n) == 0.0 and k + 1 < len(f):
+            k += 1
+            n = np.cross(p1 - p0, V[f[k]] - p0)
+        n = n / np.linalg.norm(n)
+        if n @ (c - p0) > 0:
+            n = -n
+        out.append((n, float(n @ p0)))
+    return out
+
+
+def _line_breaks(obj, p0, u):
+    """Parameters t where the line p0 + t u crosses an element surface (the
+    field jumps only there)."""
+    ts = []
+    for o, _k, stack in O.walk(_reg, int(obj)):
+        if not isinstance(o, (O.RecMag, O.Polyhedron)) or getattr(o, "is_current", False):
+            continue
+        planes = _face_planes(o)
+        for t in O._expand(stack):
+            # the copy's point map x -> M x + V: planes n.x <= d become
+            # (M^-T n).x <= d + (M^-T n).V in the lab frame
+            Mi = np.linalg.inv(t.M)
+            for n, d in planes:
+                nl = Mi.T @ n
+                dl = d + nl @ t.V
+                den = nl @ u
+                if abs(den) < 1e-300:
+                    continue
+                tt = (dl - nl @ p0) / den
+                x = p0 + tt * u
+                xl = Mi @ (x - t.V)
+                tol = 1e-9 * (1.0 + float(np.abs(xl).max()))
+                if all(m @ xl <= e + tol for m, e in planes):
+                    ts.append(float(tt))
+    return sorted(set(ts))
+
+
+_GL_X, _GL_W = np.polynomial.legendre.leggauss(20)
+
+
+def _integrate(pieces, tol_rel=1e-12, max_level=9):
+    """Sum of the integrals over the pieces (a, b, g, f): the vector function
+    f at x = g(s) times the weight g'(s) over s in [a, b]; composite 20-point
+    Gauss-Legendre, panels doubled per piece until two levels agree."""
+    level = [0] * len(pieces)
+    prev = [None] * len(pieces)
+    done = [False] * len(pieces)
+    total = None
+    for _ in range(max_level + 1):
+        todo = [i for i in range(len(pieces)) if not done[i]]
+        if not todo:
+            break
+        vals = {}
+        for fn in {id(pieces[i][3]): pieces[i][3] for i in todo}.values():
+            vals.update(_integrate_level([i for i in todo if pieces[i][3] is fn], pieces, level, fn))
+        cur = [vals[i] if i in vals else prev[i] for i in range(len(pieces))]
+        total = np.sum(cur, axis=0)
+        scale = np.abs(total).max() + 1e-300
+        for i in todo:
+            if prev[i] is not None and np.abs(vals[i] - prev[i]).max() <= tol_rel * scale:
+                done[i] = True
+            prev[i] = vals[i]
+            level[i] += 1
+    return total
+
+
+def _integrate_level(todo, pieces, level, f):
+    if True:
+        nodes, wts, owner = [], [], []
+        for i in todo:
+            a, b, g, _fn = pieces[i]
+            m = 2 ** level[i]
+            edges = np.linspace(a, b, m + 1)
+            for k in range(m):
+                lo, hi = edges[k], edges[k + 1]
+                sN = 0.5 * (hi - lo) * _GL_X + 0.5 * (hi + lo)
+                x, jac = g(sN)
+                nodes.append(x)
+                wts.append(0.5 * (hi - lo) * _GL_W * jac)
+                owner.append(np.full(len(sN), i))
+        X = np.concatenate(nodes)
+        Wt = np.concatenate(wts)
+        own = np.concatenate(owner)
+        F = np.asarray(f(X))
+        vals = {}
+        for i in todo:
+            sel = own == i
+            vals[i] = (F[sel] * Wt[sel, None]).sum(0)
+        return vals
+
+
+def FldInt(obj, kind, comp, P1, P2):
+    """Field integral along the straight line through P1 and P2 in T*mm:
+    'fin' from P1 to P2, 'inf' over the whole line (radpy.cpp FldInt,
+    radapl3.cpp:356-381).  comp: 'ib' or 'ih' with optional components
+    ('ibx', 'ihyz', ...).  Computed from the device field: the line is split
+    where it crosses element surfaces and at distances R 2^j (j = 0..16) from
+    the centre of the sources' bounding sphere (radius R), each piece
+    integrated by composite Gauss-Legendre with panel doubling to 1e-12 of the
+    integral's scale.  The two tails of an infinite line beyond 65536 R
+    (about 1e-10 of the integral) are mapped to a finite interval
+    (t = t0 + L u/(1 - u)) and take the field of the total dipole moment,
+    exact there to R/t (with current sources present, the device field)."""
+    k = str(kind).lower()
+    c = str(comp).lower()
+    if k not in ("inf", "fin") or len(c) < 2 or c[0] != "i" or c[1] not in "bh":
+        raise RuntimeError("Radia::Error033")
+    p1 = np.asarray(_vec3(P1), dtype=np.float64)
+    p2 = np.asarray(_vec3(P2), dtype=np.float64)
+    seg = p2 - p1
+    length = float(np.linalg.norm(seg))
+    if not length > 0.0:
+        raise RuntimeError("Incorrect argument: FldInt")
+    u = seg / length
+    which = c[1]
+    idx = [ {"x": 0, "y": 1, "z": 2}[ch] for ch in c[2:]] if len(c) > 2 else [0, 1, 2]
+
+    def f(X):
+        v = np.asarray(Fld(obj, which, X.reshape(-1, 3)), dtype=np.float64).reshape(-1, 3)
+        return v
+
+    brk = _line_breaks(obj, p1, u)
+    # geometric breakpoints about the sources' bounding sphere, so that every
+    # panel spans a shell over which the field varies smoothly
+    pts_src = []
+    for o, _k, stack in O.walk(_reg, int(obj)):
+        if isinstance(o, O.RecMag):
+            cc, hh = np.asarray(o.center), 0.5 * np.asarray(o.dims)
+            V = np.array([cc + hh * np.array([sx, sy, sz]) for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)])
+        elif isinstance(o, O.Polyhedron):
+            V = o.verts
+        else:
+            continue
+        for t in O._expand(stack):
+            pts_src.append(V @ t.M.T + t.V)
+    if pts_src:
+        A = np.concatenate(pts_src)
+        ctr = 0.5 * (A.min(0) + A.max(0))
+        R = max(0.5 * float(np.linalg.norm(A.max(0) - A.min(0))), 1e-9 * (1.0 + length))
+    else:
+        ctr, R = 0.5 * (p1 + p2), 0.5 * length
+    tc = float((ctr - p1) @ u)
+    far = 65536.0
+    geo = [tc + sgn * R * 2.0 ** j for j in range(0, 17) for sgn in (-1.0, 1.0)]
+    pieces = []
+    # beyond far * R the field is that of the total dipole moment to R/t
+    # (the exact face formulas lose everything to cancellation far away)
+    mom = np.zeros(3)
+    has_cur = False
+    for o, _k, stack in O.walk(_reg, int(obj)):
+        if getattr(o, "is_current", False) or isinstance(o, (O.ArcCur,)):
+            has_cur = True
+        if isinstance(o, (O.RecMag, O.Polyhedron)) and not getattr(o, "is_current", False):
+            if isinstance(o, O.RecMag):
+                vol = float(np.prod(o.dims))
+            else:
+                V = o.verts
+                cv = V.mean(0)
+                vol = sum(abs(np.dot(V[f_[0]] - cv, np.cross(V[f_[j]] - cv, V[f_[j + 1]] - cv))) / 6.0
+                          for f_ in o.faces for j in range(1, len(f_) - 1))
+            for t in O._expand(stack):
+                mom += vol * t.s * (t.M @ np.asarray(o.M))
+
+    def f_dip(X):
+        r = X - ctr[None, :]
+        rn = np.linalg.norm(r, axis=1)
+        e = r / rn[:, None]
+        return (3.0 * (e @ mom)[:, None] * e - mom[None, :]) / (4.0 * np.pi * rn[:, None] ** 3)
+
+    f_tail = f if has_cur else f_dip
+
+    def lin(a, b):
+        return (a, b, lambda s: (p1[None, :] + s[:, None] * u[None, :], np.ones_like(s)), f)
+
+    if k == "fin":
+        ts = sorted({0.0, length} | {t for t in brk + geo if 0.0 < t < length})
+        for a, b in zip(ts[:-1], ts[1:]):
+            if b > a:
+                pieces.append(lin(a, b))
+    else:
+        t_lo = min([0.0, tc - R * far] + brk)
+        t_hi = max([length, tc + R * far] + brk)
+        ts = sorted({t_lo, t_hi} | {t for t in brk + geo if t_lo < t < t_hi})
+        for a, b in zip(ts[:-1], ts[1:]):
+            if b > a:
+                pieces.append(lin(a, b))
+        L = t_hi - t_lo
+        pieces.append((0.0, 1.0, lambda s: (p1[None, :] + (t_hi + L * s / (1.0 - s))[:, None] * u[None, :],
+                                            L / (1.0 - s) ** 2), f_tail))
+        pieces.append((0.0, 1.0, lambda s: (p1[None, :] + (t_lo - L * s / (1.0 - s))[:, None] * u[None, :],
+                                            L / (1.0 - s) ** 2), f_tail))
+    tot = _integrate(pieces)
+    out = [float(tot[i]) for i in idx]
+    return out[0] if len(out) == 1 else out
+
+
+def UtiMPI(*args):
+    """MPI is not part of this build (one process per GPU, one GPU here):
+    reports rank 0 of 1."""
+    return [0, 1]
+
+
+def ObjDrwOpenGL(*args, **kw):
+    """3D viewer: graphics are not part of this build; does nothing."""
+    return 0
+
+
 # ------------------------------------------------------------------ utilities
 def FldLenRndSw(onoff):
     """Length randomization switch.  The device formulas need no random
