@@ -315,9 +315,10 @@ rh_status rh_model_create(const rh_model_desc* desc, int device, void* stream, r
         if (e == hipSuccess) e = up(&md.face_geo, F.face_geo);
         if (e == hipSuccess) e = up(&md.face_inc_off, F.face_inc_off);
         if (e == hipSuccess) e = up(&md.face_inc_elem, F.face_inc_elem);
-        if (e == hipSuccess) e = up(&md.face_inc_sign, F.face_inc_sign);
+        if (e == hipSuccess) e = up(&md.face_inc_v, F.face_inc_v);
         if (e == hipSuccess) e = up(&md.in_elem, F.in_elem);
         if (e == hipSuccess) e = up(&md.in_geo, F.in_geo);
+        if (e == hipSuccess) e = up(&md.in_F, F.in_F);
         if (e == hipSuccess) e = up(&md.tile_box, F.tile_box);
         if (e == hipSuccess) e = up(&md.edge_cont, F.edge_cont);
         if (e == hipSuccess) e = up(&md.face_cont, F.face_cont);
@@ -363,7 +364,7 @@ rh_status rh_model_mesh_counts(const rh_model* m, int64_t* counts3)
 rh_status rh_model_field_form(const rh_model* m, int32_t* form)
 {
     if (!m || !form) return fail(RH_ERR_ARG, "null argument");
-    *form = (m->dev.use_mesh && m->dev.mesh.active && !m->dev.sym) ? 2 : 1;
+    *form = (m->dev.use_mesh && m->dev.mesh.active) ? 2 : 1;
     return RH_OK;
 }
 

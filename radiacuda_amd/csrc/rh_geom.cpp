@@ -456,30 +456,75 @@ void build_mesh_form(PackedModel& P)
 {
     MeshForm& M = P.mesh;
     M = MeshForm();
-    if (P.sym || P.kind_off[3] - P.kind_off[1] < 2) return;
+    if (P.kind_off[3] - P.kind_off[1] < 2) return;
     std::vector<int64_t> elems(P.kind_idx.begin() + P.kind_off[1], P.kind_idx.begin() + P.kind_off[3]);
     std::vector<ElemView> views;
     for (int64_t j : elems) {
         views.push_back(view_elem(P, j));
         if (int(views.back().faces.size()) > kMeshPlanes) return;
     }
+    // Every element copy is a mesh element in the laboratory frame: copy c
+    // maps points x -> M_c x + V_c and fields by F_c = s_c M_c (M_c
+    // orthogonal), so the copy's H is F_c times the element's H at the mapped
+    // point.  L_e and a face's solid angle (signed by its transformed normal)
+    // are unchanged by the isometry, hence an edge incidence weighs M_j by
+    // F_c W_e and a face incidence adds (F_c^T n) . M_j to the charge of a
+    // face with normal n.  Without copies F_c = I and the weights are W_e and
+    // n, exactly.
+    struct Inst {
+        int32_t j;
+        const double* F;               // F_c, row-major
+        std::vector<double> verts;     // laboratory frame
+        std::vector<double> normals;   // laboratory frame, 3 per face
+        const ElemView* v;
+    };
+    std::vector<Inst> inst;
+    for (size_t q = 0; q < elems.size(); q++) {
+        const int64_t j = elems[q];
+        const ElemView& v = views[q];
+        for (int64_t c = P.copy_off[size_t(j)]; c < P.copy_off[size_t(j) + 1]; c++) {
+            const double* cp = P.copies.data() + kCopySize * c;  // Minv, V, F
+            double Mc[9];
+            inv3(cp, Mc);
+            Inst I;
+            I.j = int32_t(j);
+            I.F = cp + 12;
+            I.v = &v;
+            for (int k = 0; k < v.nv; k++) {
+                const double* x = v.verts + 3 * k;
+                for (int r = 0; r < 3; r++)
+                    // + 0.0: a vertex mirrored onto a symmetry plane gets +0, not -0,
+                    // so that it keys equal to the original
+                    I.verts.push_back(Mc[3 * r] * x[0] + Mc[3 * r + 1] * x[1] + Mc[3 * r + 2] * x[2] + cp[9 + r] + 0.0);
+            }
+            for (size_t f = 0; f < v.faces.size(); f++) {
+                const double* n = v.face_n[f];
+                double nl[3];
+                for (int r = 0; r < 3; r++) nl[r] = cp[r] * n[0] + cp[3 + r] * n[1] + cp[6 + r] * n[2];
+                const double ln = std::sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]);
+                for (int r = 0; r < 3; r++) I.normals.push_back(nl[r] / ln);
+            }
+            inst.push_back(std::move(I));
+        }
+    }
     std::map<VKey, int32_t> vid;
     std::vector<const double*> vpos;
     std::map<std::pair<int32_t, int32_t>, int64_t> eid;
-    std::vector<std::vector<std::pair<int32_t, const double*>>> einc;
+    std::vector<std::vector<std::pair<int32_t, std::array<double, 9>>>> einc;
     std::map<std::vector<int32_t>, int64_t> fid;
-    std::vector<std::vector<std::pair<int32_t, double>>> finc;
+    std::vector<std::vector<std::pair<int32_t, std::array<double, 3>>>> finc;
     std::vector<std::vector<int32_t>> floop;  // unique-vertex loop of the first incidence
     std::vector<const double*> fnorm;
-    for (size_t q = 0; q < elems.size(); q++) {
-        const int32_t j = int32_t(elems[q]);
-        const ElemView& v = views[q];
+    for (const Inst& I : inst) {
+        const ElemView& v = *I.v;
+        const double* F = I.F;
         std::vector<int32_t> u(size_t(v.nv));
         for (int k = 0; k < v.nv; k++) {
-            auto it = vid.find(vkey(v.verts + 3 * k));
+            const double* x = I.verts.data() + 3 * k;
+            auto it = vid.find(vkey(x));
             if (it == vid.end()) {
-                it = vid.emplace(vkey(v.verts + 3 * k), int32_t(vpos.size())).first;
-                vpos.push_back(v.verts + 3 * k);
+                it = vid.emplace(vkey(x), int32_t(vpos.size())).first;
+                vpos.push_back(x);
             }
             u[size_t(k)] = it->second;
         }
@@ -491,10 +536,18 @@ void build_mesh_form(PackedModel& P)
                 it = eid.emplace(key, int64_t(einc.size())).first;
                 einc.emplace_back();
             }
-            einc[size_t(it->second)].push_back({j, v.edge_w[e]});
+            // F W with W symmetric (xx yy zz xy xz yz)
+            const double* w = v.edge_w[e];
+            const double W[9] = {w[0], w[3], w[4], w[3], w[1], w[5], w[4], w[5], w[2]};
+            std::array<double, 9> FW;
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++)
+                    FW[size_t(3 * r + c)] = F[3 * r] * W[c] + F[3 * r + 1] * W[3 + c] + F[3 * r + 2] * W[6 + c];
+            einc[size_t(it->second)].push_back({I.j, FW});
         }
         for (size_t f = 0; f < v.faces.size(); f++) {
             const std::vector<int>& loop = v.faces[f];
+            const double* n1 = I.normals.data() + 3 * f;
             // polygons of more than 4 vertices become fan pieces of at most 4
             for (size_t s = 0; s + 2 < loop.size(); s += 2) {
                 std::vector<int32_t> piece{u[size_t(loop[0])], u[size_t(loop[s + 1])],
@@ -508,13 +561,16 @@ void build_mesh_form(PackedModel& P)
                     it = fid.emplace(key, int64_t(finc.size())).first;
                     finc.emplace_back();
                     floop.push_back(piece);
-                    fnorm.push_back(v.face_n[f]);
+                    fnorm.push_back(n1);
                 } else {
                     const double* n0 = fnorm[size_t(it->second)];
-                    const double* n1 = v.face_n[f];
                     sign = (n0[0] * n1[0] + n0[1] * n1[1] + n0[2] * n1[2] > 0) ? 1.0 : -1.0;
                 }
-                finc[size_t(it->second)].push_back({j, sign});
+                // sign F^T n of the unique face's normal n
+                const double* n = fnorm[size_t(it->second)];
+                std::array<double, 3> gv;
+                for (int c = 0; c < 3; c++) gv[size_t(c)] = sign * (F[c] * n[0] + F[3 + c] * n[1] + F[6 + c] * n[2]);
+                finc[size_t(it->second)].push_back({I.j, gv});
             }
         }
     }
@@ -683,7 +739,7 @@ void build_mesh_form(PackedModel& P)
         M.edge_cont.push_back(double(econt[o]));
         for (const auto& inc : einc[e]) {
             M.edge_inc_elem.push_back(inc.first);
-            for (int k = 0; k < 6; k++) M.edge_inc_w.push_back(inc.second[k]);
+            for (int k = 0; k < 9; k++) M.edge_inc_w.push_back(inc.second[size_t(k)]);
         }
         M.edge_inc_off.push_back(int64_t(M.edge_inc_elem.size()));
     }
@@ -718,19 +774,20 @@ void build_mesh_form(PackedModel& P)
         M.face_cont.push_back(double(fcont[o]));
         for (const auto& inc : finc[f]) {
             M.face_inc_elem.push_back(inc.first);
-            M.face_inc_sign.push_back(inc.second);
+            for (int k = 0; k < 3; k++) M.face_inc_v.push_back(inc.second[size_t(k)]);
         }
         M.face_inc_off.push_back(int64_t(M.face_inc_elem.size()));
     }
-    // inside records and tile boxes
-    for (size_t q = 0; q < elems.size(); q++) {
-        const ElemView& v = views[q];
-        M.in_elem.push_back(int32_t(elems[q]));
+    // inside records (one per element copy, with its field transform) and tile boxes
+    for (const Inst& I : inst) {
+        const ElemView& v = *I.v;
+        M.in_elem.push_back(I.j);
+        for (int k = 0; k < 9; k++) M.in_F.push_back(I.F[k]);
         M.in_geo.push_back(double(v.faces.size()));
         for (int f = 0; f < kMeshPlanes; f++) {
             if (f < int(v.faces.size())) {
-                const double* n = v.face_n[size_t(f)];
-                const double* p0 = v.verts + 3 * v.faces[size_t(f)][0];
+                const double* n = I.normals.data() + 3 * f;
+                const double* p0 = I.verts.data() + 3 * v.faces[size_t(f)][0];
                 for (int c = 0; c < 3; c++) M.in_geo.push_back(n[c]);
                 M.in_geo.push_back(n[0] * p0[0] + n[1] * p0[1] + n[2] * p0[2]);
             } else {
@@ -738,13 +795,13 @@ void build_mesh_form(PackedModel& P)
             }
         }
     }
-    for (size_t t0 = 0; t0 < elems.size(); t0 += kMeshTile) {
+    for (size_t t0 = 0; t0 < inst.size(); t0 += kMeshTile) {
         double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (size_t q = t0; q < std::min(elems.size(), t0 + kMeshTile); q++)
-            for (int k = 0; k < views[q].nv; k++)
+        for (size_t q = t0; q < std::min(inst.size(), t0 + kMeshTile); q++)
+            for (int k = 0; k < inst[q].v->nv; k++)
                 for (int c = 0; c < 3; c++) {
-                    lo[c] = std::min(lo[c], views[q].verts[3 * k + c]);
-                    hi[c] = std::max(hi[c], views[q].verts[3 * k + c]);
+                    lo[c] = std::min(lo[c], inst[q].verts[size_t(3 * k + c)]);
+                    hi[c] = std::max(hi[c], inst[q].verts[size_t(3 * k + c)]);
                 }
         for (int c = 0; c < 3; c++) M.tile_box.push_back(lo[c]);
         for (int c = 0; c < 3; c++) M.tile_box.push_back(hi[c]);

@@ -15,20 +15,22 @@ constexpr int kCopySize = 24;
 
 // Conforming-mesh form of the polyhedra for the field sweep (rh_mesh.hpp):
 // unique edges and faces with their element incidences, and per-element
-// plane sets for the inside test.  Built when the model has no transform
-// copies and every polyhedron has at most kMeshPlanes faces.
+// plane sets for the inside test.  Every transform copy of an element is a
+// mesh element in the laboratory frame.  Built when every polyhedron has at
+// most kMeshPlanes faces.
 struct MeshForm {
     bool active = false;
     std::vector<double> edge_geo;        // kMeshEdgeGeo per unique edge
     std::vector<int64_t> edge_inc_off;   // [E+1]
     std::vector<int32_t> edge_inc_elem;  // element of each incidence
-    std::vector<double> edge_inc_w;      // W_e of that element, 6 per incidence
+    std::vector<double> edge_inc_w;      // F_c W_e of that element copy, 9 per incidence (row-major)
     std::vector<double> face_geo;        // kMeshFaceGeo per unique face
     std::vector<int64_t> face_inc_off;   // [F+1]
     std::vector<int32_t> face_inc_elem;
-    std::vector<double> face_inc_sign;   // +1: the element's outward normal is n_f
+    std::vector<double> face_inc_v;      // 3 per incidence: the charge of face f gains v . M_j
     std::vector<int32_t> in_elem;        // element of each inside record
-    std::vector<double> in_geo;          // 1 + 4 kMeshPlanes per element: count, planes
+    std::vector<double> in_geo;          // 1 + 4 kMeshPlanes per element copy: count, planes
+    std::vector<double> in_F;            // 9 per inside record: F_c of the copy
     std::vector<double> tile_box;        // 6 per tile of kMeshTile inside records
     std::vector<double> edge_cont;       // 1: the edge starts at the previous edge's end
     std::vector<double> face_cont;       // 0: triangle, 1: triangle whose vertices 0, 1 are the

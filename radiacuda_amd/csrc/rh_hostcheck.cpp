@@ -198,11 +198,11 @@ int rhc_field_mesh(const rh_model_desc* d, const double* M, const double* pts, i
     for (int64_t i = 0; i < ne; i++) {
         double w[3] = {0, 0, 0};
         for (int64_t k = F.edge_inc_off[i]; k < F.edge_inc_off[i + 1]; k++) {
-            const double* W = F.edge_inc_w.data() + 6 * k;
+            const double* W = F.edge_inc_w.data() + 9 * k;
             const double* m = M + 3 * F.edge_inc_elem[k];
-            w[0] += W[0] * m[0] + W[3] * m[1] + W[4] * m[2];
-            w[1] += W[3] * m[0] + W[1] * m[1] + W[5] * m[2];
-            w[2] += W[4] * m[0] + W[5] * m[1] + W[2] * m[2];
+            w[0] += W[0] * m[0] + W[1] * m[1] + W[2] * m[2];
+            w[1] += W[3] * m[0] + W[4] * m[1] + W[5] * m[2];
+            w[2] += W[6] * m[0] + W[7] * m[1] + W[8] * m[2];
         }
         double* o = ce.data() + rh::kMeshEdge * i;
         for (int k = 0; k < rh::kMeshEdgeGeo; k++) o[k] = F.edge_geo[rh::kMeshEdgeGeo * i + k];
@@ -215,7 +215,8 @@ int rhc_field_mesh(const rh_model_desc* d, const double* M, const double* pts, i
         double s = 0;
         for (int64_t k = F.face_inc_off[i]; k < F.face_inc_off[i + 1]; k++) {
             const double* m = M + 3 * F.face_inc_elem[k];
-            s += F.face_inc_sign[k] * (g[12] * m[0] + g[13] * m[1] + g[14] * m[2]);
+            const double* v = F.face_inc_v.data() + 3 * k;
+            s += v[0] * m[0] + v[1] * m[1] + v[2] * m[2];
         }
         double* o = cf.data() + rh::kMeshFace * i;
         for (int k = 0; k < rh::kMeshFaceGeo; k++) o[k] = g[k];
@@ -227,7 +228,9 @@ int rhc_field_mesh(const rh_model_desc* d, const double* M, const double* pts, i
         const double* g = F.in_geo.data() + (1 + 4 * rh::kMeshPlanes) * i;
         double* o = ci.data() + rh::kMeshIn * i;
         o[0] = g[0];
-        for (int k = 0; k < 3; k++) o[1 + k] = M[3 * F.in_elem[i] + k];
+        const double* Fc = F.in_F.data() + 9 * i;
+        const double* m = M + 3 * F.in_elem[i];
+        for (int k = 0; k < 3; k++) o[1 + k] = Fc[3 * k] * m[0] + Fc[3 * k + 1] * m[1] + Fc[3 * k + 2] * m[2];
         for (int k = 0; k < 4 * rh::kMeshPlanes; k++) o[4 + k] = g[1 + k];
     }
     // blocks through the per-element path

@@ -331,11 +331,11 @@ __global__ __launch_bounds__(256) void k_mesh_charge_edges(MeshDev md, const dou
     double w[3] = {0.0, 0.0, 0.0};
     for (int64_t k = md.edge_inc_off[i]; k < md.edge_inc_off[i + 1]; k++) {
         const int64_t j = md.edge_inc_elem[k];
-        const double* W = md.edge_inc_w + 6 * k;
+        const double* W = md.edge_inc_w + 9 * k;  // F_c W_e, row-major
         const double m0 = M[3 * j], m1 = M[3 * j + 1], m2 = M[3 * j + 2];
-        w[0] += W[0] * m0 + W[3] * m1 + W[4] * m2;
-        w[1] += W[3] * m0 + W[1] * m1 + W[5] * m2;
-        w[2] += W[4] * m0 + W[5] * m1 + W[2] * m2;
+        w[0] += W[0] * m0 + W[1] * m1 + W[2] * m2;
+        w[1] += W[3] * m0 + W[4] * m1 + W[5] * m2;
+        w[2] += W[6] * m0 + W[7] * m1 + W[8] * m2;
     }
     const double* g = md.edge_geo + kMeshEdgeGeo * i;
     double* o = md.ch_edge + kMeshEdge * i;
@@ -363,7 +363,8 @@ __global__ __launch_bounds__(256) void k_mesh_charge_faces(MeshDev md, const dou
     double s = 0.0;
     for (int64_t k = md.face_inc_off[i]; k < md.face_inc_off[i + 1]; k++) {
         const int64_t j = md.face_inc_elem[k];
-        s += md.face_inc_sign[k] * (n0 * M[3 * j] + n1 * M[3 * j + 1] + n2 * M[3 * j + 2]);
+        const double* v = md.face_inc_v + 3 * k;
+        s += v[0] * M[3 * j] + v[1] * M[3 * j + 1] + v[2] * M[3 * j + 2];
     }
     double* o = md.ch_face + kMeshFace * i;
     for (int k = 0; k < kMeshFaceGeo; k++) o[k] = g[k];
@@ -411,10 +412,12 @@ __global__ __launch_bounds__(256) void k_mesh_charge_in(MeshDev md, const double
     const int64_t j = md.in_elem[i];
     const double* g = md.in_geo + (1 + 4 * kMeshPlanes) * i;
     double* o = md.ch_in + kMeshIn * i;
+    const double* F = md.in_F + 9 * i;  // the copy's field transform
+    const double m0 = M[3 * j], m1 = M[3 * j + 1], m2 = M[3 * j + 2];
     o[0] = g[0];
-    o[1] = M[3 * j];
-    o[2] = M[3 * j + 1];
-    o[3] = M[3 * j + 2];
+    o[1] = F[0] * m0 + F[1] * m1 + F[2] * m2;
+    o[2] = F[3] * m0 + F[4] * m1 + F[5] * m2;
+    o[3] = F[6] * m0 + F[7] * m1 + F[8] * m2;
     for (int k = 0; k < 4 * kMeshPlanes; k++) o[4 + k] = g[1 + k];
 }
 
@@ -1357,7 +1360,7 @@ void field_slot(const DevModel& m, const double* M, const double* pts, int64_t n
                        H, part);
 }
 
-bool mesh_on(const DevModel& m) { return m.use_mesh && m.mesh.active && !m.sym; }
+bool mesh_on(const DevModel& m) { return m.use_mesh && m.mesh.active; }
 
 // slices per kind slot (0 blocks, 1 tetrahedra, 2 general polyhedra); in the
 // mesh form slot 1 carries the whole mesh and slot 2 is empty

@@ -21,9 +21,10 @@ struct MeshDev {
     const double* face_geo = nullptr;
     const int64_t* face_inc_off = nullptr;
     const int32_t* face_inc_elem = nullptr;
-    const double* face_inc_sign = nullptr;
+    const double* face_inc_v = nullptr;  // 3 per face incidence
     const int32_t* in_elem = nullptr;
     const double* in_geo = nullptr;
+    const double* in_F = nullptr;        // 9 per inside record
     const double* tile_box = nullptr;
     const double* edge_cont = nullptr;
     const double* face_cont = nullptr;

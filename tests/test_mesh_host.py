@@ -92,3 +92,36 @@ def test_mesh_counts_c_yoke(L):
     f = _fields(L, P, M, W.gap_grid(info, 2))
     _B, _H, counts = f["mesh"]
     assert list(counts) == [11454, 17200, 8000]
+
+
+def test_mesh_form_with_transform_copies_equals_per_element(L):
+    """Every transform copy is a mesh element in the laboratory frame: a tet
+    mesh under a field-inverting mirror (TrfZerPara), a plain mirror
+    (TrfZerPerp) and a 4-fold rotation (TrfMlt), plus a block, give the
+    per-element sum.  Faces on the mirror planes are shared between an
+    element and its image."""
+    rng = np.random.default_rng(35)
+    rad.UtiDelAll()
+    ids = models.tet_mesh((2, 2, 2), 3.0, origin=(0.0, 0.0, 0.0), jitter=0.1, seed=36)
+    mesh = rad.ObjCnt(ids)
+    rad.TrfZerPara(mesh, [0, 0, 0], [0, 0, 1])
+    rad.TrfZerPerp(mesh, [0, 0, 0], [1, 0, 0])
+    blk = rad.ObjRecMag([15, 2, 1], [3, 4, 5], [0.1, 0.2, 0.3])
+    rot = rad.ObjCnt([rad.ObjPolyhdr([[20, 0, 0], [24, 0, 0], [20, 4, 0], [20, 0, 4]],
+                                     [[1, 2, 3], [1, 2, 4], [1, 3, 4], [2, 3, 4]], [0.2, -0.3, 0.5])])
+    rad.TrfMlt(rot, rad.TrfRot([0, 0, 0], [0, 0, 1], np.pi / 2), 4)
+    top = rad.ObjCnt([mesh, blk, rot])
+    P, items = models.packed(top)
+    M = rng.uniform(-1, 1, (len(items), 3))
+    pts = np.concatenate([rng.uniform(-5.8, 5.8, (80, 3)), rng.uniform(-30, 30, (120, 3))])
+    for fp32, tol in ((0, 1e-12), (1, 1e-5)):
+        f = _fields(L, P, M, pts, fp32)
+        Be, He, _ = f["elem"]
+        Bm, Hm, counts = f["mesh"]
+        assert np.abs(Hm - He).max() <= tol * np.abs(He).max(), fp32
+        assert np.abs(Bm - Be).max() <= tol * np.abs(Be).max(), fp32
+    # the mesh's tetrahedra x 4 copies + 4 rotated tetrahedra (the block is not a mesh element)
+    assert counts[2] == len(ids) * 4 + 4
+    # faces on the two mirror planes are shared: fewer unique faces than without sharing
+    assert counts[1] < 4 * counts[2] * 0.6
+    rad.UtiDelAll()
