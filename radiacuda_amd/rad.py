@@ -923,28 +923,29 @@ def _integrate(pieces, tol_rel=1e-12, max_level=9):
 
 
 def _integrate_level(todo, pieces, level, f):
-    if True:
-        nodes, wts, owner = [], [], []
-        for i in todo:
-            a, b, g, _fn = pieces[i]
-            m = 2 ** level[i]
-            edges = np.linspace(a, b, m + 1)
-            for k in range(m):
-                lo, hi = edges[k], edges[k + 1]
-                sN = 0.5 * (hi - lo) * _GL_X + 0.5 * (hi + lo)
-                x, jac = g(sN)
-                nodes.append(x)
-                wts.append(0.5 * (hi - lo) * _GL_W * jac)
-                owner.append(np.full(len(sN), i))
-        X = np.concatenate(nodes)
-        Wt = np.concatenate(wts)
-        own = np.concatenate(owner)
-        F = np.asarray(f(X))
-        vals = {}
-        for i in todo:
-            sel = own == i
-            vals[i] = (F[sel] * Wt[sel, None]).sum(0)
-        return vals
+    """Integrals of f over the pieces listed in todo, each split into
+    2^level panels of 20 Gauss-Legendre nodes, with one call of f."""
+    nodes, wts, owner = [], [], []
+    for i in todo:
+        a, b, g, _fn = pieces[i]
+        m = 2 ** level[i]
+        edges = np.linspace(a, b, m + 1)
+        for k in range(m):
+            lo, hi = edges[k], edges[k + 1]
+            sN = 0.5 * (hi - lo) * _GL_X + 0.5 * (hi + lo)
+            x, jac = g(sN)
+            nodes.append(x)
+            wts.append(0.5 * (hi - lo) * _GL_W * jac)
+            owner.append(np.full(len(sN), i))
+    X = np.concatenate(nodes)
+    Wt = np.concatenate(wts)
+    own = np.concatenate(owner)
+    F = np.asarray(f(X))
+    vals = {}
+    for i in todo:
+        sel = own == i
+        vals[i] = (F[sel] * Wt[sel, None]).sum(0)
+    return vals
 
 
 def FldInt(obj, kind, comp, P1, P2):
@@ -971,7 +972,7 @@ def FldInt(obj, kind, comp, P1, P2):
         raise RuntimeError("Incorrect argument: FldInt")
     u = seg / length
     which = c[1]
-    idx = [ {"x": 0, "y": 1, "z": 2}[ch] for ch in c[2:]] if len(c) > 2 else [0, 1, 2]
+    idx = [{"x": 0, "y": 1, "z": 2}[ch] for ch in c[2:]] if len(c) > 2 else [0, 1, 2]
 
     def f(X):
         v = np.asarray(Fld(obj, which, X.reshape(-1, 3)), dtype=np.float64).reshape(-1, 3)
