@@ -102,7 +102,8 @@ typedef struct rh_matrix rh_matrix;
  * (cpp/src/core/radgpu_asm.cu:889-1158) and the matrix hand-off
  * radGPU_PublishAssembledMatrix / radGPU_AdoptMatrixCache
  * (cpp/src/core/radgpu_asm.cu:40-51, cpp/src/core/radgpurlx.cu:728-746): the
- * handle owns the device matrix, no host copy is made. */
+ * handle owns the device matrix, no host copy is made.  A matrix beyond the
+ * device budget is handled as the fallback policy says (rh_set_fallback). */
 rh_status rh_matrix_assemble(rh_model *m, rh_precision prec, rh_matrix **out);
 
 /* Placement of the matrix rows.  Rows that do not fit in device memory can
@@ -118,9 +119,12 @@ typedef struct {
     double resident_fraction; /* < 0: keep as many element rows resident as the
                                * budget allows; else this fraction of them         */
     int32_t allow_streaming;  /* 0: RH_ERR_NOMEM when the matrix exceeds the
-                               * budget; 1: stream the remaining rows               */
+                               * budget; 1: stream the remaining rows;
+                               * RH_STREAM_BY_POLICY (-1): as rh_get_fallback()
+                               * says ('break' refuses, the others stream)       */
     int32_t reserved;         /* 0                                                 */
 } rh_matrix_opts;
+#define RH_STREAM_BY_POLICY (-1)
 rh_status rh_matrix_assemble_ex(rh_model *m, rh_precision prec, const rh_matrix_opts *opts,
                                 rh_matrix **out);
 /* Rows held on the device and rows streamed from host memory. */
@@ -279,6 +283,35 @@ rh_status rh_relax_adjoint_dev(rh_matrix *q, const double *d_mats, const double 
 /* Same on host arrays. */
 rh_status rh_relax_adjoint(rh_matrix *q, const double *mats, const double *rhs, double *lam,
                            const rh_relax_opts *opts, rh_relax_result *res);
+
+/* ------------------------------------------------------------------------ */
+/* Fallback policy and backend report                                         */
+/* ------------------------------------------------------------------------ */
+/* What rh_matrix_assemble (and rh_matrix_assemble_ex with allow_streaming =
+ * RH_STREAM_BY_POLICY) does with a matrix beyond the device budget.  Process
+ * wide, default RH_FALLBACK_CPU.  The reference's 'cpu' mode assembles on the
+ * host; this library has no host assembly, so 'cpu' and 'gpu_streaming' both
+ * keep the rows that fit resident and stream the rest, and 'break' refuses the
+ * matrix with RH_ERR_NOMEM.  Replaces RadSetGpuFallbackMode /
+ * RadGetGpuFallbackMode (cpp/src/core/radintrc.cpp:45-46, policy values of
+ * RadUtiGpuFallback, cpp/src/lib/radentry.cpp:934-944). */
+typedef enum {
+    RH_FALLBACK_CPU = 0,           /* 'cpu'           */
+    RH_FALLBACK_GPU_STREAMING = 1, /* 'gpu_streaming' */
+    RH_FALLBACK_BREAK = 2          /* 'break'         */
+} rh_fallback;
+/* RH_ERR_ARG (policy unchanged) for a mode outside 0..2. */
+rh_status rh_set_fallback(int32_t mode);
+int32_t rh_get_fallback(void);
+
+/* Backend of the most recent assembly (path 0: rh_matrix_assemble*) or field
+ * evaluation (path 1: rh_field*, rh_current_field*) in this process:
+ * -1 none yet, 1 device, 2 device with rows streamed from host memory
+ * (assembly only); 0 (host) is never reported, as nothing here runs on the
+ * host.  -1 for an unknown path.  Replaces RadGetLastAsmBackend
+ * (cpp/src/core/radintrc.cpp:44) and RadUtiFldLastBackend
+ * (cpp/src/lib/radentry.cpp:1043-1045, 1760-1765). */
+int32_t rh_last_backend(int32_t path);
 
 /* Diagnostics */
 const char *rh_last_error(void);

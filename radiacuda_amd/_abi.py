@@ -10,6 +10,8 @@ _i64 = ctypes.POINTER(ctypes.c_int64)
 RH_OK, RH_ERR_ARG, RH_ERR_HIP, RH_ERR_NOMEM, RH_ERR_UNSUPPORTED, RH_ERR_GEOMETRY, \
     RH_ERR_NOTCONV = range(7)
 RH_FP64, RH_MIXED = 0, 1
+RH_FALLBACK_CPU, RH_FALLBACK_GPU_STREAMING, RH_FALLBACK_BREAK = 0, 1, 2
+RH_STREAM_BY_POLICY = -1
 MAT_STRIDE = 16
 
 

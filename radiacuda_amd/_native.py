@@ -25,10 +25,27 @@ class RadHipError(RuntimeError):
     pass
 
 
+_RAW = None
+
+
 def lib():
+    """The library, after checking that a HIP device is visible."""
     global _L
     if _L is not None:
         return _L
+    L = _load()
+    if L.rh_device_count() < 1:
+        raise RadHipError("no HIP device visible; the HIP path has no CPU fallback")
+    _L = L
+    return L
+
+
+def _load():
+    """Loads and binds the library without touching a device (the process-wide
+    policy and diagnostics calls need none)."""
+    global _RAW
+    if _RAW is not None:
+        return _RAW
     if not os.path.exists(LIB_PATH):
         raise RadHipError("libradhip.so is not built (run: python -m radiacuda_amd.build); "
                           "the HIP path has no CPU fallback")
@@ -82,6 +99,10 @@ def lib():
                                        ctypes.POINTER(_abi.RelaxResult)]
     L.rh_relax_adjoint.argtypes = [_vp, _d, _d, _d, ctypes.POINTER(_abi.RelaxOpts),
                                    ctypes.POINTER(_abi.RelaxResult)]
+    L.rh_set_fallback.argtypes = [ctypes.c_int32]
+    L.rh_get_fallback.restype = ctypes.c_int32
+    L.rh_last_backend.argtypes = [ctypes.c_int32]
+    L.rh_last_backend.restype = ctypes.c_int32
     L.rh_set_timing.argtypes = [ctypes.c_int32]
     L.rh_kernel_time.argtypes = [ctypes.c_char_p, _d, _i64]
     for name in ["rh_model_create", "rh_model_destroy", "rh_model_colloc", "rh_model_counts",
@@ -92,17 +113,15 @@ def lib():
                  "rh_field_adjoint", "rh_field_adjoint_dev", "rh_relax_adjoint_dev",
                  "rh_relax_adjoint", "rh_model_set_field_form", "rh_model_field_form",
                  "rh_model_mesh_counts", "rh_matrix_assemble_ex", "rh_matrix_layout",
-                 "rh_relax_tab", "rh_relax_tab_dev"]:
+                 "rh_relax_tab", "rh_relax_tab_dev", "rh_set_fallback"]:
         getattr(L, name).restype = st
-    if L.rh_device_count() < 1:
-        raise RadHipError("no HIP device visible; the HIP path has no CPU fallback")
-    _L = L
+    _RAW = L
     return L
 
 
 def check(rc):
     if rc != RH_OK:
-        raise RadHipError("radhip error %d: %s" % (rc, lib().rh_last_error().decode()))
+        raise RadHipError("radhip error %d: %s" % (rc, _load().rh_last_error().decode()))
 
 
 def _p(a):
@@ -211,9 +230,9 @@ class Model:
         M = _c(M).reshape(-1)
         pts = _c(pts).reshape(-1, 3)
         B = np.zeros_like(pts) if want_b else None
-        H = np.zeros_like(pts)
-        check(lib().rh_field(self.h, _p(M), _p(pts), len(pts), _p(B) if want_b else None,
-                             _p(H), precision))
+        H = np.zeros_like(pts) if (want_h or not want_b) else None
+        check(lib().rh_field(self.h, _p(M), _p(pts), len(pts), _p(B) if B is not None else None,
+                             _p(H) if H is not None else None, precision))
         return B, H
 
 
@@ -224,11 +243,12 @@ class Matrix:
                  allow_streaming=False):
         """device_bytes / resident_fraction / allow_streaming: rh_matrix_opts
         (rows beyond the device budget are streamed from pinned host memory
-        when allowed)."""
+        when allowed; allow_streaming="policy" follows rh_get_fallback)."""
         self.model = model
         self.n = model.n
         self.h = _vp()
-        o = _abi.MatrixOpts(int(device_bytes), float(resident_fraction), 1 if allow_streaming else 0, 0)
+        stream = _abi.RH_STREAM_BY_POLICY if allow_streaming == "policy" else (1 if allow_streaming else 0)
+        o = _abi.MatrixOpts(int(device_bytes), float(resident_fraction), stream, 0)
         check(lib().rh_matrix_assemble_ex(model.h, int(prec), ctypes.byref(o), ctypes.byref(self.h)))
 
     def layout(self):
@@ -327,6 +347,25 @@ def mat_M(rec, H, tab=None):
         if ms != 0:
             f += ms * np.tanh(ks * h / ms)
     return (f / h) * H + mr if h != 0 else mr.copy()
+
+
+FALLBACK_NAMES = ("cpu", "gpu_streaming", "break")
+
+
+def set_fallback(mode):
+    """Process-wide fallback policy of the library (rh_set_fallback): 0 'cpu',
+    1 'gpu_streaming', 2 'break'."""
+    check(_load().rh_set_fallback(int(mode)))
+
+
+def get_fallback():
+    return int(_load().rh_get_fallback())
+
+
+def last_backend(path):
+    """rh_last_backend: path 0 assembly, 1 field; -1 none, 1 device, 2 device
+    with streamed rows."""
+    return int(_load().rh_last_backend(int(path)))
 
 
 def set_timing(on):

@@ -49,6 +49,11 @@ hipError_t enter_device(int device)
                         std::string(#call) + ": " + hipGetErrorString(e_));              \
     } while (0)
 
+// ------------------------------------------------- fallback policy, backends
+std::atomic<int32_t> g_fallback{RH_FALLBACK_CPU};
+std::atomic<int32_t> g_backend_asm{-1};
+std::atomic<int32_t> g_backend_fld{-1};
+
 // ---------------------------------------------------------------- timing
 std::atomic<bool> g_timing{false};
 struct Family {
@@ -219,6 +224,23 @@ int32_t rh_device_count(void)
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+rh_status rh_set_fallback(int32_t mode)
+{
+    if (mode < RH_FALLBACK_CPU || mode > RH_FALLBACK_BREAK)
+        return fail(RH_ERR_ARG, "fallback mode must be 0 ('cpu'), 1 ('gpu_streaming') or 2 ('break')");
+    g_fallback = mode;
+    return RH_OK;
+}
+
+int32_t rh_get_fallback(void) { return g_fallback; }
+
+int32_t rh_last_backend(int32_t path)
+{
+    if (path == 0) return g_backend_asm;
+    if (path == 1) return g_backend_fld;
+    return -1;
 }
 
 rh_status rh_set_timing(int32_t on)
@@ -471,6 +493,9 @@ rh_status rh_matrix_assemble_ex(rh_model* m, rh_precision prec, const rh_matrix_
     o.resident_fraction = -1.0;
     if (opts) o = *opts;
     if (o.resident_fraction > 1.0) return fail(RH_ERR_ARG, "resident_fraction above 1");
+    if (o.allow_streaming != 0 && o.allow_streaming != 1 && o.allow_streaming != RH_STREAM_BY_POLICY)
+        return fail(RH_ERR_ARG, "allow_streaming must be 0, 1 or RH_STREAM_BY_POLICY");
+    if (o.allow_streaming == RH_STREAM_BY_POLICY) o.allow_streaming = (g_fallback == RH_FALLBACK_BREAK) ? 0 : 1;
     RH_HIP(enter_device(m->device));
     const int64_t n = m->host.n;
     rh_matrix* q = new rh_matrix();
@@ -542,6 +567,7 @@ rh_status rh_matrix_assemble_ex(rh_model* m, rh_precision prec, const rh_matrix_
         free_matrix(q);
         return st;
     }
+    g_backend_asm = q->streamed() ? 2 : 1;
     *out = q;
     return RH_OK;
 }
@@ -550,6 +576,7 @@ rh_status rh_matrix_assemble(rh_model* m, rh_precision prec, rh_matrix** out)
 {
     rh_matrix_opts o{};
     o.resident_fraction = -1.0;
+    o.allow_streaming = RH_STREAM_BY_POLICY;
     return rh_matrix_assemble_ex(m, prec, &o, out);
 }
 
@@ -1372,6 +1399,7 @@ rh_status rh_field_dev(rh_model* m, const double* d_M, const double* d_pts, int6
     rh_status st = field_dev_impl(m, d_M, d_pts, np, d_B, d_H, prec == RH_MIXED);
     if (st != RH_OK) return st;
     RH_HIP(hipStreamSynchronize(m->stream));
+    g_backend_fld = 1;
     return RH_OK;
 }
 
@@ -1412,6 +1440,7 @@ rh_status rh_field(rh_model* m, const double* M, const double* pts, int64_t np, 
         if (e != hipSuccess) st = fail(RH_ERR_HIP, std::string("field download: ") + hipGetErrorString(e));
     }
     cleanup();
+    if (st == RH_OK) g_backend_fld = 1;
     return st;
 }
 
@@ -1679,6 +1708,7 @@ rh_status rh_current_field_dev(int device, void* stream, int64_t nsrc, const int
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     cleanup();
     if (e != hipSuccess) return fail(RH_ERR_HIP, std::string("current field: ") + hipGetErrorString(e));
+    g_backend_fld = 1;
     return RH_OK;
 }
 

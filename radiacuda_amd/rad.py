@@ -17,9 +17,6 @@ _reg = {}
 _next = [1]
 _gen = [0]             # structure generation (invalidates cached device models)
 _rand = ["on"]         # FldLenRndSw state
-_last_fld = ["none"]
-_last_asm = ["none"]
-_fallback = ["cpu"]
 
 
 def _bump():
@@ -659,18 +656,13 @@ def _assemble_with_policy(model):
     import os
     budget = int(os.environ.get("RADHIP_MATRIX_DEVICE_BYTES", "0"))
     frac = float(os.environ.get("RADHIP_MATRIX_RESIDENT_FRACTION", "-1"))
-    stream = _fallback[0] in ("gpu_streaming", "cpu")
     try:
-        Q = model.assemble(N.RH_MIXED, device_bytes=budget, resident_fraction=frac,
-                           allow_streaming=stream or frac >= 0)
+        return model.assemble(N.RH_MIXED, device_bytes=budget, resident_fraction=frac,
+                              allow_streaming=True if frac >= 0 else "policy")
     except N.RadHipError as e:
         if "exceeds the device budget" in str(e) or "does not fit" in str(e):
-            _last_asm[0] = "none"
             raise RuntimeError(_ERROR602) from e
         raise
-    _res, streamed = Q.layout()
-    _last_asm[0] = "gpu_streaming" if streamed else "gpu"
-    return Q
 
 
 def _opt(opts, name, default):
@@ -771,7 +763,20 @@ def _component(v, comp):
 
 def Fld(obj, comp, P, use_gpu=True, precision="double"):
     """Field component(s) comp ('b','bx','h','hz',...) at one point or a list
-    of points (radpy.cpp:2389-2456)."""
+    of points (radpy.cpp:2389-2456).  precision 'single', 'float' or 'fp32'
+    with use_gpu evaluates the B integrals in FP32 (FP64 accumulation), as the
+    reference's use_gpu code 2; 'double' and 'fp64' (and use_gpu=False, the
+    reference's FP64 host path) evaluate them in FP64; H is always FP64.  Any
+    other precision string raises the reference's argument error."""
+    fp32 = False
+    if precision is not None and precision != "":
+        if not isinstance(precision, str):
+            raise RuntimeError("Incorrect function arguments: Fld")
+        if precision in ("single", "float", "fp32"):
+            fp32 = bool(use_gpu)
+        elif precision not in ("double", "fp64"):
+            raise RuntimeError("Incorrect function arguments: Fld: precision must be 'double' "
+                               "or 'single'")
     comp = str(comp).lower()
     if comp[0] not in "bh":
         raise NotImplementedError("only B and H components are supported")
@@ -783,10 +788,14 @@ def Fld(obj, comp, P, use_gpu=True, precision="double"):
     H = np.zeros_like(pts)
     if model is not None:
         M = np.stack([o.M for (o, _k, _s) in mags])
-        prec = N.RH_MIXED if str(precision).lower() == "single" else N.RH_FP64
-        b, h = model.field(M, pts, precision=prec)
-        B += b
-        H += h
+        if comp[0] == "b" and fp32:
+            # FP32 integrals for B only, as the reference's use_gpu code 2
+            b, _h = model.field(M, pts, want_h=False, precision=N.RH_MIXED)
+            B += b
+        else:
+            b, h = model.field(M, pts, precision=N.RH_FP64)
+            B += b
+            H += h
     if cur is not None:
         c = N.current_field(cur, pts)
         B += c
@@ -796,7 +805,6 @@ def Fld(obj, comp, P, use_gpu=True, precision="double"):
             v = t.s * (t.M @ o.B)
             B += v
             H += v
-    _last_fld[0] = "gpu"
     src = B if comp[0] == "b" else H
     res = [_component(v, comp) for v in src]
     return res[0] if single else res
@@ -1097,21 +1105,33 @@ def UtiDelAll():
     return 0
 
 
+def _backend_name(code):
+    # radpy.cpp:3237-3276: 1 'gpu', 0 'cpu', otherwise 'none'; a matrix with
+    # streamed rows (code 2) is serviced by the device too
+    return "gpu" if code in (1, 2) else ("cpu" if code == 0 else "none")
+
+
 def UtiFldLastBackend():
-    return _last_fld[0]
+    """Backend of the most recent field evaluation (rh_last_backend(1))."""
+    return _backend_name(N.last_backend(1))
 
 
 def UtiAsmLastBackend():
-    return _last_asm[0]
+    """Backend of the most recent interaction-matrix assembly (rh_last_backend(0))."""
+    return _backend_name(N.last_backend(0))
 
 
 def UtiGpuFallback(mode=None):
+    """Sets and/or reports the library's fallback policy for a matrix beyond
+    the device budget (rh_set_fallback; radpy.cpp:2919-2945): 'cpu'
+    (default) and 'gpu_streaming' stream the rows that do not fit from host
+    memory, 'break' raises Radia::Error602.  Names are matched exactly."""
     if mode is not None:
-        m = str(mode).lower()
-        if m not in ("cpu", "gpu_streaming", "break"):
-            raise RuntimeError("UtiGpuFallback: mode must be 'cpu', 'gpu_streaming' or 'break'")
-        _fallback[0] = m
-    return _fallback[0]
+        m = mode if isinstance(mode, str) else None
+        if m not in N.FALLBACK_NAMES:
+            raise RuntimeError("UtiGpuFallback: expected 'cpu', 'gpu_streaming' or 'break'")
+        N.set_fallback(N.FALLBACK_NAMES.index(m))
+    return N.FALLBACK_NAMES[N.get_fallback()]
 
 
 def UtiVer():
