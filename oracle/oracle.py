@@ -36,6 +36,7 @@ def lib():
             build()
         L = ctypes.CDLL(path)
         L.ora_set_randomization.argtypes = [ctypes.c_int]
+        L.ora_set_threads.argtypes = [ctypes.c_int]
         L.ora_recmag_Q.argtypes = [_d, _d, _d, _d]
         L.ora_recmag_BH.argtypes = [_d, _d, _d, _d, _d, _d]
         L.ora_reccur_BH.argtypes = [_d, _d, _d, _d, _d, _d]
@@ -77,6 +78,11 @@ def _v(x, dtype=np.float64):
 
 def set_randomization(on):
     lib().ora_set_randomization(1 if on else 0)
+
+
+def set_threads(n):
+    """Threads of the oracle's OpenMP loops (1 = the reference's serial loops)."""
+    lib().ora_set_threads(int(n))
 
 
 # ---------------------------------------------------------------- elements

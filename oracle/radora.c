@@ -20,9 +20,15 @@
  *    following gmTrans (ext/genmath/gmtrans.h:30-85).
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Thread count of the OpenMP loops (assembly rows, field points); the
+ * reference's assembly and field loops are serial (radintrc.cpp:831-878,
+ * radapl3.cpp:170-178), which nthreads = 1 reproduces for timing. */
+void ora_set_threads(int nthreads) { omp_set_num_threads(nthreads > 0 ? nthreads : 1); }
 
 #define ORA_PI_REC 3.141592653589793238 /* radrec.cpp:128 */
 #define ORA_PI_PLN 3.14159265358979     /* radplnr2.cpp:32 */

@@ -274,14 +274,21 @@ RH_HD double rh_atan2(double y, double x)
     return copysign(a, y);
 }
 
-// atan2 for sweeps in which most angles are small: when every active lane of
-// the wave has x > 0 and |y| < tan(pi/8) x, the octant reduction of rh_atan2
-// is the identity for every lane, and the wave evaluates only the polynomial
-// on t = y/x.  Those are the operations rh_atan2 performs for such a lane (the
-// polynomial is odd, so the sign of y passes through unchanged), so both
-// branches give identical results; any other wave takes rh_atan2.  A wave
-// whose angles are all below pi/16 uses a shorter polynomial fitted to that
-// range: its lanes' results agree with the other branches to 1e-17.
+// atan2 for sweeps in which most angles are small.  Three tiers, chosen per
+// wave by a vote over its active lanes:
+// - every lane has x > 0 and |y| < tan(pi/16) x: a degree-6 polynomial in t^2
+//   on t = y/x fitted to that range (within 7.4e-18 of atan t in exact
+//   arithmetic);
+// - every lane has x > 0 and |y| < tan(pi/8) x: the degree-10 polynomial of
+//   rh_atan2 on t = y/x, the operations rh_atan2 performs for such a lane
+//   (its octant reduction is the identity there), so this tier and rh_atan2
+//   give identical results;
+// - any other wave: rh_atan2.
+// A lane's result therefore depends on which points share its wave: a lane
+// below pi/16 gets the short polynomial or the long one as its wave-mates
+// decide, and the two differ by rounding (about 1e-17 absolute).  A sweep's
+// result for one point can change at that level with the order or batching
+// of the points.
 RH_HD double rh_atan2_wave(double y, double x)
 {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -308,6 +315,10 @@ RH_HD double rh_atan2_wave(double y, double x)
 #endif
     return rh_atan2(y, x);
 }
+// FP32 counterpart: degree 2 below pi/16 (within 9e-9 of atan t in float
+// arithmetic, 1.6e-9 in exact arithmetic) and degree 4 below pi/8 (1.8e-8);
+// the tiers differ by float rounding, so a point's result can change by a few
+// 1e-8 of an angle with the order or batching of the points.
 RH_HD float rh_atan2_wave(float y, float x)
 {
 #ifdef __HIP_DEVICE_COMPILE__

@@ -189,6 +189,103 @@ def cut_convex(verts, faces, normal, offset, tol):
     return compact(lower), compact(upper)
 
 
+class LayerError(ValueError):
+    """A layer structure the reference refuses: code 111 (a non-convex layer
+    polygon) or 63 (any other improper layer definition)."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def _ccw_layer(pts, tol):
+    """Layer points as a counter-clockwise convex loop (1 or 2 points are
+    returned as given).  LayerError 111 for a non-convex polygon, 63 for a
+    degenerate one."""
+    P = np.asarray(pts, dtype=np.float64).reshape(-1, 2)
+    if len(P) < 3:
+        return P
+    x, y = P[:, 0], P[:, 1]
+    area2 = float(np.sum(x * np.roll(y, -1) - np.roll(x, -1) * y))
+    size = float(np.ptp(P, axis=0).max())
+    if abs(area2) <= tol * size * size:
+        raise LayerError(63, "degenerate layer polygon")
+    if area2 < 0:
+        P = P[::-1].copy()
+    e = np.roll(P, -1, axis=0) - P
+    cr = e[:, 0] * np.roll(e, -1, axis=0)[:, 1] - e[:, 1] * np.roll(e, -1, axis=0)[:, 0]
+    if (cr < -tol * size * size).any():
+        raise LayerError(111, "non-convex layer polygon")
+    return P
+
+
+def layer_segment(pa, za, pb, zb, tol=1e-9):
+    """Convex polyhedron spanned by two parallel convex layers: polygon pa at
+    z = za and polygon pb at z = zb, each of 1 or 2 points (an apex or a
+    ridge) or a convex polygon.  Returns (verts, faces): the points of pa then
+    those of pb, the layer polygons as faces when they have 3 or more points,
+    and the lateral faces of the hull: each layer edge with the other layer's
+    supporting vertex in the edge's outward direction (a triangle), or with
+    the other layer's parallel edge (a quadrilateral).  Radia's segment
+    construction (radTApplication::SetUpOnePolyhedronSegment,
+    cpp/src/core/radapl4.cpp:519-700) builds the same faces by searching
+    each edge's lowest point on the other layer."""
+    A = _ccw_layer(pa, tol)
+    B = _ccw_layer(pb, tol)
+    m, k = len(A), len(B)
+    if m < 3 and k < 3 and not (m == 2 and k == 2):
+        raise LayerError(63, "two consecutive layers of fewer than 3 points must both hold 2 points")
+    size = max(float(np.ptp(np.concatenate([A, B]), axis=0).max()), abs(zb - za))
+    if m == 2 and k == 2:
+        ea, eb = A[1] - A[0], B[1] - B[0]
+        if abs(ea[0] * eb[1] - ea[1] * eb[0]) <= tol * size * size:
+            raise LayerError(63, "parallel ridges in consecutive layers")
+
+    def edges(P, off):
+        # (outward normal angle, start index, end index) of each edge; a
+        # two-point layer has both directions of its one segment
+        out = []
+        n = len(P)
+        if n < 2:
+            return out
+        for i in range(n if n > 2 else 2):
+            j = (i + 1) % n
+            e = P[j] - P[i]
+            out.append((math.atan2(-e[0], e[1]), off + i, off + j, i))
+        return out
+
+    def support(P, ang):
+        u = np.array([math.cos(ang), math.sin(ang)])
+        return int(np.argmax(P @ u))
+
+    verts = np.array([[x, y, za] for x, y in A] + [[x, y, zb] for x, y in B])
+    faces = []
+    if m >= 3:
+        faces.append(list(range(m)))
+    if k >= 3:
+        faces.append(list(range(m, m + k)))
+    ea, eb = edges(A, 0), edges(B, m)
+    used_b = set()
+    angtol = 1e-9
+    for (ang, i0, i1, _i) in ea:
+        par = None
+        for (bang, j0, j1, j) in eb:
+            d = math.remainder(bang - ang, 2.0 * math.pi)
+            if abs(d) <= angtol and j not in used_b:
+                par = (j0, j1, j)
+                break
+        if par is not None:
+            used_b.add(par[2])
+            faces.append([i0, i1, par[1], par[0]])
+        else:
+            faces.append([i0, i1, m + support(B, ang)])
+    for (bang, j0, j1, j) in eb:
+        if j in used_b:
+            continue
+        faces.append([j0, j1, support(A, bang)])
+    return verts, faces
+
+
 def division_planes(lo, hi, count, ratio):
     """Offsets of the count-1 cutting planes between the extreme vertex
     projections lo < hi, the piece widths in geometric progression with

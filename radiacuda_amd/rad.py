@@ -62,23 +62,39 @@ def ObjPolyhdr(V, F, M=(0, 0, 0), *args, **kw):
     return _add(O.Polyhedron(np.asarray(V, dtype=np.float64), faces, _vec3(M)))
 
 
+_ERROR063 = ("Radia::Error063::::Multiple extruded polygon can not be generated from this input: "
+             "incorrect definition of horizontal slice polygons.")
+_ERROR111 = ("Radia::Error111::::Multiple extruded polygon can not be generated from this input: "
+             "Non-convex horizontal slice polygon encountered.")
+
+
 def ObjMltExtPgn(slices, M=(0, 0, 0)):
-    """Polyhedra between consecutive planar slices [[[x,y],...], z] (radapl1.cpp:859-880)."""
+    """Convex polyhedra between consecutive horizontal layers
+    [[[x, y], ...], z] (radpy.cpp ObjMltExtPgn; radapl1.cpp:859-880,
+    radapl4.cpp:435-510, 1036-1064).  A layer holds 1 point (an apex), 2 points
+    (a ridge) or a convex polygon, and consecutive layers may hold different
+    numbers of points; the z values are distinct and strictly monotonic.  Each
+    layer pair becomes one polyhedron, the convex hull of the two layers
+    (objects.layer_segment); several are grouped in a container.  Errors as
+    the reference's: Error111 for a non-convex layer polygon, Error063 for
+    any other improper layer structure."""
     M = _vec3(M)
+    try:
+        layers = [(np.asarray(p, dtype=np.float64).reshape(-1, 2), float(z)) for p, z in slices]
+    except (TypeError, ValueError):
+        raise RuntimeError(_ERROR063)
+    if len(layers) < 2 or max(len(p) for p, _z in layers) < 2:
+        raise RuntimeError(_ERROR063)
+    dz = np.diff([z for _p, z in layers])
+    if not ((dz > 0).all() or (dz < 0).all()):
+        raise RuntimeError(_ERROR063)
     ids = []
-    for (pa, za), (pb, zb) in zip(slices[:-1], slices[1:]):
-        pa = np.asarray(pa, dtype=np.float64)
-        pb = np.asarray(pb, dtype=np.float64)
-        if len(pa) != len(pb):
-            raise RuntimeError("slices must have equal vertex counts")
-        k = len(pa)
-        verts = [[x, y, za] for x, y in pa] + [[x, y, zb] for x, y in pb]
-        faces = [list(range(k)), list(range(k, 2 * k))]
-        for i in range(k):
-            j = (i + 1) % k
-            faces.append([i, j, k + j, k + i])
-        o = O.Polyhedron(np.asarray(verts), faces, M)
-        ids.append(_add(o))
+    for (pa, za), (pb, zb) in zip(layers[:-1], layers[1:]):
+        try:
+            verts, faces = O.layer_segment(pa, za, pb, zb)
+        except O.LayerError as e:
+            raise RuntimeError(_ERROR111 if e.code == 111 else _ERROR063)
+        ids.append(_add(O.Polyhedron(verts, faces, M)))
     return ids[0] if len(ids) == 1 else ObjCnt(ids)
 
 

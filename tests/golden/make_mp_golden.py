@@ -9,6 +9,9 @@ The closed form is what the mathematics fixes: H = (1/4pi) sum_f sigma_f
 [sum_e L_e m_e - Omega_f n_f], sigma_f = M.n_f (face-charge model of a
 uniformly magnetized polyhedron), evaluated at 50 digits.
 
+Also writes mp50_yoke.json (see yoke_flagged) and, with the argument
+"full", mp50_yoke_full.json (see yoke_full).
+
 Run from the repository root: python tests/golden/make_mp_golden.py
 """
 import json
@@ -23,7 +26,7 @@ sys.path.insert(0, ROOT)
 mp.mp.dps = 50
 
 
-def H_mp(verts, faces, M, p):
+def H_mp(verts, faces, M, p, as_float=True):
     V = [[mp.mpf(float(x)) for x in r] for r in verts]
     P = [mp.mpf(float(x)) for x in p]
     dot = lambda a, b: sum(a[i] * b[i] for i in range(3))
@@ -60,7 +63,7 @@ def H_mp(verts, faces, M, p):
             om += 2 * mp.atan2(num, den)
         for k in range(3):
             H[k] -= sig * om * n[k] / (4 * mp.pi)
-    return [float(x) for x in H]
+    return [float(x) for x in H] if as_float else H
 
 
 def main():
@@ -97,7 +100,137 @@ def main():
     json.dump({"generator": "tests/golden/make_mp_golden.py", "digits": 50, "cases": cases},
               open(out, "w"), indent=1)
     print("wrote", out, len(cases))
+    yoke_flagged()
+
+
+def Q_mp(verts, faces, p):
+    """3x3 block H = Q M of one polyhedron at p, column by column, 50 digits."""
+    Q = np.zeros((3, 3))
+    for c in range(3):
+        M = [0.0, 0.0, 0.0]
+        M[c] = 1.0
+        Q[:, c] = H_mp(verts, faces, M, p)
+    return Q
+
+
+def yoke_flagged():
+    """mp50_yoke.json: 50-digit blocks of the C-yoke elements that the
+    oracle cannot reproduce to 1e-10.
+
+    - C-yoke (8,000 tetrahedra): the elements with a face tilted within 1e-7 of
+      -z (the reference's face rotation drops the tilt), and the eight
+      unflagged elements with the largest oracle discrepancy at gap points,
+      each at 8 seeded gap points;
+    - large yoke (40,000 tetrahedra): its tilted-face elements at the
+      collocation points (vertex means) of rows 0, 12345 and 39999 and at 4
+      seeded gap points.
+    """
+    from radiacuda_amd import objects as O
+    from radiacuda_amd import rad
+    from radiacuda_amd import workloads as W
+    from tests import models
+    out = {"generator": "tests/golden/make_mp_golden.py", "digits": 50}
+    rad.UtiDelAll()
+    _top, iron, _coil, info = W.c_yoke()
+    items = list(O.walk(rad._reg, iron))
+    flagged = [int(e) for e in models.tilted_minus_z(items)]
+    worst_unflagged = [6141, 6743, 3319, 6090, 5351, 4953, 3488, 7032]
+    rng = np.random.default_rng(7)
+    pts = W.gap_grid(info, 100)
+    ps = pts[rng.choice(len(pts), 8, replace=False)]
+    blocks = {}
+    for e in flagged + worst_unflagged:
+        o = items[e][0]
+        blocks[str(e)] = [Q_mp(o.verts, o.faces, x).tolist() for x in ps]
+    out["c_yoke"] = {"flagged": flagged, "unflagged_worst": worst_unflagged, "points": ps.tolist(),
+                     "Q": blocks}
+    rad.UtiDelAll()
+    _top, iron, _coil, info = W.c_yoke(ny=50)
+    items = list(O.walk(rad._reg, iron))
+    flagged = [int(e) for e in models.tilted_minus_z(items)]
+    rows = [0, 12345, 39999]
+    X = np.array([np.mean(np.asarray(items[i][0].verts), axis=0) for i in rows])
+    rng = np.random.default_rng(8)
+    pts = W.gap_grid(info, 20)
+    ps = pts[rng.choice(len(pts), 4, replace=False)]
+    allp = np.concatenate([X, ps])
+    blocks = {}
+    for e in flagged:
+        o = items[e][0]
+        blocks[str(e)] = [Q_mp(o.verts, o.faces, x).tolist() for x in allp]
+    out["large_yoke"] = {"flagged": flagged, "rows": rows, "colloc": X.tolist(), "points": ps.tolist(),
+                         "Q": blocks}
+    rad.UtiDelAll()
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mp50_yoke.json")
+    json.dump(out, open(path, "w"), indent=0)
+    print("wrote", path)
+
+
+def synthetic_M(n):
+    """Magnetization of element j, exact in binary64 on any machine:
+    ((7919 j mod 997)/997 - 1/2, (104729 j mod 991)/991 - 1/2,
+    1 + (1299709 j mod 983)/(2 * 983))."""
+    j = np.arange(n, dtype=np.int64)
+    return np.stack([((7919 * j) % 997) / 997.0 - 0.5, ((104729 * j) % 991) / 991.0 - 0.5,
+                     1.0 + ((1299709 * j) % 983) / (2.0 * 983.0)], axis=1)
+
+
+def _field_chunk(args):
+    key, lo, hi = args
+    from radiacuda_amd import objects as O
+    from radiacuda_amd import rad
+    from radiacuda_amd import workloads as W
+    rad.UtiDelAll()
+    _top, iron, _coil, _info = W.c_yoke(**({} if key == "c_yoke" else {"ny": 50}))
+    items = list(O.walk(rad._reg, iron))
+    g = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "mp50_yoke.json")))[key]
+    ps = np.array(g["points"])
+    M = synthetic_M(len(items))
+    acc = [[mp.mpf(0)] * 3 for _ in ps]
+    for e in range(lo, hi):
+        o = items[e][0]
+        for i, x in enumerate(ps):
+            h = H_mp_exact(o.verts, o.faces, M[e], x)
+            for k in range(3):
+                acc[i][k] += h[k]
+    return [[mp.nstr(v, 40) for v in row] for row in acc]
+
+
+def H_mp_exact(verts, faces, M, p):
+    """H_mp without the final rounding to float."""
+    return H_mp(verts, faces, M, p, as_float=False)
+
+
+def yoke_full():
+    """mp50_yoke_full.json: B (= H, the points lie outside the iron) of the
+    whole C-yoke (8,000 tetrahedra) and large yoke (40,000) with the
+    magnetization synthetic_M, at the gap points of mp50_yoke.json, summed in
+    50-digit arithmetic over every element."""
+    import multiprocessing as mpr
+    out = {"generator": "tests/golden/make_mp_golden.py full", "digits": 50,
+           "M": "synthetic_M (tests/golden/make_mp_golden.py)"}
+    for key, n in (("c_yoke", 8000), ("large_yoke", 40000)):
+        step = 500
+        jobs = [(key, lo, min(n, lo + step)) for lo in range(0, n, step)]
+        with mpr.Pool(8) as pool:
+            parts = pool.map(_field_chunk, jobs)
+        tot = [[mp.mpf(0)] * 3 for _ in parts[0]]
+        for part in parts:
+            for i, row in enumerate(part):
+                for k in range(3):
+                    tot[i][k] += mp.mpf(row[k])
+        g = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "mp50_yoke.json")))[key]
+        out[key] = {"points": g["points"], "B": [[float(v) for v in row] for row in tot]}
+        print(key, "done", flush=True)
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mp50_yoke_full.json")
+    json.dump(out, open(path, "w"), indent=1)
+    print("wrote", path)
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "yoke":
+        yoke_flagged()
+    elif len(sys.argv) > 1 and sys.argv[1] == "full":
+        yoke_full()
+    else:
+        main()

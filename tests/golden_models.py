@@ -1,7 +1,8 @@
 """The models behind the reference's golden solve results
 (tests/_mpi_solve_reference.json), rebuilt through the drop-in API exactly as
-tests/test_mpi_solve.py:33-80 builds them, and the oracle's end-to-end solve
-of them (assembly, method-4 relaxation, field, coil)."""
+tests/test_mpi_solve.py:33-80 builds them ("small", "mixed" and "large"), and
+the oracle's end-to-end solve of them (assembly, method-4 relaxation, field,
+coil)."""
 import numpy as np
 
 from oracle import oracle as OR
@@ -14,6 +15,17 @@ POINTS = [[25.0, 10.0, 0.0], [60.0, 30.0, 5.0]]
 def build(kind="small"):
     rad.UtiDelAll()
     rng = np.random.default_rng(11)
+    if kind == "large":
+        # 1,100 RecMag blocks (tests/test_mpi_solve.py:37-46), no symmetries
+        ids = []
+        for _ in range(1100):
+            center = rng.uniform([-80, -80, -40], [80, 80, 40]).tolist()
+            dims = rng.uniform(2.0, 4.0, 3).tolist()
+            ids.append(rad.ObjRecMag(center, dims))
+        iron = rad.ObjCnt(ids)
+        rad.MatApl(iron, rad.MatSatIsoFrm([20000, 2], [0.1, 2], [0.1, 2]))
+        coil = rad.ObjRaceTrk([0, 0, 30], [70, 90], [0, 0], 15, 10, 3.0, "man", "z")
+        return rad.ObjCnt([iron, coil]), iron, coil
     faces = [[1, 2, 3], [1, 4, 2], [2, 4, 3], [3, 4, 1]]
     ids = []
     for _ in range(60):
@@ -50,7 +62,7 @@ def coil_field_oracle(coil, pts):
     return B
 
 
-def oracle_solve(kind="small", prec=1e-5, max_iter=3000):
+def oracle_solve(kind="small", prec=1e-5, max_iter=3000, return_state=False):
     top, iron, coil = build(kind)
     items = [it for it in O.walk(rad._reg, iron)]
     P = O.pack_elements(items)
@@ -67,4 +79,6 @@ def oracle_solve(kind="small", prec=1e-5, max_iter=3000):
     M, _H, _it, stats = OR.relax_m4(Q.astype(np.float32), mats, hext, prec, max_iter)
     B, _ = OR.field(om, M.ravel(), POINTS)
     B = B + coil_field_oracle(coil, POINTS)
+    if return_state:
+        return B, stats[0], M
     return B, stats[0]

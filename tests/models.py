@@ -240,3 +240,30 @@ def hybrid_undulator(n_periods=2, period=46.0, gap=20.0, offset=1.0, pole=(45.0,
     rad.TrfZerPara(top, [0, 0, 0], [0, 0, 1])
     rad.TrfZerPerp(top, [0, 0, 0], [0, 1, 0])
     return top, steel
+
+
+def tilted_minus_z(items, tol=1e-7):
+    """Indices of polyhedra with a face whose outward unit normal n satisfies
+    0 < |1 + n_z| <= tol (tol = the reference's RelLengthTol) without being
+    exactly -z.  For such a face the reference's face rotation
+    (radvlpgn.cpp:357-371) takes the axis-aligned frame diag(1, -1, -1) and
+    drops the face's tilt, so the oracle, which restates it, is off by up to
+    ~1e-3 of the element's field there; a face exactly normal to -z loses
+    nothing."""
+    out = []
+    for e, it in enumerate(items):
+        o = it[0]
+        if not hasattr(o, "faces"):
+            continue
+        V = np.asarray(o.verts, dtype=np.float64)
+        c = V.mean(axis=0)
+        for f in o.faces:
+            a, b, d = V[f[0]], V[f[1]], V[f[2]]
+            nv = np.cross(b - a, d - a)
+            nv /= np.linalg.norm(nv)
+            if np.dot(nv, a - c) < 0:
+                nv = -nv
+            if abs(1.0 + nv[2]) <= tol and (nv[0] != 0.0 or nv[1] != 0.0):
+                out.append(e)
+                break
+    return np.asarray(out, dtype=np.int64)

@@ -42,6 +42,40 @@ def test_fp64_degree6_polynomial_within_bound():
     assert (err[nz] / np.abs(np.arctan(t[nz]))).max() <= 5e-16
 
 
+def _mp_fit_error(c, hi, count=2001):
+    """max |t + t^3 p(t^2) - atan t| over [0, hi] in 40-digit arithmetic (the
+    polynomial is odd), with the coefficients as stored."""
+    import mpmath as mp
+    mp.mp.dps = 40
+    cc = [mp.mpf(float(v)) for v in c]
+    worst = mp.mpf(0)
+    for i in range(count):
+        t = mp.mpf(hi) * i / (count - 1)
+        u = t * t
+        p = cc[-1]
+        for v in reversed(cc[:-1]):
+            p = p * u + v
+        worst = max(worst, abs(t + t * u * p - mp.atan(t)))
+    return float(worst)
+
+
+def test_fp64_degree6_fit_error_in_exact_arithmetic():
+    """The fit itself, free of float64 rounding: within 1e-17 of atan on the
+    whole tier (7.4e-18 documented).  numpy's arctan carries about 1 ulp of its
+    own error, so only exact arithmetic resolves this claim."""
+    src = _source()
+    body = re.search(r"d_atan16_p\[7\]\s*=\s*\{([^}]*)\}", src).group(1)
+    c = [float(v) for v in body.replace("\n", " ").split(",")]
+    assert _mp_fit_error(c, np.tan(np.pi / 16)) <= 1e-17
+
+
+def test_fp32_degree2_fit_error_in_exact_arithmetic():
+    src = _source()
+    m = re.search(r"const float p = ([-0-9.]+)f \+ u \* \(([-0-9.]+)f \+ u \* ([-0-9.]+)f\);", src)
+    c = [float(np.float32(m.group(i))) for i in (1, 2, 3)]
+    assert _mp_fit_error(c, float(np.float32(0.19891237))) <= 2e-9
+
+
 def test_fp32_degree2_polynomial_within_bound():
     src = _source()
     m = re.search(r"const float p = ([-0-9.]+)f \+ u \* \(([-0-9.]+)f \+ u \* ([-0-9.]+)f\);", src)

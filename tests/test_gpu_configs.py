@@ -9,6 +9,8 @@ points) and size-independent properties elsewhere.
 * Halbach ring: 20,000 hexahedral polyhedra, MatLin(0.06/0.17, 1.2 T), B at
   2,000,000 points in the bore, FP32 vs FP64 field kernels.
 """
+import json
+import os
 import time
 
 import numpy as np
@@ -20,8 +22,11 @@ from radiacuda_amd import objects as O
 from radiacuda_amd import rad
 from radiacuda_amd import workloads as W
 from tests import models
+from tests.golden.make_mp_golden import synthetic_M
 
 pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 @pytest.fixture(autouse=True)
@@ -138,36 +143,83 @@ def test_large_yoke_120k_matrix():
         assert res.converged and res.misfit <= 1e-5
         out[prec] = (R, M)
         Q.close()
-    # sampled rows against the host build of the same integrals (exact parity)
-    # and against the oracle, whose edge line form loses up to ~1e-8 absolute
-    # on near-vertical face-frame edges of jittered tetrahedra (pinned against
-    # 50-digit values in tests/golden/mp50_tets.json)
+    # sampled rows: against the host build of the same integrals, against the
+    # oracle on every column but the tilted-face elements (a face within 1e-7
+    # of -z: the reference's face rotation drops its tilt, radvlpgn.cpp:357-371),
+    # and those columns against 50-digit values (tests/golden/mp50_yoke.json)
     R64, M64 = out[N.RH_FP64]
     R32, M32 = out[N.RH_MIXED]
     Rh = np.concatenate([_host_rows(P, i) for i in rows])
     om = OR.prepare_model(P)
     Ro = np.concatenate([OR.assemble(om, rows=(i, i + 1)) for i in rows])
+    flagged = models.tilted_minus_z(items)
+    gold = json.load(open(os.path.join(HERE, "golden", "mp50_yoke.json")))["large_yoke"]
+    assert sorted(flagged.tolist()) == sorted(gold["flagged"]) and gold["rows"] == rows
+    fc = np.zeros(3 * n, bool)
+    for e in flagged:
+        fc[3 * e:3 * e + 3] = True
     scale = np.abs(Ro).max()
-    assert np.abs(R64 - Rh).max() <= 1e-12 * scale
-    assert np.abs(R64 - Ro).max() <= 1e-6 * scale
+    e_host = np.abs(R64 - Rh).max() / scale
+    e_unfl = np.abs(R64 - Ro)[:, ~fc].max() / scale
+    e_fl = e_blk = 0.0
+    for r, i in enumerate(rows):
+        assert np.abs(X[i] - np.array(gold["colloc"][r])).max() <= 1e-13 * np.abs(X[i]).max()
+        rs = np.abs(Ro[3 * r:3 * r + 3]).max()
+        for e in flagged:
+            Qmp = np.array(gold["Q"][str(e)][r])
+            Qg = R64[3 * r:3 * r + 3, 3 * e:3 * e + 3]
+            e_fl = max(e_fl, np.abs(Qg - Qmp).max() / rs)
+            e_blk = max(e_blk, np.abs(Qg - Qmp).max() / np.abs(Qmp).max())
+    print("large yoke rows: vs host build %.2e, vs oracle without the %d tilted-face columns %.2e "
+          "(all columns %.2e); tilted-face blocks vs 50 digits %.2e of the row, %.2e per block"
+          % (e_host, len(flagged), e_unfl, np.abs(R64 - Ro).max() / scale, e_fl, e_blk), flush=True)
+    assert e_host <= 1e-12
+    assert e_unfl <= 1e-10
+    assert e_fl <= 1e-12
+    assert e_blk <= 1e-10
     assert np.abs(R32 - R64).max() <= 1e-7 * scale
     # both solves reach the same state to within the stopping tolerance
     dM = np.abs(M32 - M64).max() / np.abs(M64).max()
     print("large yoke: refined mixed vs FP64 solve, max |dM|/max|M| = %.2e" % dM, flush=True)
-    assert dM <= 1e-3
-    # field at sampled gap points: against the host build of the same
-    # integrals, and against the oracle to the accuracy of the reference's
-    # face rotation on this mesh's nearly horizontal faces (an element off by
-    # 3.9e-7 absolute is pinned at 50 digits in tests/golden/mp50_tets.json)
+    assert dM <= 1e-4
+    # field at gap points: host build; oracle without the tilted-face
+    # elements; those elements against 50-digit values
     pts = W.gap_grid(info, 20)[::97]
     B, _H = model.field(M64, pts)
     Bh, _Hh = _host_field(P, M64, pts)
-    Bo, _Ho, _bad = models.oracle_field(P, M64, pts)
-    scale = np.abs(Bo).max()
-    print("large yoke field: vs host build %.2e, vs oracle %.2e"
-          % (np.abs(B - Bh).max() / scale, np.abs(B - Bo).max() / scale), flush=True)
-    assert np.abs(B - Bh).max() <= 1e-12 * scale
-    assert np.abs(B - Bo).max() <= 1e-5 * scale
+    Mz = M64.copy()
+    Mz[flagged] = 0.0
+    Bz = model.field(Mz, pts)[0]
+    Boz, _Ho, _bad = models.oracle_field(P, Mz, pts)
+    gp = np.array(gold["points"])
+    Mf = np.zeros_like(M64)
+    Mf[flagged] = M64[flagged]
+    Bf = model.field(Mf, gp)[0]
+    Bf_mp = np.zeros_like(Bf)
+    for e in flagged:
+        Bf_mp += np.einsum("pab,b->pa", np.array(gold["Q"][str(e)][len(rows):]), M64[e])
+    e_fl = np.abs(Bf - Bf_mp).max() / np.abs(model.field(M64, gp)[0]).max()
+    scale = np.abs(Boz).max()
+    print("large yoke field: vs host build %.2e, vs oracle without the tilted-face elements %.2e; "
+          "tilted-face elements vs 50 digits %.2e of the field"
+          % (np.abs(B - Bh).max() / np.abs(Bh).max(), np.abs(Bz - Boz).max() / scale, e_fl), flush=True)
+    # the whole model with a fixed synthetic magnetization against its field
+    # summed over all 40,000 elements at 50 digits
+    full = json.load(open(os.path.join(HERE, "golden", "mp50_yoke_full.json")))["large_yoke"]
+    Ms = synthetic_M(n)
+    fp = np.array(full["points"])
+    Bmp = np.array(full["B"])
+    e_mp = np.abs(model.field(Ms, fp)[0] - Bmp).max() / np.abs(Bmp).max()
+    Bmp_o = models.oracle_field(P, Ms, fp)[0]
+    print("large yoke whole-model field vs 50 digits: FP64 device path %.2e, oracle %.2e"
+          % (e_mp, np.abs(Bmp_o - Bmp).max() / np.abs(Bmp).max()), flush=True)
+    assert e_mp <= 1e-12
+    assert np.abs(B - Bh).max() <= 1e-12 * np.abs(Bh).max()
+    # the oracle's own error on the other elements is 5e-10 of this field (its
+    # edge and face-frame forms lose up to 4e-8 of a distant element's field;
+    # the 50-digit sum above pins the device path at 1e-14)
+    assert np.abs(Bz - Boz).max() <= 2e-9 * scale
+    assert e_fl <= 1e-12
     model.close()
 
 
@@ -195,7 +247,7 @@ def test_halbach_field_2m_points():
     err32 = np.abs(Bs[N.RH_MIXED] - Bs[N.RH_FP64]).max() / scale
     print("halbach: solve %.3f s; field FP64 %.2f s, FP32 %.2f s (2M points); FP32 vs FP64 %.2e"
           % (t_solve, times[N.RH_FP64], times[N.RH_MIXED], err32), flush=True)
-    assert err32 <= 1e-5
+    assert err32 <= 1e-6
     rng = np.random.default_rng(3)
     samp = rng.choice(len(pts), 48, replace=False)
     Bo, _Ho, _bad = models.oracle_field(P, M, pts[samp])

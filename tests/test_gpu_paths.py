@@ -301,3 +301,74 @@ def test_repeated_solve_and_field_are_bit_identical(oracle_off):
     for a, b in zip(runs[0][1:], runs[1][1:]):
         assert np.array_equal(a, b)
     rad.UtiDel(k)
+
+
+@pytest.mark.parametrize("shape", ["recmag", "rotated_polyhedron"])
+@pytest.mark.parametrize("mr", [1.2, [0.4, -0.7, 0.9]])
+def test_matlin_anisotropic_cube_closed_form(shape, mr):
+    """One cube of MatLin([ksi_par, ksi_perp], mr) in a uniform background
+    field H0.  A cube's self field at its centre is -M/3 in any orientation,
+    so the relaxed magnetization solves (I + K/3) M = K H0 + M_r exactly, with
+    K = ksi_perp I + (ksi_par - ksi_perp) L L^T built here from the
+    reference's component formula (radmater.h:292-303): L is the element's
+    initial magnetization direction for a scalar mr (radmater.h:235-250), the
+    direction of the vector mr otherwise, and M_r = |mr| L.  B at the centre
+    is H0 + 2M/3."""
+    print("case: MatLin cube closed form", shape, mr, flush=True)
+    kpar, kper = 0.06, 0.17
+    m0 = np.array([0.3, 0.5, 0.8])
+    if shape == "recmag":
+        c = rad.ObjRecMag([1.0, -2.0, 3.0], [10.0, 10.0, 10.0], list(m0))
+        ctr = np.array([1.0, -2.0, 3.0])
+    else:
+        R = np.linalg.qr(np.random.default_rng(9).normal(size=(3, 3)))[0]
+        ctr = np.array([4.0, 1.0, -2.0])
+        c = models.hex_block(ctr, [10.0, 10.0, 10.0], m0, rot=R)
+    rad.MatApl(c, rad.MatLin([kpar, kper], mr))
+    H0 = np.array([0.05, -0.2, 0.3])
+    top = rad.ObjCnt([c, rad.ObjBckg(list(H0))])
+    k = rad.RlxPre(top)
+    res = rad.RlxAuto(k, 1e-13, 1000)
+    assert res[0] <= 1e-13
+    M = np.stack([o.M for (o, _k, _s) in rad._reg[k].items])[0]
+    if np.ndim(mr) == 0:
+        L = m0 / np.linalg.norm(m0)
+        Mr = mr * L
+    else:
+        Mr = np.asarray(mr, dtype=np.float64)
+        L = Mr / np.linalg.norm(Mr)
+    d = kpar - kper
+    K = np.array([[kpar * L[0] ** 2 + kper * (L[1] ** 2 + L[2] ** 2), d * L[0] * L[1], d * L[0] * L[2]],
+                  [d * L[0] * L[1], kpar * L[1] ** 2 + kper * (L[0] ** 2 + L[2] ** 2), d * L[1] * L[2]],
+                  [d * L[0] * L[2], d * L[1] * L[2], kpar * L[2] ** 2 + kper * (L[0] ** 2 + L[1] ** 2)]])
+    Mx = np.linalg.solve(np.eye(3) + K / 3.0, K @ H0 + Mr)
+    assert np.abs(M - Mx).max() <= 1e-10 * np.abs(Mx).max()
+    B = np.array(rad.Fld(top, "b", list(ctr)))
+    Bx = H0 + 2.0 * Mx / 3.0
+    assert np.abs(B - Bx).max() <= 1e-10 * np.abs(Bx).max()
+
+
+def test_sweep_results_depend_on_point_order_only_at_rounding_level():
+    """The wave-uniform atan2 tiers (rh_atan2_wave) pick a polynomial per wave,
+    so a point's result depends on which points share its wave, within the
+    tiers' agreement: ~1e-17 of an angle in FP64, a few 1e-8 in FP32.  A
+    permuted and a strided subset of the same points give the same fields to
+    1e-12 (FP64) and 1e-6 (FP32 integrals) of the field scale."""
+    print("case: point order / batching of the mesh-form sweep", flush=True)
+    ids = models.tet_mesh((4, 3, 3), 5.0, jitter=0.1, seed=12)
+    top = rad.ObjCnt(ids)
+    P, _items = models.packed(top)
+    rng = np.random.default_rng(13)
+    pts = np.concatenate([rng.uniform(-5, 25, (30000, 3)), rng.uniform(-60, 80, (30000, 3))])
+    perm = rng.permutation(len(pts))
+    with N.Model(P) as m:
+        assert m.field_form() == 2
+        for prec, tol in ((N.RH_FP64, 1e-12), (N.RH_MIXED, 1e-6)):
+            B = m.field(P["M"], pts, precision=prec)[0]
+            Bp = np.empty_like(B)
+            Bp[perm] = m.field(P["M"], pts[perm], precision=prec)[0]
+            Bs = m.field(P["M"], pts[::7], precision=prec)[0]
+            s = np.abs(B).max()
+            dp, ds = np.abs(Bp - B).max() / s, np.abs(Bs - B[::7]).max() / s
+            print("prec %d: permuted %.2e, subset %.2e" % (prec, dp, ds), flush=True)
+            assert dp <= tol and ds <= tol

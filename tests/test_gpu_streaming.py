@@ -83,3 +83,38 @@ def test_fallback_policy(monkeypatch):
         rad.UtiDel(k)
     finally:
         rad.UtiGpuFallback("cpu")
+
+
+def test_streamed_nonlinear_solve_reaches_fixed_point():
+    """Half the rows streamed from pinned host memory: the nonlinear (tanh
+    steel) solve reaches the FP64 fixed point of the matrix it stores (Newton
+    reference on the exported matrix) and that of the oracle's matrix, as the
+    in-core solve does (radgpurlx.cu:258-459 is the reference's streamed
+    pass)."""
+    from oracle import oracle as OR
+    print("case: streamed nonlinear solve vs FP64 fixed point", flush=True)
+    OR.set_randomization(False)
+    ids = models.tet_mesh((3, 3, 2), 4.0, jitter=0.1, seed=7, M=[0, 0, 0])
+    iron = rad.ObjCnt(ids)
+    rad.MatApl(iron, rad.MatSatIsoFrm([2118, 1.362], [63.06, 0.2605], [17.138, 0.4917]))
+    src = rad.ObjRecCur([-8, 6, 4], [4, 12, 8], [0, 0, 2.0])
+    P, items = models.packed(iron)
+    n = len(items)
+    mats = np.stack([o.mat.record(o.M) for (o, _k, _s) in items])
+    with N.Model(P) as model:
+        hext = N.current_field(O.pack_currents(list(O.walk(rad._reg, src))), model.colloc())
+        Qo, _jit = models.oracle_assemble(P)
+        for prec in (N.RH_FP64, N.RH_MIXED):
+            Q = model.assemble(prec, resident_fraction=0.5, allow_streaming=True)
+            res_rows, str_rows = Q.layout()
+            assert str_rows == 3 * (n - n // 2)
+            r, M, _H = Q.relax(mats, hext, np.zeros((n, 3)), 1e-11, 3000)
+            assert r.converged and r.misfit <= 1e-11
+            Qs = Q.export()
+            Ms = models.fp64_fixed_point(Qs if prec == N.RH_FP64 else Qo, mats, hext, M0=M)
+            err = np.abs(M - Ms).max() / np.abs(Ms).max()
+            print("streamed %s: %d resident / %d streamed rows, %d iterations, vs FP64 fixed point %.2e"
+                  % ("fp64" if prec == N.RH_FP64 else "mixed+refine", res_rows, str_rows, r.iters, err),
+                  flush=True)
+            assert err <= 1e-7
+            Q.close()

@@ -94,3 +94,32 @@ def test_field_list_and_manual_relaxation():
     b3 = rad.FldLst(top, "b", p1, p2, 11, "noarg")
     assert len(b3) == 11 and all(len(v) == 3 for v in b3)
     assert len(rad.FldLst(top, "bz", p1, p2, 1, "noarg")) == 101
+
+
+def test_example04_sphere_on_the_gpu():
+    """The reference's examples/RADIA_Example04.py: a 15 x 15 polyhedral
+    sphere from ObjMltExtPgn (apex layers, 15-gon layers), M = 1 T along x.
+    rad.Fld on the GPU: B inside is 2/3 M up to the polyhedral approximation
+    (the example's claim), and the device field equals the oracle's to 1e-10
+    at points off the face planes, inside and outside."""
+    from tests.test_mltextpgn import example04_sphere
+    print("case: Example04 sphere (ObjMltExtPgn)", flush=True)
+    rad.UtiDelAll()
+    rad.FldLenRndSw("off")
+    OR.set_randomization(False)
+    s = example04_sphere()
+    b0 = np.array(rad.Fld(s, "b", [0, 0, 0]))
+    assert abs(b0[0] - 2.0 / 3.0) <= 2e-3 and np.abs(b0[1:]).max() <= 1e-3
+    ys = np.linspace(-0.99, 0.99, 301)
+    bx = np.array(rad.Fld(s, "bx", [[0, y, 0] for y in ys]))
+    assert np.abs(bx[np.abs(ys) <= 0.9] - 2.0 / 3.0).max() <= 5e-3
+    P, _items = models.packed(s)
+    rng = np.random.default_rng(4)
+    pts = np.concatenate([rng.uniform(-0.7, 0.7, (200, 3)), rng.uniform(-3, 3, (200, 3))])
+    B = np.array(rad.Fld(s, "b", pts.tolist()))
+    Bo, _Ho, bad = models.oracle_field(P, P["M"], pts)
+    assert not bad.any()
+    assert np.abs(B - Bo).max() <= 1e-10 * np.abs(Bo).max()
+    Bl, _Hl, badl = models.oracle_field(P, P["M"], np.array([[0, y, 0] for y in ys]))
+    ok = ~badl
+    assert np.abs(bx[ok] - Bl[ok, 0]).max() <= 1e-10 * np.abs(Bl[:, 0]).max()
