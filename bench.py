@@ -34,16 +34,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Q-blocks/s, rad.Solve s, Fld points/s: mixed vs FP64 GPU at stated error"
-# Declared algorithmic FLOPs, counting + - * / sqrt as 1, log as 20 and atan2
-# as 30, from the integrals as written:
-# - per (point, tetrahedron) of the per-element form (rh_math.hpp): 4 sqrt,
-#   6 log, 6 div, 4 atan2;
-# - mesh form (rh_mesh.hpp): per (point, unique edge) 2 sqrt, 1 div, 1 log;
-#   per (point, unique triangle) 3 sqrt, 1 atan2; per (point, quadrilateral)
-#   4 sqrt, 2 atan2.
-FLOP_TET_FIELD = 821
-FLOP_TET_BLOCK = 794
-FLOP_MESH_EDGE = 74
+# Algorithmic FLOPs per evaluation of the FP64 integrals, counted from the
+# kernel math instantiated with an op-counting scalar (tests/test_opcount.py
+# recounts them): + - * / sqrt count 1, a fused multiply-add 2, a log 20 and
+# an atan2 30 (fixed weights).  Per (point, tetrahedron) field 480 and block
+# 669 (6 logs, 3 atan2, 4 sqrt); mesh form per (point, unique edge) 56 (1 log,
+# 2 sqrt), per (point, unique triangle) 93 (1 atan2, 3 sqrt).
+FLOP_TET_FIELD = 480
+FLOP_TET_BLOCK = 669
+FLOP_MESH_EDGE = 56
 FLOP_MESH_TRI = 93
 PEAK_FP64_VALU = 78.6   # TFLOP/s, MI355X vector FP64
 PEAK_FP32_VALU = 157.3  # TFLOP/s, MI355X vector FP32

@@ -9,6 +9,7 @@
 #include "rh_eval.hpp"
 #include "rh_geom.hpp"
 #include "rh_mesh.hpp"
+#include "rh_opcount.hpp"
 
 namespace {
 thread_local std::string g_err;
@@ -173,6 +174,122 @@ int rhc_adj_grad(const rh_model_desc* d, const double* M, const double* pts, int
         }
         for (int a = 0; a < 3; a++) gX[3 * p + a] = gx[a];
     }
+    return RH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Charged mesh records (the device's k_mesh_charge_* output) for M.
+void charge_mesh(const rh::MeshForm& F, const double* M, std::vector<double>& ce, std::vector<double>& cf,
+                 std::vector<double>& ci)
+{
+    const int64_t ne = F.n_edges(), nf = F.n_faces(), ni = F.n_in();
+    ce.assign(size_t(rh::kMeshEdge * ne), 0.0);
+    cf.assign(size_t(rh::kMeshFace * nf), 0.0);
+    ci.assign(size_t(rh::kMeshIn * ni), 0.0);
+    for (int64_t i = 0; i < ne; i++) {
+        double w[3] = {0, 0, 0};
+        for (int64_t k = F.edge_inc_off[i]; k < F.edge_inc_off[i + 1]; k++) {
+            const double* W = F.edge_inc_w.data() + 9 * k;
+            const double* m = M + 3 * F.edge_inc_elem[k];
+            w[0] += W[0] * m[0] + W[1] * m[1] + W[2] * m[2];
+            w[1] += W[3] * m[0] + W[4] * m[1] + W[5] * m[2];
+            w[2] += W[6] * m[0] + W[7] * m[1] + W[8] * m[2];
+        }
+        double* o = ce.data() + rh::kMeshEdge * i;
+        for (int k = 0; k < rh::kMeshEdgeGeo; k++) o[k] = F.edge_geo[rh::kMeshEdgeGeo * i + k];
+        o[10] = w[0];
+        o[11] = w[1];
+        o[12] = w[2];
+    }
+    for (int64_t i = 0; i < nf; i++) {
+        const double* g = F.face_geo.data() + rh::kMeshFaceGeo * i;
+        double s = 0;
+        for (int64_t k = F.face_inc_off[i]; k < F.face_inc_off[i + 1]; k++) {
+            const double* m = M + 3 * F.face_inc_elem[k];
+            const double* v = F.face_inc_v.data() + 3 * k;
+            s += v[0] * m[0] + v[1] * m[1] + v[2] * m[2];
+        }
+        double* o = cf.data() + rh::kMeshFace * i;
+        for (int k = 0; k < rh::kMeshFaceGeo; k++) o[k] = g[k];
+        o[18] = s * g[12];
+        o[19] = s * g[13];
+        o[20] = s * g[14];
+    }
+    for (int64_t i = 0; i < ni; i++) {
+        const double* g = F.in_geo.data() + (1 + 4 * rh::kMeshPlanes) * i;
+        double* o = ci.data() + rh::kMeshIn * i;
+        o[0] = g[0];
+        const double* Fc = F.in_F.data() + 9 * i;
+        const double* m = M + 3 * F.in_elem[i];
+        for (int k = 0; k < 3; k++) o[1 + k] = Fc[3 * k] * m[0] + Fc[3 * k + 1] * m[1] + Fc[3 * k + 2] * m[2];
+        for (int k = 0; k < 4 * rh::kMeshPlanes; k++) o[4 + k] = g[1 + k];
+    }
+}
+
+void put_counts(const opc::Counts& c, double evals, double* o)
+{
+    const double e = evals > 0 ? evals : 1.0;
+    o[0] = c.add / e;
+    o[1] = c.mul / e;
+    o[2] = c.div / e;
+    o[3] = c.sqrt / e;
+    o[4] = c.fma / e;
+    o[5] = c.log / e;
+    o[6] = c.atan2 / e;
+}
+}  // namespace
+
+extern "C" {
+
+// Operation counts of the FP64 integrals as written, from the kernel math
+// instantiated with an op-counting scalar (rh_opcount.hpp), averaged over
+// the np points and the model's entities.  out (28 doubles), four groups of
+// [add/sub, mul, div, sqrt, fma, log, atan2] per evaluation:
+//   0: (point, unique mesh edge)   mesh_edge_L
+//   1: (point, unique mesh face)   mesh_face_omega (triangles and quadrilaterals)
+//   2: (point, tetrahedron) block  eval_tet (assembly)
+//   3: (point, tetrahedron) field  field_tet (per-element field sweep)
+// Operations the kernels perform in FP64 outside the templated integrals
+// (vertex offsets, accumulation of the weighted terms) are not included.
+// Returns 2 when the model has no mesh form.
+int rhc_opcount(const rh_model_desc* d, const double* M, const double* pts, int64_t np, double* out)
+{
+    rh::PackedModel P;
+    rh_status st = rh::pack_model(d, &P, &g_err);
+    if (st != RH_OK) return st;
+    const rh::MeshForm& F = P.mesh;
+    if (!F.active) return 2;
+    std::vector<double> ce, cf, ci;
+    charge_mesh(F, M, ce, cf, ci);
+    const int64_t ne = F.n_edges(), nf = F.n_faces();
+    opc::Counts c[4];
+    double evals[4] = {0, 0, 0, 0};
+    for (int64_t p = 0; p < np; p++) {
+        const double* x = pts + 3 * p;
+        opc::counter() = &c[0];
+        for (int64_t i = 0; i < ne; i++) (void)rh::mesh_edge_L<opc::OpC>(ce.data() + rh::kMeshEdge * i, x);
+        evals[0] += double(ne);
+        opc::counter() = &c[1];
+        for (int64_t i = 0; i < nf; i++) (void)rh::mesh_face_omega<opc::OpC>(cf.data() + rh::kMeshFace * i, x);
+        evals[1] += double(nf);
+        for (int64_t j = 0; j < P.n; j++) {
+            if (P.kind[j] != rh::EK_TET) continue;
+            const double* rec = P.data.data() + P.off[j];
+            double q6[6], H[3], ch[rh::kChargedTet];
+            opc::counter() = &c[2];
+            (void)rh::eval_tet<opc::OpC>(rec, x, q6);
+            evals[2] += 1.0;
+            opc::counter() = nullptr;
+            rh::charge_tet(rec, M + 3 * j, ch);
+            opc::counter() = &c[3];
+            (void)rh::field_tet<opc::OpC>(ch, x, H);
+            evals[3] += 1.0;
+        }
+    }
+    opc::counter() = nullptr;
+    for (int g = 0; g < 4; g++) put_counts(c[g], evals[g], out + 7 * g);
     return RH_OK;
 }
 
