@@ -227,6 +227,22 @@ rh_status rh_field(rh_model *m, const double *M, const double *pts, int64_t np, 
 rh_status rh_field_dev(rh_model *m, const double *d_M, const double *d_pts, int64_t np,
                        double *d_B, double *d_H, rh_precision prec);
 
+/* No counterpart in the reference.  rh_field_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): same arguments, results and
+ * synchronisation, and no hipMalloc/hipFree or other allocation.  Its partial
+ * sums use a fixed device array of RH_NOALLOC_SCRATCH_BYTES per GPU, reserved
+ * with the library's code on every GPU it runs on until the process ends;
+ * the element splits are reduced to fit it.  When even one split per element
+ * kind does not fit (6 doubles per point per element kind: more than 699,050
+ * points on a model holding two kinds, 466,033 on one holding three) the call
+ * returns RH_ERR_ARG before any launch, naming the largest point count that
+ * fits.  Calls on one GPU share that array, and the model's charged-record
+ * buffers: calls that overlap on one GPU race. */
+#define RH_NOALLOC_SCRATCH_BYTES (64u << 20)
+rh_status rh_field_dev_noalloc(rh_model *m, const double *d_M, const double *d_pts, int64_t np,
+                               double *d_B, double *d_H, rh_precision prec);
+
 /* Current sources.  Record kinds (12 doubles each):
  *   0 block current: centre[3], dims[3], J[3]          (radrec.cpp:49-315, J branch)
  *   1 arc current:   centre[3], rmin, rmax, phimin, phimax, height, j, nseg

@@ -84,6 +84,7 @@ def _load():
                                    ctypes.POINTER(_abi.RelaxOpts), ctypes.POINTER(_abi.RelaxResult)]
     L.rh_field.argtypes = [_vp, _d, _d, ctypes.c_int64, _d, _d, ctypes.c_int]
     L.rh_field_dev.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, _vp, ctypes.c_int]
+    L.rh_field_dev_noalloc.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, _vp, ctypes.c_int]
     L.rh_current_field.argtypes = [ctypes.c_int, _vp, ctypes.c_int64, _i32, _d, _d, _d, _d, _d,
                                    ctypes.c_int64, _d]
     L.rh_current_field_dev.argtypes = [ctypes.c_int, _vp, ctypes.c_int64, _i32, _d, _d, _d, _d,
@@ -108,7 +109,7 @@ def _load():
     for name in ["rh_model_create", "rh_model_destroy", "rh_model_colloc", "rh_model_counts",
                  "rh_matrix_assemble", "rh_matrix_destroy", "rh_matrix_export",
                  "rh_matrix_device_ptr", "rh_matrix_reassemble", "rh_relax", "rh_relax_dev",
-                 "rh_field", "rh_field_dev", "rh_current_field", "rh_current_field_dev",
+                 "rh_field", "rh_field_dev", "rh_field_dev_noalloc", "rh_current_field", "rh_current_field_dev",
                  "rh_set_timing", "rh_kernel_time", "rh_model_vertex_count",
                  "rh_field_adjoint", "rh_field_adjoint_dev", "rh_relax_adjoint_dev",
                  "rh_relax_adjoint", "rh_model_set_field_form", "rh_model_field_form",

@@ -24,7 +24,7 @@ def _run(cmd):
 
 def build(verbose=True):
     objs = []
-    for src in ["rh_kernels.hip", "rh_adjoint.hip", "rh_api.cpp", "rh_geom.cpp"]:
+    for src in ["rh_kernels.hip", "rh_adjoint.hip", "rh_noalloc.hip", "rh_api.cpp", "rh_geom.cpp"]:
         obj = os.path.join(CSRC, "_build", src + ".o")
         os.makedirs(os.path.dirname(obj), exist_ok=True)
         lang = ["-x", "hip"] if src.endswith(".hip") else []

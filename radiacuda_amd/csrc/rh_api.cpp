@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/radhip.h"
+#include "rh_api_internal.hpp"
 #include "rh_geom.hpp"
 #include "rh_kernels.hpp"
 #include "rh_math.hpp"
@@ -1402,6 +1403,42 @@ rh_status rh_field_dev(rh_model* m, const double* d_M, const double* d_pts, int6
     g_backend_fld = 1;
     return RH_OK;
 }
+
+}  // extern "C"
+
+namespace rh_internal {
+
+rh_status fail(rh_status st, const std::string& msg) { return ::fail(st, msg); }
+
+int model_device(const rh_model* m) { return m->device; }
+
+hipError_t enter_device(int device) { return ::enter_device(device); }
+
+rh_status field_dev_into(rh_model* m, const double* d_M, const double* d_pts, int64_t np, double* d_B,
+                         double* d_H, rh_precision prec, double* work, int64_t work_cap)
+{
+    if (!m || (!d_B && !d_H) || np < 0) return fail(RH_ERR_ARG, "bad argument");
+    if (np > 0 && (!d_M || !d_pts)) return fail(RH_ERR_ARG, "null argument");
+    if (prec != RH_FP64 && prec != RH_MIXED) return fail(RH_ERR_ARG, "unknown precision");
+    if (!work || work_cap < 0) return fail(RH_ERR_ARG, "no work buffer");
+    const int64_t slices = rh::field_slices(m->dev, np, work_cap);
+    if (slices > 1 && 6 * np * slices > work_cap)
+        return fail(RH_ERR_ARG, "rh_field_dev_noalloc: " + std::to_string(np) + " points need " +
+                                    std::to_string(6 * np * slices * 8) + " bytes of partial sums for this model, "
+                                    "beyond the fixed scratch of " + std::to_string(work_cap * 8) +
+                                    " bytes; split the points into calls of at most " +
+                                    std::to_string(work_cap / (6 * slices)) + " points");
+    RH_HIP(::enter_device(m->device));
+    RH_HIP(rh::launch_field(m->dev, d_M, d_pts, np, d_B, d_H, work, m->d_charged, prec == RH_MIXED, m->stream,
+                            work_cap));
+    RH_HIP(hipStreamSynchronize(m->stream));
+    g_backend_fld = 1;
+    return RH_OK;
+}
+
+}  // namespace rh_internal
+
+extern "C" {
 
 rh_status rh_field(rh_model* m, const double* M, const double* pts, int64_t np, double* B_out,
                    double* H_out, rh_precision prec)

@@ -1364,7 +1364,7 @@ bool mesh_on(const DevModel& m) { return m.use_mesh && m.mesh.active; }
 
 // slices per kind slot (0 blocks, 1 tetrahedra, 2 general polyhedra); in the
 // mesh form slot 1 carries the whole mesh and slot 2 is empty
-void slot_plan(const DevModel& m, int64_t np, int64_t sp[3])
+void slot_plan(const DevModel& m, int64_t np, int64_t sp[3], int64_t work_cap)
 {
     sp[0] = slot_splits(np, m.koff[1] - m.koff[0]);
     if (mesh_on(m)) {
@@ -1374,14 +1374,28 @@ void slot_plan(const DevModel& m, int64_t np, int64_t sp[3])
         sp[1] = slot_splits(np, m.koff[2] - m.koff[1]);
         sp[2] = slot_splits(np, m.koff[3] - m.koff[2]);
     }
+    if (work_cap < 0) return;
+    // halve the largest split count until the partials fit or every slot is
+    // down to one split
+    for (;;) {
+        const int64_t total = sp[0] + sp[1] + sp[2];
+        if (total <= 1 || 6 * np * total <= work_cap) return;
+        int big = 0;
+        for (int k = 1; k < 3; k++)
+            if (sp[k] > sp[big]) big = k;
+        if (sp[big] <= 1) return;
+        sp[big] = (sp[big] + 1) / 2;
+    }
 }
 
 template <class R, bool SYM>
 hipError_t field_all(const DevModel& m, const double* M, const double* pts, int64_t np, double* B,
-                     double* H, double* work, hipStream_t s)
+                     double* H, double* work, int64_t work_cap, hipStream_t s)
 {
     int64_t sp[3];
-    slot_plan(m, np, sp);
+    slot_plan(m, np, sp, work_cap);
+    if (work_cap >= 0 && sp[0] + sp[1] + sp[2] > 1 && 6 * np * (sp[0] + sp[1] + sp[2]) > work_cap)
+        return hipErrorInvalidValue;
     const int64_t total = sp[0] + sp[1] + sp[2];
     double* part = (total > 1) ? work : nullptr;
     int64_t slice0 = 0;
@@ -1404,16 +1418,16 @@ hipError_t field_all(const DevModel& m, const double* M, const double* pts, int6
 }
 }  // namespace
 
-int64_t field_slices(const DevModel& m, int64_t np)
+int64_t field_slices(const DevModel& m, int64_t np, int64_t work_cap)
 {
     int64_t sp[3];
-    slot_plan(m, np, sp);
+    slot_plan(m, np, sp, work_cap);
     return std::max<int64_t>(1, sp[0] + sp[1] + sp[2]);
 }
 
 hipError_t launch_field(const DevModel& m0, const double* M, const double* pts, int64_t np,
                         double* B, double* H, double* work, double* charged, bool fp32,
-                        hipStream_t s)
+                        hipStream_t s, int64_t work_cap)
 {
     if (np == 0) return hipSuccess;
     if (m0.n == 0) {
@@ -1439,10 +1453,10 @@ hipError_t launch_field(const DevModel& m0, const double* M, const double* pts, 
                            charged);
         m.charged = charged;
     }
-    if (fp32) return m.sym ? field_all<float, true>(m, M, pts, np, B, H, work, s)
-                           : field_all<float, false>(m, M, pts, np, B, H, work, s);
-    return m.sym ? field_all<double, true>(m, M, pts, np, B, H, work, s)
-                 : field_all<double, false>(m, M, pts, np, B, H, work, s);
+    if (fp32) return m.sym ? field_all<float, true>(m, M, pts, np, B, H, work, work_cap, s)
+                           : field_all<float, false>(m, M, pts, np, B, H, work, work_cap, s);
+    return m.sym ? field_all<double, true>(m, M, pts, np, B, H, work, work_cap, s)
+                 : field_all<double, false>(m, M, pts, np, B, H, work, work_cap, s);
 }
 
 hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t ld, const double* x,

@@ -60,13 +60,18 @@ hipError_t launch_assemble(const DevModel& m, void* Q, int elem_bytes, int64_t l
                            int64_t r0, int64_t r1, hipStream_t s);
 
 // Number of partial-sum slices the field kernels use for np points (>= 1);
-// the work buffer needs 6 * np * slices doubles when slices > 1.
-int64_t field_slices(const DevModel& m, int64_t np);
+// the work buffer needs 6 * np * slices doubles when slices > 1.  work_cap >= 0
+// bounds that buffer (in doubles): the element splits are reduced until the
+// partials fit, and the result exceeds the cap only when every kind slot is
+// already down to one split (the caller must then refuse the call).  A negative
+// work_cap leaves the splits as chosen for occupancy.
+int64_t field_slices(const DevModel& m, int64_t np, int64_t work_cap = -1);
 // B, H (either may be null) at np points.  charged: kChargedTet doubles per
-// tetrahedron, rebuilt from M before the sweep.
+// tetrahedron, rebuilt from M before the sweep.  work: 6 * np * slices doubles
+// (field_slices with the same work_cap) when slices > 1.
 hipError_t launch_field(const DevModel& m, const double* M, const double* pts, int64_t np,
                         double* B, double* H, double* work, double* charged, bool fp32,
-                        hipStream_t s);
+                        hipStream_t s, int64_t work_cap = -1);
 
 // y = Q x (+ add), Q: nrows x ld (ld multiple of 256, padded columns zero,
 // x padded with zeros to ld).
