@@ -35,7 +35,9 @@ typedef enum {
 
 typedef enum {
     RH_FP64 = 0,  /* FP64 arithmetic, FP64 matrix storage                          */
-    RH_MIXED = 1  /* FP64 arithmetic, FP32 matrix storage (Radia's own storage)    */
+    RH_MIXED = 1  /* FP32 matrix storage (Radia's own storage) with FP64 self
+                   * blocks; far blocks integrated in FP32, near ones in FP64;
+                   * field integrals in FP32 with FP64 accumulation            */
 } rh_precision;
 
 /* ------------------------------------------------------------------------ */
@@ -107,10 +109,12 @@ typedef struct rh_matrix rh_matrix;
 rh_status rh_matrix_assemble(rh_model *m, rh_precision prec, rh_matrix **out);
 
 /* Placement of the matrix rows.  Rows that do not fit in device memory can
- * live in pinned host memory and be streamed through two 64 MiB device tiles
- * in every matrix pass, the upload of one tile overlapping the pass over the
- * other; results do not depend on the split (each row is reduced the same
- * way wherever it lives).  Replaces the reference's streaming matrix context
+ * live in pinned host memory and be streamed through two device tiles (64 MiB,
+ * or the rows of 64 elements when larger) in every matrix pass, the upload of
+ * one tile overlapping the pass over the other.  The resident part and every
+ * tile hold whole groups of 64 elements (the resident count is rounded down
+ * to one), so results do not depend on the split: each block is integrated
+ * and each row reduced the same way wherever it lives.  Replaces the reference's streaming matrix context
  * and its UtiGpuFallback 'gpu_streaming' policy (cpp/src/core/radgpurlx.cu:258-459,
  * cpp/src/core/radintrc.cpp:796-805). */
 typedef struct {
@@ -122,7 +126,10 @@ typedef struct {
                                * budget; 1: stream the remaining rows;
                                * RH_STREAM_BY_POLICY (-1): as rh_get_fallback()
                                * says ('break' refuses, the others stream)       */
-    int32_t reserved;         /* 0                                                 */
+    int32_t fp64_integrals;   /* RH_MIXED only.  0: far blocks (beyond two
+                               * circumradii of a block or tetrahedron) are
+                               * integrated in FP32, near and self blocks in FP64;
+                               * 1: every block integrated in FP64               */
 } rh_matrix_opts;
 #define RH_STREAM_BY_POLICY (-1)
 rh_status rh_matrix_assemble_ex(rh_model *m, rh_precision prec, const rh_matrix_opts *opts,

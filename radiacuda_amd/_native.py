@@ -241,15 +241,17 @@ class Matrix:
     """Device-resident interaction matrix (rh_matrix)."""
 
     def __init__(self, model, prec=RH_MIXED, device_bytes=0, resident_fraction=-1.0,
-                 allow_streaming=False):
-        """device_bytes / resident_fraction / allow_streaming: rh_matrix_opts
-        (rows beyond the device budget are streamed from pinned host memory
-        when allowed; allow_streaming="policy" follows rh_get_fallback)."""
+                 allow_streaming=False, fp64_integrals=False):
+        """device_bytes / resident_fraction / allow_streaming / fp64_integrals:
+        rh_matrix_opts (rows beyond the device budget are streamed from pinned
+        host memory when allowed; allow_streaming="policy" follows
+        rh_get_fallback; fp64_integrals integrates every block of an RH_MIXED
+        matrix in FP64 instead of the far ones in FP32)."""
         self.model = model
         self.n = model.n
         self.h = _vp()
         stream = _abi.RH_STREAM_BY_POLICY if allow_streaming == "policy" else (1 if allow_streaming else 0)
-        o = _abi.MatrixOpts(int(device_bytes), float(resident_fraction), stream, 0)
+        o = _abi.MatrixOpts(int(device_bytes), float(resident_fraction), stream, 1 if fp64_integrals else 0)
         check(lib().rh_matrix_assemble_ex(model.h, int(prec), ctypes.byref(o), ctypes.byref(self.h)))
 
     def layout(self):

@@ -89,6 +89,17 @@ struct Timer {
         if (!g_timing || ev.empty()) return;
         (void)hipEventRecord(ev.back().b, s);
     }
+    // Keeps the first k recorded launches only (launches that returned at
+    // once, after a device-driven loop had finished, are not statistics).
+    void keep(size_t k)
+    {
+        while (ev.size() > k) {
+            (void)hipEventSynchronize(ev.back().b);
+            (void)hipEventDestroy(ev.back().a);
+            (void)hipEventDestroy(ev.back().b);
+            ev.pop_back();
+        }
+    }
     ~Timer()
     {
         if (ev.empty()) return;
@@ -128,6 +139,83 @@ void dfree(void* p)
     if (p) (void)hipFree(p);
 }
 
+// Anderson-accelerated (type II, depth kHist) under-relaxed block-Jacobi
+// iteration on M = G(M), G = per-element implicit material update given the
+// field of all other elements (the fixed point of radGPU_RelaxAuto,
+// cpp/src/core/radgpurlx.cu:799-1349, with an acceleration of our own).
+constexpr int kHist = 8;
+static_assert(kHist == rh::kHistMax, "the relaxation step holds kHistMax differences");
+
+// Work arrays of the relaxation: kept with the matrix and reused by every
+// solve on it (allocated by the first one).
+struct RelaxWork {
+    int64_t n3 = 0, ld = 0, nb = 0, db = 0;
+    double* x = nullptr;   // [ld] current iterate (padded)
+    double* y = nullptr;   // [n3]
+    double* F = nullptr;   // [n3]
+    double* Fp = nullptr;  // [n3] previous F
+    double* xp = nullptr;  // [n3] previous x
+    double* dX[kHist] = {nullptr};
+    double* dF[kHist] = {nullptr};
+    double* part = nullptr;
+    double* dots = nullptr;
+    rh::RelaxDevState* st = nullptr;   // state of the device-driven loop
+    rh::RelaxDevState* hst = nullptr;  // pinned: two read-back slots and an upload slot
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double* hbuf = nullptr;            // pinned read-back of the host-driven loops
+    double* H64 = nullptr;             // [n3] FP64 refinement: field at the collocation points
+    double* hx = nullptr;              // [n3] FP64 refinement: corrected external field
+    ~RelaxWork()
+    {
+        dfree(x);
+        dfree(y);
+        dfree(F);
+        dfree(Fp);
+        dfree(xp);
+        for (int k = 0; k < kHist; k++) {
+            dfree(dX[k]);
+            dfree(dF[k]);
+        }
+        dfree(part);
+        dfree(dots);
+        dfree(st);
+        dfree(H64);
+        dfree(hx);
+        if (hst) (void)hipHostFree(hst);
+        if (hbuf) (void)hipHostFree(hbuf);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    // All arrays of a solve on n elements (3n rows, leading dimension ld).
+    hipError_t alloc(int64_t n, int64_t n3_, int64_t ld_)
+    {
+        n3 = n3_;
+        ld = ld_;
+        nb = rh::relax_blocks(n);
+        db = rh::dot_blocks(n3);
+        hipError_t e = dalloc(&x, size_t(ld));
+        if (e == hipSuccess) e = dalloc(&y, size_t(n3));
+        if (e == hipSuccess) e = dalloc(&F, size_t(n3));
+        if (e == hipSuccess) e = dalloc(&Fp, size_t(n3));
+        if (e == hipSuccess) e = dalloc(&xp, size_t(n3));
+        for (int k = 0; k < kHist && e == hipSuccess; k++) {
+            e = dalloc(&dX[k], size_t(n3));
+            if (e == hipSuccess) e = dalloc(&dF[k], size_t(n3));
+        }
+        if (e == hipSuccess) e = dalloc(&part, size_t(rh::kRelaxPart * nb));
+        if (e == hipSuccess) e = dalloc(&dots, size_t(17 * db));
+        if (e == hipSuccess) e = dalloc(&st, 1);
+        if (e == hipSuccess) e = dalloc(&H64, size_t(n3));
+        if (e == hipSuccess) e = dalloc(&hx, size_t(n3));
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&hst), 3 * sizeof(rh::RelaxDevState));
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&hbuf), size_t(rh::kRelaxPart * nb + 17 * db) * sizeof(double));
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+        return e;
+    }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------- handles
@@ -162,6 +250,7 @@ struct rh_matrix {
     rh_model* m = nullptr;  // must outlive the matrix
     int device = 0;
     rh_precision prec = RH_MIXED;
+    bool far32 = false;  // far blocks integrated in FP32 (RH_MIXED)
     int elem_bytes = 4;
     int64_t n3 = 0, ld = 0;
     void* d_Q = nullptr;  // rows of elements [0, res_elems)
@@ -175,6 +264,7 @@ struct rh_matrix {
     hipStream_t copy = nullptr;
     hipEvent_t ev_ready[2] = {nullptr, nullptr};  // tile b holds data for the consumer
     hipEvent_t ev_free[2] = {nullptr, nullptr};   // tile b may be overwritten
+    RelaxWork* rw = nullptr;  // relaxation work, kept between solves
     size_t row_bytes() const { return size_t(ld) * size_t(elem_bytes); }
     bool streamed() const { return res_elems < n3 / 3; }
 };
@@ -420,7 +510,7 @@ static rh_status assemble_into(rh_matrix* q)
     const int64_t n = m->host.n;
     Timer t("asm", s);
     t.begin();
-    RH_HIP(rh::launch_assemble(m->dev, q->d_Q, q->elem_bytes, q->ld, q->d_qdiag, 0, q->res_elems, s));
+    RH_HIP(rh::launch_assemble(m->dev, q->d_Q, q->elem_bytes, q->ld, q->d_qdiag, 0, q->res_elems, s, q->far32));
     // streamed rows: assemble a tile on the device, copy it to pinned host
     // memory on the copy stream while the next tile is assembled
     int64_t tile = 0;
@@ -428,7 +518,7 @@ static rh_status assemble_into(rh_matrix* q)
         const int b = int(tile & 1);
         const int64_t e1 = std::min(n, e0 + q->tile_elems);
         RH_HIP(hipStreamWaitEvent(s, q->ev_free[b], 0));
-        RH_HIP(rh::launch_assemble(m->dev, q->stage[b], q->elem_bytes, q->ld, q->d_qdiag, e0, e1, s));
+        RH_HIP(rh::launch_assemble(m->dev, q->stage[b], q->elem_bytes, q->ld, q->d_qdiag, e0, e1, s, q->far32));
         RH_HIP(hipEventRecord(q->ev_ready[b], s));
         RH_HIP(hipStreamWaitEvent(q->copy, q->ev_ready[b], 0));
         char* dst = static_cast<char*>(q->h_Q) + size_t(3 * (e0 - q->res_elems)) * q->row_bytes();
@@ -481,6 +571,7 @@ static void free_matrix(rh_matrix* q)
         if (q->ev_free[b]) (void)hipEventDestroy(q->ev_free[b]);
     }
     if (q->copy) (void)hipStreamDestroy(q->copy);
+    delete q->rw;
     delete q;
 }
 
@@ -497,12 +588,14 @@ rh_status rh_matrix_assemble_ex(rh_model* m, rh_precision prec, const rh_matrix_
     if (o.allow_streaming != 0 && o.allow_streaming != 1 && o.allow_streaming != RH_STREAM_BY_POLICY)
         return fail(RH_ERR_ARG, "allow_streaming must be 0, 1 or RH_STREAM_BY_POLICY");
     if (o.allow_streaming == RH_STREAM_BY_POLICY) o.allow_streaming = (g_fallback == RH_FALLBACK_BREAK) ? 0 : 1;
+    if (o.fp64_integrals != 0 && o.fp64_integrals != 1) return fail(RH_ERR_ARG, "fp64_integrals must be 0 or 1");
     RH_HIP(enter_device(m->device));
     const int64_t n = m->host.n;
     rh_matrix* q = new rh_matrix();
     q->m = m;
     q->device = m->device;
     q->prec = prec;
+    q->far32 = (prec == RH_MIXED) && o.fp64_integrals == 0;
     q->elem_bytes = (prec == RH_FP64) ? 8 : 4;
     q->n3 = 3 * n;
     q->ld = std::max<int64_t>(256, (q->n3 + 255) / 256 * 256);
@@ -531,6 +624,11 @@ rh_status rh_matrix_assemble_ex(rh_model* m, rh_precision prec, const rh_matrix_
         delete q;
         return fail(RH_ERR_NOMEM, "matrix does not fit and streaming is not allowed");
     }
+    // the resident rows and every streamed tile start at a multiple of the
+    // assembly's 64-row groups, so each block is integrated with the same
+    // wave-uniform choices (FP32 or FP64 integrals) wherever its row lives
+    constexpr int64_t kRowGroup = 64;
+    if (res < n) res = res / kRowGroup * kRowGroup;
     q->res_elems = res;
     const size_t dev_bytes = size_t(res) * elem_rows;
     hipError_t e = hipMalloc(&q->d_Q, std::max<size_t>(dev_bytes, 16));
@@ -538,7 +636,8 @@ rh_status rh_matrix_assemble_ex(rh_model* m, rh_precision prec, const rh_matrix_
     // padded columns must read as zero in the matrix pass
     if (e == hipSuccess && dev_bytes) e = hipMemsetAsync(q->d_Q, 0, dev_bytes, m->stream);
     if (e == hipSuccess && res < n) {
-        q->tile_elems = std::max<int64_t>(1, int64_t((size_t(1) << 26) / elem_rows));
+        q->tile_elems = int64_t((size_t(1) << 26) / elem_rows) / kRowGroup * kRowGroup;
+        q->tile_elems = std::max<int64_t>(kRowGroup, q->tile_elems);
         q->tile_elems = std::min(q->tile_elems, n - res);
         const size_t tb = size_t(q->tile_elems) * elem_rows;
         for (int b = 0; b < 2 && e == hipSuccess; b++) {
@@ -644,39 +743,6 @@ rh_status rh_matrix_export(const rh_matrix* q, int64_t row0, int64_t nrows, doub
 // ---------------------------------------------------------------- relaxation
 namespace {
 
-// Anderson-accelerated (type II, depth kHist) under-relaxed block-Jacobi
-// iteration on M = G(M), G = per-element implicit material update given the
-// field of all other elements (the fixed point of radGPU_RelaxAuto,
-// cpp/src/core/radgpurlx.cu:799-1349, with an acceleration of our own).
-constexpr int kHist = 8;
-static_assert(kHist <= rh::kHistMax, "the fused relaxation step holds at most kHistMax differences");
-
-struct RelaxWork {
-    double* x = nullptr;   // [ld] current iterate (padded)
-    double* y = nullptr;   // [n3]
-    double* F = nullptr;   // [n3]
-    double* Fp = nullptr;  // [n3] previous F
-    double* xp = nullptr;  // [n3] previous x
-    double* dX[kHist] = {nullptr};
-    double* dF[kHist] = {nullptr};
-    double* part = nullptr;
-    double* dots = nullptr;
-    ~RelaxWork()
-    {
-        dfree(x);
-        dfree(y);
-        dfree(F);
-        dfree(Fp);
-        dfree(xp);
-        for (int k = 0; k < kHist; k++) {
-            dfree(dX[k]);
-            dfree(dF[k]);
-        }
-        dfree(part);
-        dfree(dots);
-    }
-};
-
 // Solves (G + lambda I) g = r for a small symmetric positive semi-definite G.
 bool small_solve(int m, std::vector<double> G, std::vector<double> r, std::vector<double>& g)
 {
@@ -729,11 +795,107 @@ struct AdjointPass {
     int64_t chunks = 0;
 };
 
+// Device-driven form of the forward iteration below: the matrix pass, the
+// relaxation step, the Anderson step and the update all read the loop state
+// on the device.  For a resident matrix the host enqueues iterations in
+// batches without waiting for any of them, copies the state back after each
+// batch and looks at it one batch late; iterations enqueued after the loop
+// finished return at once.  The batch after the first few is sized from the
+// residual's decay.  A streamed matrix (whose passes upload tiles from host
+// memory) runs one iteration per batch and is looked at after each, so no
+// pass is enqueued beyond the last; its iterates are the resident matrix's,
+// bit for bit.
+rh_status relax_run_dev(rh_matrix* q, const double* d_mats, const double* d_tab, const double* d_hext,
+                        double* d_M, double* d_H, const rh_relax_opts* opts, RelaxWork& w, rh_relax_result* res,
+                        bool keep_if_converged)
+{
+    const bool lagged = !q->streamed();
+    rh_model* m = q->m;
+    hipStream_t s = m->stream;
+    const int64_t n = m->host.n, n3 = q->n3;
+    const int max_iter = std::max(1, int(opts->max_iter));
+    rh::RelaxDevState& h = w.hst[2];
+    std::memset(&h, 0, sizeof(h));
+    h.max_iter = max_iter;
+    h.keep_if_converged = keep_if_converged ? 1 : 0;
+    h.prec = opts->prec;
+    h.beta = (opts->omega0 > 0.0) ? opts->omega0 : 0.5;
+    h.best = INFINITY;
+    h.n = double(n);
+    RH_HIP(hipMemcpyAsync(w.st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    rh::HistBufs hb{};
+    hb.Fp = w.Fp;
+    hb.xp = w.xp;
+    for (int k = 0; k < kHist; k++) {
+        hb.dF[k] = w.dF[k];
+        hb.dX[k] = w.dX[k];
+    }
+    Timer tm("matvec", s);
+    int enq = 0, batch = 0, cur = 0, want = lagged ? 4 : 1;
+    int it_prev = 0;
+    double mis_prev = 0.0;
+    for (;;) {
+        const int cnt = std::min(want, max_iter - enq);
+        for (int j = 0; j < cnt; j++) {
+            tm.begin();
+            if (lagged)
+                RH_HIP(rh::launch_matvec(q->d_Q, q->elem_bytes, n3, q->ld, w.x, nullptr, w.y, s, &w.st->done));
+            else
+                RH_HIP(matrix_pass(q, w.x, w.y, s));
+            tm.end();
+            RH_HIP(rh::launch_relax_step_dev(n, d_mats, d_tab, d_hext, q->d_qdiag, w.y, w.x, w.F, d_H, w.part, hb,
+                                             w.st, s));
+            RH_HIP(rh::launch_anderson_dev(w.nb, w.part, w.st, s));
+            RH_HIP(rh::launch_update_dev(n3, w.x, w.F, hb, d_M, w.st, s));
+        }
+        enq += cnt;
+        RH_HIP(hipMemcpyAsync(&w.hst[cur], w.st, sizeof(rh::RelaxDevState), hipMemcpyDeviceToHost, s));
+        RH_HIP(hipEventRecord(w.ev[cur], s));
+        bool stop = enq >= max_iter;
+        if (!lagged) {
+            RH_HIP(hipEventSynchronize(w.ev[cur]));
+            if (w.hst[cur].done) stop = true;
+        } else if (batch > 0) {
+            RH_HIP(hipEventSynchronize(w.ev[cur ^ 1]));
+            const rh::RelaxDevState& p = w.hst[cur ^ 1];
+            if (p.done) {
+                stop = true;
+            } else if (p.it > it_prev && mis_prev > 0.0 && p.misfit < mis_prev && p.misfit > p.prec) {
+                // iterations left at the observed rate, less those in flight
+                const double rate = std::log(p.misfit / mis_prev) / double(p.it - it_prev);
+                const double left = std::log(p.prec / p.misfit) / rate;
+                const int inflight = enq - p.it;
+                want = int(std::min(32.0, std::max(1.0, std::ceil(left) - inflight)));
+            } else {
+                want = 4;
+            }
+            it_prev = p.it;
+            mis_prev = p.misfit;
+        }
+        if (stop) break;
+        batch++;
+        cur ^= 1;
+    }
+    RH_HIP(hipStreamSynchronize(s));
+    const rh::RelaxDevState& f = w.hst[cur];
+    tm.keep(size_t(std::max(0, f.it)));
+    if (f.fail) return fail(RH_ERR_NOTCONV, "relaxation produced a non-finite residual");
+    res->misfit = f.misfit;
+    res->max_m = f.maxM;
+    res->max_h = f.maxH;
+    res->iters += f.it;
+    res->matvecs += f.it;
+    res->converged = f.misfit <= opts->prec ? 1 : 0;
+    return RH_OK;
+}
+
 rh_status relax_run(rh_matrix* q, const double* d_mats, const double* d_tab, const double* d_hext,
                     double* d_M, double* d_H, const rh_relax_opts* opts, RelaxWork& w, double* hbuf,
                     int64_t nb, int64_t db, rh_relax_result* res,
                     const AdjointPass* adj = nullptr, bool keep_if_converged = false)
 {
+    if (!adj && !std::getenv("RADHIP_RELAX_HOST_LOOP"))
+        return relax_run_dev(q, d_mats, d_tab, d_hext, d_M, d_H, opts, w, res, keep_if_converged);
     rh_model* m = q->m;
     hipStream_t s = m->stream;
     const int64_t n = m->host.n, n3 = q->n3;
@@ -1236,20 +1398,19 @@ rh_status rh_relax_tab_dev(rh_matrix* q, const double* d_mats, const double* d_t
                 return fail(RH_ERR_ARG, "tabulated law of record " + std::to_string(i) + " outside the table");
         }
     }
-    RelaxWork w;
-    RH_HIP(dalloc(&w.x, size_t(q->ld)));
-    RH_HIP(dalloc(&w.y, size_t(n3)));
-    RH_HIP(dalloc(&w.F, size_t(n3)));
-    RH_HIP(dalloc(&w.Fp, size_t(n3)));
-    RH_HIP(dalloc(&w.xp, size_t(n3)));
-    for (int k = 0; k < kHist; k++) {
-        RH_HIP(dalloc(&w.dX[k], size_t(n3)));
-        RH_HIP(dalloc(&w.dF[k], size_t(n3)));
+    if (!q->rw) {
+        RelaxWork* rw = new RelaxWork();
+        const hipError_t e = rw->alloc(n, n3, q->ld);
+        if (e != hipSuccess) {
+            delete rw;
+            return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_HIP,
+                        std::string("relaxation work: ") + hipGetErrorString(e));
+        }
+        q->rw = rw;
     }
-    const int64_t nb = rh::relax_blocks(n);
-    const int64_t db = rh::dot_blocks(n3);
-    RH_HIP(dalloc(&w.part, size_t(rh::kRelaxPart * nb)));
-    RH_HIP(dalloc(&w.dots, size_t(17 * db)));
+    RelaxWork& w = *q->rw;
+    const int64_t nb = w.nb;
+    const int64_t db = w.db;
     RH_HIP(hipMemsetAsync(w.x, 0, size_t(q->ld) * sizeof(double), s));
     // d_H starts the per-element nonlinear solves (previous local field); a
     // zero start keeps repeated solves bit-identical
@@ -1262,13 +1423,8 @@ rh_status rh_relax_tab_dev(rh_matrix* q, const double* d_mats, const double* d_t
             RH_HIP(hipMemcpy2DAsync(w.x + k, 3 * sizeof(double), d_mats + 13 + k, 16 * sizeof(double),
                                     sizeof(double), size_t(n), hipMemcpyDeviceToDevice, s));
     }
-    // one pinned read-back per iteration: residual partials, then dot partials
-    double* hbuf = nullptr;
-    RH_HIP(hipHostMalloc(reinterpret_cast<void**>(&hbuf), size_t(rh::kRelaxPart * nb + 17 * db) * sizeof(double)));
-    struct PinnedFree {
-        double* p;
-        ~PinnedFree() { (void)hipHostFree(p); }
-    } pinned_guard{hbuf};
+    // pinned read-back of the host-driven loops: residual partials, then dot partials
+    double* hbuf = w.hbuf;
     const bool nk_method = (opts->method == 11);
     NKWork nk;
     if (nk_method) {
@@ -1289,18 +1445,8 @@ rh_status rh_relax_tab_dev(rh_matrix* q, const double* d_mats, const double* d_t
     // corrected system.
     const int passes = (q->prec == RH_MIXED) ? (opts->refine == 0 ? 1 : std::max(0, int(opts->refine))) : 0;
     if (passes == 0) return RH_OK;
-    double *d_H64 = nullptr, *d_hx = nullptr;
-    struct Free2 {
-        double** a;
-        double** b;
-        ~Free2()
-        {
-            dfree(*a);
-            dfree(*b);
-        }
-    } free2{&d_H64, &d_hx};
-    RH_HIP(dalloc(&d_H64, size_t(n3)));
-    RH_HIP(dalloc(&d_hx, size_t(n3)));
+    double* d_H64 = w.H64;
+    double* d_hx = w.hx;
     const rh_relax_opts o2 = *opts;
     for (int pss = 0; pss < passes; pss++) {
         st = field_dev_impl(m, d_M, m->d_colloc, n, nullptr, d_H64, false, "refine");

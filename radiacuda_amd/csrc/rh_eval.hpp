@@ -75,6 +75,77 @@ RH_HD void block_Q(const ModelView& mv, int64_t j, const double x[3], double q9[
     }
 }
 
+// Near test of the mixed-precision assembly: the point (element frame) lies
+// within kNearRadii circumradii of the element's centre (blocks: the half
+// diagonal about the centre; tetrahedra: the farthest vertex from the vertex
+// mean).  General polyhedra always count as near.
+constexpr double kNearRadii = 2.0;
+template <int KIND>
+RH_HD bool near_element(int kind, const double* __restrict__ rec, const double x[3])
+{
+    const int K = (KIND >= 0) ? KIND : (kind == EK_REC ? 0 : (kind == EK_TET ? 1 : 2));
+    if (K == 0) {
+        const double dx = x[0] - rec[0], dy = x[1] - rec[1], dz = x[2] - rec[2];
+        const double r2 = rec[3] * rec[3] + rec[4] * rec[4] + rec[5] * rec[5];
+        return dx * dx + dy * dy + dz * dz < kNearRadii * kNearRadii * r2;
+    }
+    if (K == 1) {
+        double c[3];
+        for (int k = 0; k < 3; k++) c[k] = 0.25 * (rec[k] + rec[3 + k] + rec[6 + k] + rec[9 + k]);
+        double r2 = 0.0;
+        for (int v = 0; v < 4; v++) {
+            const double a = rec[3 * v] - c[0], b = rec[3 * v + 1] - c[1], d = rec[3 * v + 2] - c[2];
+            r2 = fmax(r2, a * a + b * b + d * d);
+        }
+        const double dx = x[0] - c[0], dy = x[1] - c[1], dz = x[2] - c[2];
+        return dx * dx + dy * dy + dz * dz < kNearRadii * kNearRadii * r2;
+    }
+    return true;
+}
+
+// One element's Q at xl (element frame) in mixed precision: FP64 integrals
+// when the point is near (on the device: when any lane of the wave is near,
+// so the wave takes one path), FP32 integrals otherwise.
+template <int KIND>
+RH_HD void eval_mixed(int kind, const double* __restrict__ rec, const double xl[3], double q6[6])
+{
+    bool near = near_element<KIND>(kind, rec, xl);
+#ifdef __HIP_DEVICE_COMPILE__
+    near = __any(near);
+#endif
+    if (near)
+        eval_k<double, KIND>(kind, rec, xl, q6);
+    else
+        eval_k<float, KIND>(kind, rec, xl, q6);
+}
+
+// block_Q with FP32 integrals for the far elements and FP64 for the near
+// ones (self and neighbours), copy by copy.
+template <bool SYM, int KIND>
+RH_HD void block_Q_mixed(const ModelView& mv, int64_t j, const double x[3], double q9[9])
+{
+    const int kind = (KIND < 0) ? mv.kind[j] : 0;
+    const double* rec = mv.data + mv.off[j];
+    if (!SYM) {
+        double q6[6];
+        eval_mixed<KIND>(kind, rec, x, q6);
+        q6_to_q9(q6, q9);
+        return;
+    }
+    for (int k = 0; k < 9; k++) q9[k] = 0.0;
+    for (int64_t c = mv.copy_off[j]; c < mv.copy_off[j + 1]; c++) {
+        const double* cp = mv.copies + 24 * c;
+        double xl[3], q6[6], ql[9];
+        apply_inv(cp, x, xl);
+        eval_mixed<KIND>(kind, rec, xl, q6);
+        q6_to_q9(q6, ql);
+        const double* F = cp + 12;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                q9[3 * a + b] += F[3 * a] * ql[b] + F[3 * a + 1] * ql[3 + b] + F[3 * a + 2] * ql[6 + b];
+    }
+}
+
 // Adds the B and H of the elements idx[k0..k1) (all of kind slot KIND, or of
 // any kind when KIND < 0; idx == nullptr means the identity list) at x.
 template <class R, bool SYM, int KIND = -1>

@@ -25,7 +25,9 @@ extern "C" {
 
 const char* rhc_last_error(void) { return g_err.c_str(); }
 
-// Interaction matrix rows [i0, i1) (3 rows each) in FP64 via the host build.
+// Interaction matrix rows [i0, i1) (3 rows each) via the host build: fp32 0
+// FP64 integrals, 1 FP32 integrals, 2 the mixed assembly's choice (FP32 for
+// far blocks, FP64 near).
 int rhc_assemble(const rh_model_desc* d, int64_t i0, int64_t i1, int32_t fp32, double* out)
 {
     rh::PackedModel P;
@@ -37,7 +39,8 @@ int rhc_assemble(const rh_model_desc* d, int64_t i0, int64_t i1, int32_t fp32, d
         const double* x = P.colloc.data() + 3 * i;
         for (int64_t j = 0; j < P.n; j++) {
             double q[9];
-            if (fp32) rh::block_Q<float, true>(mv, j, x, q);
+            if (fp32 == 2) rh::block_Q_mixed<true, -1>(mv, j, x, q);
+            else if (fp32) rh::block_Q<float, true>(mv, j, x, q);
             else rh::block_Q<double, true>(mv, j, x, q);
             const double* rt = P.rowt.data() + 9 * i;
             for (int a = 0; a < 3; a++)

@@ -69,8 +69,14 @@ struct AsmTile {
     static constexpr int W = (U + 1) * 8 / int(sizeof(OUT));  // LDS row stride (elements)
 };
 
-template <class R, bool SYM, bool ROWT, class OUT, int KIND>
-__global__ __launch_bounds__(64) void k_assemble(DevModel m, OUT* __restrict__ Q, int64_t ld,
+#ifdef RH_ASM_WAVES
+#define RH_ASM_ATTR __attribute__((amdgpu_waves_per_eu(RH_ASM_WAVES)))
+#else
+#define RH_ASM_ATTR
+#endif
+// FAR32: far blocks in FP32 arithmetic, near ones in FP64 (block_Q_mixed).
+template <class R, bool SYM, bool ROWT, class OUT, int KIND, bool FAR32 = false>
+__global__ __launch_bounds__(64) RH_ASM_ATTR void k_assemble(DevModel m, OUT* __restrict__ Q, int64_t ld,
                                                  double* __restrict__ qdiag, int64_t jt, int64_t r0,
                                                  int64_t r1)
 {
@@ -98,7 +104,10 @@ __global__ __launch_bounds__(64) void k_assemble(DevModel m, OUT* __restrict__ Q
         for (int s = 0; s < cnt; s++) {
             const int64_t j = m.kidx[kb + s];
             double q[9];
-            block_Q<R, SYM, KIND>(mv, j, x, q);
+            if constexpr (FAR32)
+                block_Q_mixed<SYM, KIND>(mv, j, x, q);
+            else
+                block_Q<R, SYM, KIND>(mv, j, x, q);
             if (ROWT) {
                 double t[9];
                 for (int a = 0; a < 3; a++)
@@ -170,7 +179,7 @@ __global__ __launch_bounds__(64) void k_assemble(DevModel m, OUT* __restrict__ Q
 
 template <bool SYM, bool ROWT, class OUT>
 hipError_t asm_dispatch(const DevModel& m, OUT* Q, int64_t ld, double* qdiag, int64_t r0, int64_t r1,
-                        hipStream_t s)
+                        hipStream_t s, bool far32)
 {
     if (r1 <= r0) return hipSuccess;
     const int64_t gx = (r1 - r0 + 63) / 64;
@@ -182,8 +191,14 @@ hipError_t asm_dispatch(const DevModel& m, OUT* Q, int64_t ld, double* qdiag, in
         int64_t jt = std::max<int64_t>(4, (cnt + gy_target - 1) / gy_target);
         jt = std::min<int64_t>(jt, 256);
         const dim3 grid{unsigned(gx), unsigned((cnt + jt - 1) / jt), 1u};
-        if (slot == 0)
+        // FP32 storage: far blocks in FP32 arithmetic (general polyhedra stay FP64)
+        constexpr bool mixed = sizeof(OUT) == 4;
+        if (slot == 0 && mixed && far32)
+            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 0, mixed>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1);
+        else if (slot == 0)
             hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 0>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1);
+        else if (slot == 1 && mixed && far32)
+            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 1, mixed>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1);
         else if (slot == 1)
             hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 1>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1);
         else
@@ -634,8 +649,10 @@ template <int RW>
 __global__ __launch_bounds__(256) void k_matvec_f32(const float* __restrict__ Q, int64_t nrows,
                                                     int64_t ld, const double* __restrict__ x,
                                                     const double* __restrict__ add,
-                                                    double* __restrict__ y)
+                                                    double* __restrict__ y,
+                                                    const int32_t* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int lane = threadIdx.x & 63;
     const int64_t r0 = (int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * RW;
     if (r0 >= nrows) return;
@@ -669,8 +686,10 @@ template <int RW>
 __global__ __launch_bounds__(256) void k_matvec_f64(const double* __restrict__ Q, int64_t nrows,
                                                     int64_t ld, const double* __restrict__ x,
                                                     const double* __restrict__ add,
-                                                    double* __restrict__ y)
+                                                    double* __restrict__ y,
+                                                    const int32_t* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int lane = threadIdx.x & 63;
     const int64_t r0 = (int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * RW;
     if (r0 >= nrows) return;
@@ -1224,17 +1243,19 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
-// One relaxation step with its Anderson bookkeeping (one wave per block).
+// One relaxation step with its Anderson bookkeeping (one wave per block):
+// lane i updates element i; dF[k] (k < nh, oldest first) are the stored
+// differences, the newest of which (dF[nh - 1]) is the one written here when
+// dFw is set.
 static_assert(kRelaxLanes == 64, "k_relax_step reduces over one wave per block");
-__global__ __launch_bounds__(kRelaxLanes) void k_relax_step(int64_t n, const double* __restrict__ mats,
-                                                            const double* __restrict__ tab,
-                                                            const double* __restrict__ hext,
-                                                            const double* __restrict__ qdiag,
-                                                            const double* __restrict__ y,
-                                                            const double* __restrict__ M,
-                                                            double* __restrict__ F,
-                                                            double* __restrict__ Hout,
-                                                            double* __restrict__ partials, HistPack hp)
+__device__ __forceinline__ void relax_step_block(int64_t n, const double* __restrict__ mats,
+                                                 const double* __restrict__ tab,
+                                                 const double* __restrict__ hext,
+                                                 const double* __restrict__ qdiag,
+                                                 const double* __restrict__ y, const double* __restrict__ M,
+                                                 double* __restrict__ F, double* __restrict__ Hout,
+                                                 double* __restrict__ partials, double* Fp, double* xp,
+                                                 double* dFw, double* dXw, const double* const* dF, int nh)
 {
     const int64_t i = int64_t(blockIdx.x) * kRelaxLanes + threadIdx.x;
     double f2 = 0.0, mm = 0.0, hh = 0.0;
@@ -1257,22 +1278,22 @@ __global__ __launch_bounds__(kRelaxLanes) void k_relax_step(int64_t n, const dou
             F[j] = d;
             Hout[j] = hl[k];
             f2 += d * d;
-            if (hp.dFw) {
-                dfn[k] = d - hp.Fp[j];
-                hp.dFw[j] = dfn[k];
-                hp.dXw[j] = xk - hp.xp[j];
+            if (dFw) {
+                dfn[k] = d - Fp[j];
+                dFw[j] = dfn[k];
+                dXw[j] = xk - xp[j];
             }
-            hp.Fp[j] = d;
-            hp.xp[j] = xk;
+            Fp[j] = d;
+            xp[j] = xk;
         }
         mm = sqrt(mp[0] * mp[0] + mp[1] * mp[1] + mp[2] * mp[2]);
         hh = sqrt(hl[0] * hl[0] + hl[1] * hl[1] + hl[2] * hl[2]);
 #pragma unroll
         for (int h = 0; h < kHistMax; h++) {
-            if (h < hp.nh) {
+            if (h < nh) {
                 double a = 0.0, b = 0.0;
                 for (int k = 0; k < 3; k++) {
-                    const double v = (h == hp.nh - 1) ? dfn[k] : hp.dF[h][3 * i + k];
+                    const double v = (h == nh - 1) ? dfn[k] : dF[h][3 * i + k];
                     a += dfn[k] * v;
                     b += v * fi[k];
                 }
@@ -1286,7 +1307,7 @@ __global__ __launch_bounds__(kRelaxLanes) void k_relax_step(int64_t n, const dou
     hh = wave_max(hh);
 #pragma unroll
     for (int k = 0; k < 2 * kHistMax; k++)
-        if ((k % kHistMax) < hp.nh) dd[k] = wave_sum(dd[k]);
+        if ((k % kHistMax) < nh) dd[k] = wave_sum(dd[k]);
     if (threadIdx.x == 0) {
         double* o = partials + int64_t(kRelaxPart) * blockIdx.x;
         o[0] = f2;
@@ -1294,6 +1315,203 @@ __global__ __launch_bounds__(kRelaxLanes) void k_relax_step(int64_t n, const dou
         o[2] = hh;
 #pragma unroll
         for (int k = 0; k < 2 * kHistMax; k++) o[3 + k] = dd[k];
+    }
+}
+
+__global__ __launch_bounds__(kRelaxLanes) void k_relax_step(int64_t n, const double* __restrict__ mats,
+                                                            const double* __restrict__ tab,
+                                                            const double* __restrict__ hext,
+                                                            const double* __restrict__ qdiag,
+                                                            const double* __restrict__ y,
+                                                            const double* __restrict__ M,
+                                                            double* __restrict__ F,
+                                                            double* __restrict__ Hout,
+                                                            double* __restrict__ partials, HistPack hp)
+{
+    relax_step_block(n, mats, tab, hext, qdiag, y, M, F, Hout, partials, hp.Fp, hp.xp, hp.dFw, hp.dXw, hp.dF,
+                     hp.nh);
+}
+
+// The same step in the device-driven loop: history slots from the state.
+__global__ __launch_bounds__(kRelaxLanes) void k_relax_step_dev(int64_t n, const double* __restrict__ mats,
+                                                                const double* __restrict__ tab,
+                                                                const double* __restrict__ hext,
+                                                                const double* __restrict__ qdiag,
+                                                                const double* __restrict__ y,
+                                                                const double* __restrict__ M,
+                                                                double* __restrict__ F,
+                                                                double* __restrict__ Hout,
+                                                                double* __restrict__ partials, HistBufs hb,
+                                                                const RelaxDevState* __restrict__ st)
+{
+    if (st->done) return;
+    const int nh = st->hist, head = st->head;
+    const double* dF[kHistMax];
+#pragma unroll
+    for (int k = 0; k < kHistMax; k++) dF[k] = hb.dF[(head - nh + k + 2 * kHistMax) % kHistMax];
+    const int w = (head - 1 + kHistMax) % kHistMax;
+    relax_step_block(n, mats, tab, hext, qdiag, y, M, F, Hout, partials, hb.Fp, hb.xp,
+                     st->ext ? hb.dF[w] : nullptr, st->ext ? hb.dX[w] : nullptr, dF, nh);
+}
+
+// The Anderson step of the device-driven loop (one workgroup of 256 lanes):
+// the step partials reduced in a fixed order, then on lane 0 the residual
+// and convergence test, the restart safeguard, the history Gram matrix, the
+// regularized Cholesky solve of the least-squares problem and the update
+// coefficients, and the history extension for the next iteration.  The same
+// arithmetic as the host-driven loop.
+__global__ __launch_bounds__(256) void k_anderson_dev(int64_t nb, const double* __restrict__ partials,
+                                                      RelaxDevState* __restrict__ st)
+{
+    if (st->done) return;
+    __shared__ double red[256][kRelaxPart];
+    double acc[kRelaxPart];
+#pragma unroll
+    for (int k = 0; k < kRelaxPart; k++) acc[k] = 0.0;
+    for (int64_t b = threadIdx.x; b < nb; b += 256) {
+        const double* o = partials + int64_t(kRelaxPart) * b;
+        acc[0] += o[0];
+        acc[1] = fmax(acc[1], o[1]);
+        acc[2] = fmax(acc[2], o[2]);
+#pragma unroll
+        for (int k = 3; k < kRelaxPart; k++) acc[k] += o[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kRelaxPart; k++) red[threadIdx.x][k] = acc[k];
+    __syncthreads();
+    for (int stp = 128; stp > 0; stp >>= 1) {
+        if (int(threadIdx.x) < stp) {
+            red[threadIdx.x][0] += red[threadIdx.x + stp][0];
+            red[threadIdx.x][1] = fmax(red[threadIdx.x][1], red[threadIdx.x + stp][1]);
+            red[threadIdx.x][2] = fmax(red[threadIdx.x][2], red[threadIdx.x + stp][2]);
+            for (int k = 3; k < kRelaxPart; k++) red[threadIdx.x][k] += red[threadIdx.x + stp][k];
+        }
+        __syncthreads();
+    }
+    // lane 0 alone from here; its small arrays live in LDS
+    __shared__ double A[kHistMax * kHistMax], L[kHistMax * kHistMax], z[kHistMax], gam[kHistMax];
+    __shared__ int slot[kHistMax];
+    if (threadIdx.x != 0) return;
+    const double* v = red[0];
+    const int hist = st->hist;
+    const int head = st->head;
+    st->it += 1;
+    const double misfit = sqrt(v[0] / st->n);
+    st->misfit = misfit;
+    st->maxM = fmax(1e-17, v[1]);
+    st->maxH = fmax(1e-17, v[2]);
+    if (!isfinite(misfit)) {
+        st->fail = 1;
+        st->done = 2;
+        return;
+    }
+    if (misfit <= st->prec && st->it == 1 && st->keep_if_converged) {
+        st->done = 2;
+        return;
+    }
+    if (misfit <= st->prec || st->it >= st->max_iter) {
+        st->done = 1;
+        return;
+    }
+    for (int k = 0; k < hist; k++) slot[k] = (head - hist + k + 2 * kHistMax) % kHistMax;
+    bool plain = (hist == 0);
+    int nh = hist, hd = head;
+    if (misfit > 1e4 * st->best) {
+        nh = 0;
+        hd = 0;
+        st->beta = fmax(0.05, 0.5 * st->beta);
+        plain = true;
+    }
+    st->best = fmin(st->best, misfit);
+    const double beta = st->beta;
+    if (!plain) {
+        const int newest = slot[hist - 1];
+        for (int k = 0; k < hist; k++) {
+            st->G[newest * kHistMax + slot[k]] = v[3 + k];
+            st->G[slot[k] * kHistMax + newest] = v[3 + k];
+        }
+        // (G + lambda I) gam = r, G the Gram matrix of the differences in use
+        double tr = 0.0;
+        for (int a = 0; a < hist; a++)
+            for (int b = 0; b < hist; b++) A[a * kHistMax + b] = st->G[slot[a] * kHistMax + slot[b]];
+        for (int a = 0; a < hist; a++) tr += A[a * kHistMax + a];
+        const double lam = 1e-10 * (tr > 0 ? tr / hist : 1.0);
+        for (int a = 0; a < hist; a++) A[a * kHistMax + a] += lam;
+        bool ok = true;
+        for (int a = 0; a < hist && ok; a++)
+            for (int b = 0; b <= a; b++) {
+                double sm = A[a * kHistMax + b];
+                for (int k = 0; k < b; k++) sm -= L[a * kHistMax + k] * L[b * kHistMax + k];
+                if (a == b) {
+                    if (!(sm > 0.0)) {
+                        ok = false;
+                        break;
+                    }
+                    L[a * kHistMax + a] = sqrt(sm);
+                } else {
+                    L[a * kHistMax + b] = sm / L[b * kHistMax + b];
+                }
+            }
+        if (ok) {
+            for (int a = 0; a < hist; a++) {
+                double sm = v[3 + kHistMax + a];
+                for (int k = 0; k < a; k++) sm -= L[a * kHistMax + k] * z[k];
+                z[a] = sm / L[a * kHistMax + a];
+            }
+            for (int a = hist - 1; a >= 0; a--) {
+                double sm = z[a];
+                for (int k = a + 1; k < hist; k++) sm -= L[k * kHistMax + a] * gam[k];
+                gam[a] = sm / L[a * kHistMax + a];
+            }
+        } else {
+            nh = 0;
+            hd = 0;
+            plain = true;
+        }
+    }
+    st->cF = beta;
+    for (int b = 0; b < kHistMax; b++) {
+        st->cdX[b] = 0.0;
+        st->cdF[b] = 0.0;
+    }
+    if (!plain) {
+        for (int k = 0; k < hist; k++) {
+            st->cdX[slot[k]] = -gam[k];
+            st->cdF[slot[k]] = -beta * gam[k];
+        }
+    }
+    // the next iteration writes a new difference into slot hd
+    st->ext = 1;
+    st->head = (hd + 1) % kHistMax;
+    st->hist = nh + 1 < kHistMax ? nh + 1 : kHistMax;
+}
+
+// The update of the device-driven loop (grid-stride over the 3n components).
+__global__ __launch_bounds__(256) void k_update_dev(int64_t n3, double* __restrict__ x,
+                                                    const double* __restrict__ F, HistBufs hb,
+                                                    double* __restrict__ M_out,
+                                                    const RelaxDevState* __restrict__ st)
+{
+    const int done = st->done;
+    if (done == 2) return;
+    if (done == 1) {
+        for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n3; i += int64_t(gridDim.x) * 256)
+            M_out[i] = x[i] + F[i];
+        return;
+    }
+    const double cF = st->cF;
+    double cX[kHistMax], cD[kHistMax];
+#pragma unroll
+    for (int b = 0; b < kHistMax; b++) {
+        cX[b] = st->cdX[b];
+        cD[b] = st->cdF[b];
+    }
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n3; i += int64_t(gridDim.x) * 256) {
+        double acc = x[i] + cF * F[i];
+#pragma unroll
+        for (int b = 0; b < kHistMax; b++)
+            if (cX[b] != 0.0 || cD[b] != 0.0) acc += cX[b] * hb.dX[b][i] + cD[b] * hb.dF[b][i];
+        x[i] = acc;
     }
 }
 
@@ -1316,19 +1534,19 @@ __global__ __launch_bounds__(256) void k_current(int64_t ns, const int32_t* __re
 
 // ---------------------------------------------------------------- launchers
 hipError_t launch_assemble(const DevModel& m, void* Q, int elem_bytes, int64_t ld, double* qdiag,
-                           int64_t r0, int64_t r1, hipStream_t s)
+                           int64_t r0, int64_t r1, hipStream_t s, bool far32)
 {
     if (m.n == 0 || r1 <= r0) return hipSuccess;
     if (elem_bytes == 4) {
         float* q = static_cast<float*>(Q);
-        if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, r0, r1, s);
-        if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, r0, r1, s);
-        return asm_dispatch<true, true>(m, q, ld, qdiag, r0, r1, s);
+        if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, r0, r1, s, far32);
+        if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, r0, r1, s, far32);
+        return asm_dispatch<true, true>(m, q, ld, qdiag, r0, r1, s, far32);
     }
     double* q = static_cast<double*>(Q);
-    if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, r0, r1, s);
-    if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, r0, r1, s);
-    return asm_dispatch<true, true>(m, q, ld, qdiag, r0, r1, s);
+    if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, r0, r1, s, false);
+    if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, r0, r1, s, false);
+    return asm_dispatch<true, true>(m, q, ld, qdiag, r0, r1, s, false);
 }
 
 namespace {
@@ -1460,7 +1678,7 @@ hipError_t launch_field(const DevModel& m0, const double* M, const double* pts, 
 }
 
 hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t ld, const double* x,
-                         const double* add, double* y, hipStream_t s)
+                         const double* add, double* y, hipStream_t s, const int32_t* skip)
 {
     if (nrows <= 0) return hipSuccess;
 // rows per wave of the matrix pass: on the C-yoke (24,000 x 24,064) 4 rows
@@ -1474,10 +1692,10 @@ hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t l
     const int64_t blocks = (waves + 3) / 4;
     if (elem_bytes == 4)
         hipLaunchKernelGGL((k_matvec_f32<RW>), dim3(unsigned(blocks)), dim3(256), 0, s,
-                           static_cast<const float*>(Q), nrows, ld, x, add, y);
+                           static_cast<const float*>(Q), nrows, ld, x, add, y, skip);
     else
         hipLaunchKernelGGL((k_matvec_f64<RW>), dim3(unsigned(blocks)), dim3(256), 0, s,
-                           static_cast<const double*>(Q), nrows, ld, x, add, y);
+                           static_cast<const double*>(Q), nrows, ld, x, add, y, skip);
     return hipGetLastError();
 }
 
@@ -1499,6 +1717,30 @@ hipError_t launch_relax_step(int64_t n, const double* mats, const double* tab, c
     if (hp.nh < 0 || hp.nh > kHistMax || (hp.nh > 0 && !hp.dFw)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_relax_step, dim3(unsigned(relax_blocks(n))), dim3(kRelaxLanes), 0, s, n, mats, tab,
                        hext, qdiag, y, M, F, Hout, partials, hp);
+    return hipGetLastError();
+}
+
+hipError_t launch_relax_step_dev(int64_t n, const double* mats, const double* tab, const double* hext,
+                                 const double* qdiag, const double* y, const double* M, double* F,
+                                 double* Hout, double* partials, const HistBufs& hb, const RelaxDevState* st,
+                                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_relax_step_dev, dim3(unsigned(relax_blocks(n))), dim3(kRelaxLanes), 0, s, n, mats, tab,
+                       hext, qdiag, y, M, F, Hout, partials, hb, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_anderson_dev(int64_t nb, const double* partials, RelaxDevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_anderson_dev, dim3(1), dim3(256), 0, s, nb, partials, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_dev(int64_t n3, double* x, const double* F, const HistBufs& hb, double* M_out,
+                             const RelaxDevState* st, hipStream_t s)
+{
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(4096, (n3 + 255) / 256));
+    hipLaunchKernelGGL(k_update_dev, dim3(unsigned(blocks)), dim3(256), 0, s, n3, x, F, hb, M_out, st);
     return hipGetLastError();
 }
 

@@ -56,8 +56,11 @@ struct DevModel {
 // Interaction-matrix rows of elements [r0, r1) into Q (row-major, leading
 // dimension ld elements of elem_bytes = 4 or 8; Q points at row 3*r0); the
 // 3x3 diagonal blocks go to qdiag [9n] in FP64 and are stored as zeros in Q.
+// far32 (FP32 storage only): blocks and tetrahedra far from the collocation
+// point (beyond kNearRadii circumradii) are integrated in FP32, near ones and
+// general polyhedra in FP64.
 hipError_t launch_assemble(const DevModel& m, void* Q, int elem_bytes, int64_t ld, double* qdiag,
-                           int64_t r0, int64_t r1, hipStream_t s);
+                           int64_t r0, int64_t r1, hipStream_t s, bool far32 = false);
 
 // Number of partial-sum slices the field kernels use for np points (>= 1);
 // the work buffer needs 6 * np * slices doubles when slices > 1.  work_cap >= 0
@@ -74,9 +77,10 @@ hipError_t launch_field(const DevModel& m, const double* M, const double* pts, i
                         hipStream_t s, int64_t work_cap = -1);
 
 // y = Q x (+ add), Q: nrows x ld (ld multiple of 256, padded columns zero,
-// x padded with zeros to ld).
+// x padded with zeros to ld).  skip (may be null): the pass returns at once
+// when *skip is non-zero (a device-driven loop that has finished).
 hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t ld, const double* x,
-                         const double* add, double* y, hipStream_t s);
+                         const double* add, double* y, hipStream_t s, const int32_t* skip = nullptr);
 
 // Per-element implicit update: proposal M_prop = M_mat(H) with
 // H = y + hext + Qii M_prop solved locally; F = M_prop - M.  partials gets
@@ -107,6 +111,44 @@ struct HistPack {
 hipError_t launch_relax_step(int64_t n, const double* mats, const double* tab, const double* hext,
                              const double* qdiag, const double* y, const double* M, double* F,
                              double* Hout, double* partials, const HistPack& hp, hipStream_t s);
+
+// ------------------------------------------------------------ device-driven loop
+// State of the Anderson relaxation loop kept on the device, so that the host
+// enqueues iterations without waiting for each one: a pass of the matrix, the
+// relaxation step, the Anderson step (k_anderson_dev: residual, convergence
+// test, history Gram matrix, small least-squares solve, next coefficients)
+// and the update.  Once the Anderson step finds the loop finished it sets
+// done, and the kernels of iterations enqueued after that return at once.
+struct RelaxDevState {
+    int32_t it;         // iterations completed
+    int32_t done;       // 0 running; 1 finished, the update writes M = x + F; 2 finished, M kept
+    int32_t fail;       // non-finite residual
+    int32_t hist;       // differences in use in the current iteration
+    int32_t head;       // next difference slot (the newest is head - 1)
+    int32_t ext;        // the current iteration writes a difference into slot head - 1
+    int32_t max_iter;
+    int32_t keep_if_converged;  // a first iterate that meets prec is kept as it is
+    double prec, beta, best, misfit, maxM, maxH, n;
+    double G[kHistMax * kHistMax];  // dF_a . dF_b by physical slot
+    double cF;                      // update: x += cF F + sum_b cdX[b] dX[b] + cdF[b] dF[b]
+    double cdX[kHistMax], cdF[kHistMax];
+};
+struct HistBufs {
+    double* Fp;
+    double* xp;
+    double* dF[kHistMax];
+    double* dX[kHistMax];
+};
+// relaxation step of the device-driven loop (slots and history from st)
+hipError_t launch_relax_step_dev(int64_t n, const double* mats, const double* tab, const double* hext,
+                                 const double* qdiag, const double* y, const double* M, double* F,
+                                 double* Hout, double* partials, const HistBufs& hb, const RelaxDevState* st,
+                                 hipStream_t s);
+// Anderson step: one workgroup reduces the nb step partials and advances st.
+hipError_t launch_anderson_dev(int64_t nb, const double* partials, RelaxDevState* st, hipStream_t s);
+// x += cF F + sum_b (cdX[b] dX[b] + cdF[b] dF[b]); when st->done == 1, M_out = x + F.
+hipError_t launch_update_dev(int64_t n3, double* x, const double* F, const HistBufs& hb, double* M_out,
+                             const RelaxDevState* st, hipStream_t s);
 
 // out = hext + rowT*H64 - y - Qii*M per element (rowt may be null = identity).
 hipError_t launch_refine_defect(int64_t n, const double* H64, const double* rowt, const double* y,

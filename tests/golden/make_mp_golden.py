@@ -9,8 +9,9 @@ The closed form is what the mathematics fixes: H = (1/4pi) sum_f sigma_f
 [sum_e L_e m_e - Omega_f n_f], sigma_f = M.n_f (face-charge model of a
 uniformly magnetized polyhedron), evaluated at 50 digits.
 
-Also writes mp50_yoke.json (see yoke_flagged) and, with the argument
-"full", mp50_yoke_full.json (see yoke_full).
+Also writes mp50_yoke.json (see yoke_flagged), with the argument "full"
+mp50_yoke_full.json (see yoke_full) and with "large" mp50_large_blocks.json
+(see large_blocks).
 
 Run from the repository root: python tests/golden/make_mp_golden.py
 """
@@ -227,10 +228,38 @@ def yoke_full():
     print("wrote", path)
 
 
+def large_blocks():
+    """mp50_large_blocks.json: 50-digit interaction blocks of the reference's
+    "large" golden model (1,100 random RecMag blocks, tests/test_mpi_solve.py:37-46):
+    the six far blocks (16 to 27 block sizes away) where the reference's block
+    closed form (radrec.cpp) loses most, as (row i, column j, Q_ij)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import golden_models as G
+    from radiacuda_amd import objects as O
+    from radiacuda_amd import rad
+    G.build("large")
+    _top, iron, _coil = G.build("large")
+    items = list(O.walk(rad._reg, iron))
+    faces = [[0, 1, 3, 2], [4, 5, 7, 6], [0, 1, 5, 4], [2, 3, 7, 6], [0, 2, 6, 4], [1, 3, 7, 5]]
+    out = {"generator": "tests/golden/make_mp_golden.py large", "digits": 50, "blocks": []}
+    for i, j in [(925, 78), (448, 391), (457, 671), (962, 1090), (984, 14), (94, 1057)]:
+        c, dm = np.asarray(items[j][0].center), np.asarray(items[j][0].dims)
+        V = np.array([[c[0] + sx * dm[0] / 2, c[1] + sy * dm[1] / 2, c[2] + sz * dm[2] / 2]
+                      for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)])
+        x = np.asarray(items[i][0].center)
+        out["blocks"].append({"i": i, "j": j, "Q": Q_mp(V, faces, x).tolist()})
+    rad.UtiDelAll()
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mp50_large_blocks.json")
+    json.dump(out, open(path, "w"), indent=1)
+    print("wrote", path)
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "yoke":
         yoke_flagged()
     elif len(sys.argv) > 1 and sys.argv[1] == "full":
         yoke_full()
+    elif len(sys.argv) > 1 and sys.argv[1] == "large":
+        large_blocks()
     else:
         main()

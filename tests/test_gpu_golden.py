@@ -40,11 +40,12 @@ def test_large_golden_recmag_model():
     random RecMag blocks, chi = 20,000 iron, a racetrack coil.  Its
     cpu_asm_cpu_relax values are the method-4 state after the 3,000-sweep cap
     (misfit 0.174), which the oracle reproduces (tests/test_oracle.py).  Here:
-    the device assembly against the oracle (every 3x3 block <= 1e-10), the
-    device field and coil at that method-4 state against the reference's B
-    (the reference test's 1e-6), and the device solver through the drop-in
-    API to prec 1e-5 (where method 4 stopped short) against the FP64 fixed
-    point of the device matrix."""
+    the device assembly against the oracle, the device field and coil at that
+    method-4 state against the reference's B (the reference test's 1e-6), and
+    the device solver through the drop-in API to prec 1e-5 (where method 4
+    stopped short), then on to the FP64 Newton fixed point of its matrix.  The device blocks are compared per row scale
+    (the reference's block closed form loses up to 3e-4 of far blocks), and
+    six far blocks against 50-digit values (tests/golden/mp50_large_blocks.json)."""
     from oracle import oracle as OR
     from radiacuda_amd import _native as N
     from radiacuda_amd import objects as O
@@ -62,16 +63,29 @@ def test_large_golden_recmag_model():
         Q = m.assemble(N.RH_FP64)
         Qd = Q.export()
         Qo, _mask = models.oracle_assemble(P)
-        e_q = models.block_rel(Qd, Qo, n)
+        # against the oracle per row scale: the reference's block closed form
+        # (radrec.cpp:80-301) loses up to 3e-4 of a block 16 or more block
+        # sizes away, 1.2e-9 of the row; those blocks against 50 digits
+        row = np.abs(Qo).reshape(n, 9 * n).max(axis=1)
+        e_q = (np.abs(Qd - Qo).reshape(n, 3, n, 3).max(axis=(1, 3)) / row[:, None]).max()
+        gmp = json.load(open(os.path.join(HERE, "golden", "mp50_large_blocks.json")))["blocks"]
+        e_mp = max(np.abs(Qd[3 * g["i"]:3 * g["i"] + 3, 3 * g["j"]:3 * g["j"] + 3] - np.array(g["Q"])).max()
+                   / np.abs(np.array(g["Q"])).max() for g in gmp)
+        e_mp_o = max(np.abs(Qo[3 * g["i"]:3 * g["i"] + 3, 3 * g["j"]:3 * g["j"] + 3] - np.array(g["Q"])).max()
+                     / np.abs(np.array(g["Q"])).max() for g in gmp)
         cur = O.pack_currents(list(O.walk(rad._reg, coil)))
         pts = np.array(golden_models.POINTS)
         B = m.field(M4, pts)[0] + N.current_field(cur, pts)
         e_b = np.abs(B - ref).max() / np.abs(ref).max()
-        print("large golden: device blocks vs oracle %.2e; device field at the method-4 state vs "
-              "reference %.2e" % (e_q, e_b), flush=True)
-        assert e_q <= 1e-10
+        print("large golden: device blocks vs oracle %.2e of the row; far blocks vs 50 digits: device "
+              "%.2e, oracle %.2e; device field at the method-4 state vs reference %.2e"
+              % (e_q, e_mp, e_mp_o, e_b), flush=True)
+        assert e_q <= 3e-9
+        assert e_mp <= 1e-10
         assert e_b <= 1e-6
-        # the device solver through the drop-in API
+        # the device solver through the drop-in API to the reference's prec
+        # 1e-5, then on to the fixed point (chi = 20,000: prec 1e-5 on the
+        # implicit misfit still leaves M ~0.6 % from it)
         k = rad.RlxPre(top)
         res = rad.RlxAuto(k, 1e-5, 3000, 4)
         print("large golden: RlxAuto(1e-5, 3000) ->", res, flush=True)
@@ -80,8 +94,10 @@ def test_large_golden_recmag_model():
         Mg = np.stack([o.M for (o, _k, _s) in rad._reg[k].items])
         X = m.colloc()
         hext = N.current_field(cur, X)
-        Mn = models.fp64_fixed_point(Qd, mats, hext, M0=Mg)
-        dM = np.abs(Mg - Mn).max() / np.abs(Mn).max()
-        print("large golden: device solve vs FP64 fixed point %.2e" % dM, flush=True)
-        assert dM <= 1e-3
+        rt, Mt, _Ht = Q.relax(mats, hext, Mg, 1e-12, 20000, keep_m=True)
+        Mn = models.fp64_fixed_point(Qd, mats, hext, M0=Mt)
+        dM = np.abs(Mt - Mn).max() / np.abs(Mn).max()
+        print("large golden: FP64 device solve on to misfit %.1e (%d more iterations) vs FP64 Newton fixed "
+              "point %.2e" % (rt.misfit, rt.iters, dM), flush=True)
+        assert rt.misfit <= 1e-12 and dM <= 1e-8
         Q.close()

@@ -241,3 +241,35 @@ def test_sliver_tetrahedron_stays_finite_and_exact(L, t):
     if t >= 1e-4:
         Qo, _jit = models.oracle_assemble(P)
         assert np.abs(Q - Qo).max() <= 5e-5 * np.abs(Qo).max()
+
+
+def test_mixed_assembly_far_blocks_in_fp32(L):
+    """The mixed assembly's integration choice (host build of block_Q_mixed):
+    blocks whose collocation point lies within two circumradii of the source
+    element are integrated in FP64 (bit-identical to the FP64 assembly), the
+    rest in FP32.  On C-yoke rows a far block errs by up to ~1e-3 of its own
+    size, but every error stays below 1e-7 of the row's scale, the size of
+    FP32 storage rounding."""
+    from radiacuda_amd import workloads as W
+    rad.UtiDelAll()
+    _top, iron, _coil, _info = W.c_yoke()
+    items = list(O.walk(rad._reg, iron))
+    P = O.pack_elements(items)
+    n = len(items)
+    desc, keep = _abi.model_desc(P)
+    cen = np.array([np.mean(it[0].verts, axis=0) for it in items])
+    rad_ = np.array([np.sqrt(((it[0].verts - c) ** 2).sum(axis=1).max()) for it, c in zip(items, cen)])
+    for i in (0, 4321, 7999):
+        o64 = np.zeros((3, 3 * n))
+        omx = np.zeros((3, 3 * n))
+        assert L.rhc_assemble(ctypes.byref(desc), ctypes.c_int64(i), ctypes.c_int64(i + 1), ctypes.c_int32(0),
+                              o64.ctypes.data_as(dp)) == 0
+        assert L.rhc_assemble(ctypes.byref(desc), ctypes.c_int64(i), ctypes.c_int64(i + 1), ctypes.c_int32(2),
+                              omx.ctypes.data_as(dp)) == 0
+        d = np.abs(omx - o64).reshape(3, n, 3).max(axis=(0, 2))
+        s = np.abs(o64).reshape(3, n, 3).max(axis=(0, 2))
+        near = np.linalg.norm(cen - cen[i], axis=1) < 2.0 * rad_
+        assert near[i] and (d[near] == 0).all()
+        assert (d[~near] / s[~near]).max() > 1e-5
+        assert d.max() <= 1e-7 * np.abs(o64).max()
+    rad.UtiDelAll()
