@@ -8,8 +8,10 @@ Workloads (seeded, synthetic, built through the drop-in API; no files):
 * large yoke (the headline, `value`): 40,000 jittered tetrahedra of a C-shaped
   dipole yoke, MatSatIsoFrm steel, two racetrack coils; a step = assemble the
   120,000 x 120,000 interaction matrix + coil field at the collocation points
-  + relax to prec 1e-5 from the remanent state.  FP64 (115 GB FP64 matrix)
-  and mixed (57.6 GB FP32 matrix, FP64 self blocks, FP64 refinement);
+  + relax to prec 1e-5 from the remanent state.  FP64 and mixed (FP32 matrix,
+  FP64 self blocks, FP64 refinement); the matrix is stored in the
+  symmetric-block layout (5 planes per element row: 64 GB in FP64, 32 GB in
+  FP32) unless --dense-rows asks for row-major 3x3 blocks (115 / 57.6 GB);
 * Halbach ring: 20,000 hexahedral polyhedra, MatLin 0.06/0.17, B at 2,000,000
   points in the bore; a step = one field sweep, FP64 and FP32 integrals;
 * C-yoke: 8,000 tetrahedra; a step = assemble + relax (prec 1e-4) + B of the
@@ -156,7 +158,8 @@ def run_large_yoke(ctx, args, cpu):
         if args.only and label != args.only:
             continue
         log("large yoke %s: allocating and assembling the matrix" % label)
-        Q = model.assemble(prec)
+        Q = model.assemble(prec, dense_rows=args.dense_rows)
+        planes, mld = Q.format()
         opts = _abi.RelaxOpts(args.prec_large, args.max_iter, 0, 0.0, 9, 0)
         res = _abi.RelaxResult()
 
@@ -181,27 +184,32 @@ def run_large_yoke(ctx, args, cpu):
                  solve_s=phase["solve"] / K, iters=res.iters, misfit=res.misfit,
                  converged=bool(res.converged), refined=res.refined,
                  kernel_ms={f: kt[f][0] for f in kt}, kernel_counts={f: kt[f][1] for f in kt},
-                 M=d_M.cpu().numpy(), B=d_B.cpu().numpy())
+                 M=d_M.cpu().numpy(), B=d_B.cpu().numpy(), planes=planes, mld=mld)
         Q.close()
         out[label] = r
         log("large yoke %s: %.1f ms/step (asm %.1f ms, solve %.3f s in %d it, misfit %.2e)"
             % (label, r["ms_per_step"], r["asm_ms"], r["solve_s"], r["iters"], r["misfit"]))
     model.close()
     n3 = 3 * n
-    ld = (n3 + 255) // 256 * 256
     sec = {"elements": n, "matrix": "%dx%d" % (n3, n3), "prec": args.prec_large}
     for label, r in out.items():
         es = 8.0 if label == "fp64" else 4.0
         mv = r["kernel_ms"]["matvec"]
-        bytes_pass = es * n3 * ld + 72.0 * n
+        # stored matrix bytes (every element row, padding included) + the
+        # iterate read and the product written
+        stored = es * (r["planes"] * n * r["mld"] if r["planes"] else n3 * r["mld"])
+        bytes_pass = stored + 72.0 * n
+        r["bytes_pass"] = bytes_pass
         sec[label] = {
+            "layout": ("symmetric blocks, %d planes of %d" % (r["planes"], r["mld"]) if r["planes"]
+                       else "row-major 3x3 blocks, rows of %d" % r["mld"]),
             "step_ms": r["ms_per_step"], "end_to_end_qblocks_per_s": n * n / (r["ms_per_step"] * 1e-3),
             "asm_ms": r["asm_ms"], "asm_qblocks_per_s": n * n / (r["asm_ms"] * 1e-3),
             "asm_kernel_ms": r["kernel_ms"]["asm"], "solve_s": r["solve_s"], "solve_iters": r["iters"],
             "solve_misfit": r["misfit"], "matrix_passes": r["kernel_counts"]["matvec"] // max(args.steps, 1),
             "matvec_kernel_ms": mv, "matvec_TBps": bytes_pass / (mv * 1e-3) / 1e12 if mv > 0 else None,
             "matvec_hbm_frac": bytes_pass / (mv * 1e-3) / (PEAK_HBM * 1e9) if mv > 0 else None,
-            "matrix_GB": es * n3 * ld / 1e9}
+            "matrix_GB": stored / 1e9}
         if label == "mixed":
             sec[label]["fp64_refinement_passes"] = r["refined"]
             sec[label]["refine_field_ms"] = r["kernel_ms"]["refine"]
@@ -257,9 +265,16 @@ def cpu_baseline_large(P, mats, n, r64, args):
     for label, th in (("single_thread", 1), ("openmp", threads)):
         OR.set_threads(th)
         tc = time.perf_counter()
-        for i in samp:
-            OR.assemble(om, rows=(int(i), int(i) + 1))
-        t_asm = (time.perf_counter() - tc) * n / rows
+        if th == 1:
+            for i in samp:
+                OR.assemble(om, rows=(int(i), int(i) + 1))
+            nrows = rows
+        else:
+            # one contiguous block of rows per call, so every thread has rows
+            r0 = int(samp[0]) if int(samp[0]) + rows * th <= n else n - rows * th
+            OR.assemble(om, rows=(r0, r0 + rows * th))
+            nrows = rows * th
+        t_asm = (time.perf_counter() - tc) * n / nrows
         tc = time.perf_counter()
         _M, _H, it, stats = OR.relax_m4(Qs, mats_s, hext, args.prec_large, 3000)
         t_small = time.perf_counter() - tc
@@ -362,7 +377,8 @@ def run_c_yoke(ctx, args, cpu):
     for label, prec in (("fp64", N.RH_FP64), ("mixed", N.RH_MIXED)):
         if args.only and label != args.only:
             continue
-        Q = model.assemble(prec)
+        Q = model.assemble(prec, dense_rows=args.dense_rows)
+        planes = Q.format()[0]
         opts = _abi.RelaxOpts(args.prec, args.max_iter, 0, 0.0, 9, 0)
         res = _abi.RelaxResult()
 
@@ -388,7 +404,8 @@ def run_c_yoke(ctx, args, cpu):
         out[label] = dict(ms_per_step=1e3 * wall / K, asm_ms=1e3 * phase["asm"] / K,
                           solve_s=phase["solve"] / K, fld_ms=1e3 * phase["fld"] / K, iters=res.iters,
                           misfit=res.misfit, refined=res.refined,
-                          kernel_ms={f: kt[f][0] for f in kt}, B=d_B.cpu().numpy(), M=d_M.cpu().numpy())
+                          kernel_ms={f: kt[f][0] for f in kt}, B=d_B.cpu().numpy(), M=d_M.cpu().numpy(),
+                          planes=planes)
         Q.close()
         r = out[label]
         log("C-yoke %s: %.1f ms/step (asm %.2f ms, solve %.3f s in %d it, fld %.1f ms)"
@@ -405,7 +422,7 @@ def run_c_yoke(ctx, args, cpu):
         sec[label] = {"end_to_end_ms": r["ms_per_step"], "end_to_end_points_per_s": npts / (r["ms_per_step"] * 1e-3),
                       "asm_qblocks_per_s": n * n / (r["asm_ms"] * 1e-3), "asm_kernel_ms": r["kernel_ms"]["asm"],
                       "solve_s": r["solve_s"], "solve_iters": r["iters"], "solve_misfit": r["misfit"],
-                      "matvec_kernel_ms": r["kernel_ms"]["matvec"],
+                      "matvec_kernel_ms": r["kernel_ms"]["matvec"], "matrix_planes": r["planes"],
                       "fld_points_per_s": npts / (r["fld_ms"] * 1e-3), "fld_kernel_ms": fl,
                       "fld_declared_TFLOPs": flops / (fl * 1e-3) / 1e12 if fl > 0 else None}
     if "fp64" in out:
@@ -449,6 +466,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline legs")
     ap.add_argument("--cpu-rows", type=int, default=16, help="large-yoke rows in the CPU sample")
     ap.add_argument("--cpu-points", type=int, default=16, help="Halbach points in the CPU sample")
+    ap.add_argument("--dense-rows", action="store_true",
+                    help="store row-major 3x3 blocks even where the symmetric-block layout applies")
     ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
                     help="time one precision only (profiler runs); prints no JSON line")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -486,16 +505,17 @@ def main():
     cpu_base = cpu_baseline_large(P_large, mats_large, n, r64, args) if cpu else None
     # roofline of the headline's dominant kernel: the FP64 matrix pass
     n3 = 3 * n
-    ld = (n3 + 255) // 256 * 256
     mv_ms = r64["kernel_ms"]["matvec"]
-    bytes_pass = 8.0 * n3 * ld + 72.0 * n
+    bytes_pass = r64["bytes_pass"]
     achieved = bytes_pass / (mv_ms * 1e-3) / 1e9
+    kname = ("k_matvec_sym_f64<%d> (FP64 matrix pass, large yoke)" % r64["planes"] if r64["planes"]
+             else "k_matvec_f64<4> (FP64 matrix pass, large yoke)")
     traffic = None
     traffic_src = None
     try:
         with open(os.path.join(ROOT, "profiles", "matvec_traffic.json")) as fh:
             tr = json.load(fh)
-        if tr.get("workload_elements") == n:
+        if tr.get("workload_elements") == n and tr.get("planes") == r64["planes"]:
             traffic = float(tr["traffic_bytes"])
             traffic_src = "profiles/matvec_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)"
     except (OSError, ValueError, KeyError):
@@ -513,7 +533,7 @@ def main():
                                "to end with a 1M-point map" % (n, n3, n3, args.prec_large),
                    "elements": n, "matrix": "%dx%d" % (n3, n3)},
         "large_yoke": large, "halbach": halb, "c_yoke": cyoke,
-        "roofline": {"bound": "hbm", "kernel": "k_matvec_f64<4> (FP64 matrix pass, large yoke)",
+        "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s", "frac": achieved / PEAK_HBM,
                      "traffic": traffic, "traffic_source": traffic_src,
                      "bytes_per_launch": bytes_pass, "avg_launch_ms": mv_ms},

@@ -130,6 +130,12 @@ typedef struct {
                                * circumradii of a block or tetrahedron) are
                                * integrated in FP32, near and self blocks in FP64;
                                * 1: every block integrated in FP64               */
+    int32_t dense_rows;       /* 0: a model without transform copies or row
+                               * transforms (every block a symmetric tensor) is
+                               * stored in the symmetric-block layout (see
+                               * rh_matrix_format); 1: always row-major 3x3
+                               * blocks                                          */
+    int32_t reserved;         /* 0                                               */
 } rh_matrix_opts;
 #define RH_STREAM_BY_POLICY (-1)
 rh_status rh_matrix_assemble_ex(rh_model *m, rh_precision prec, const rh_matrix_opts *opts,
@@ -140,8 +146,22 @@ rh_status rh_matrix_destroy(rh_matrix *q);
 /* Copies rows [row0, row0+nrows) of the 3n x 3n matrix to host doubles. */
 rh_status rh_matrix_export(const rh_matrix *q, int64_t row0, int64_t nrows, double *out);
 /* Raw device pointer and element size (4 or 8 bytes) of the stored matrix
- * (RH_ERR_UNSUPPORTED when part of it is streamed). */
+ * (RH_ERR_UNSUPPORTED when part of it is streamed), in the layout that
+ * rh_matrix_format reports. */
 rh_status rh_matrix_device_ptr(const rh_matrix *q, void **dptr, int32_t *elem_bytes);
+/* Storage layout of the matrix (no counterpart in the reference, which stores
+ * row-major 3x3 blocks, radintrc.h:126).
+ *   planes == 0: row-major, element row i = rows 3i..3i+2 of *ld values
+ *     (columns 3j..3j+2 hold block (i, j); padding zero).
+ *   planes == 5 or 6: symmetric-block layout, used when no element has
+ *     transform copies or a row transform, so that every off-diagonal block
+ *     is the symmetric field tensor of source j at collocation point i.
+ *     Element row i = planes rows of *ld values: (xx, xy, xz, yy, yz[, zz])
+ *     of blocks (i, 0 .. n-1).  With 5 planes zz = -(xx + yy) (the tensor is
+ *     traceless outside its source); a model where a collocation point lies
+ *     inside another element is stored with 6.
+ * The diagonal blocks are held apart in FP64 and read as zeros in both. */
+rh_status rh_matrix_format(const rh_matrix *q, int32_t *planes, int64_t *ld);
 /* Re-assembles into an existing matrix (same model); the timed unit of work. */
 rh_status rh_matrix_reassemble(rh_matrix *q);
 

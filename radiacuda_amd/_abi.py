@@ -31,7 +31,8 @@ class RelaxOpts(ctypes.Structure):
 
 class MatrixOpts(ctypes.Structure):
     _fields_ = [("device_bytes", ctypes.c_int64), ("resident_fraction", ctypes.c_double),
-                ("allow_streaming", ctypes.c_int32), ("fp64_integrals", ctypes.c_int32)]
+                ("allow_streaming", ctypes.c_int32), ("fp64_integrals", ctypes.c_int32),
+                ("dense_rows", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class RelaxResult(ctypes.Structure):

@@ -73,6 +73,7 @@ def _load():
     L.rh_matrix_destroy.argtypes = [_vp]
     L.rh_matrix_export.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, _d]
     L.rh_matrix_device_ptr.argtypes = [_vp, ctypes.POINTER(_vp), _i32]
+    L.rh_matrix_format.argtypes = [_vp, _i32, _i64]
     L.rh_matrix_reassemble.argtypes = [_vp]
     L.rh_relax.argtypes = [_vp, _d, _d, _d, _d, ctypes.POINTER(_abi.RelaxOpts),
                            ctypes.POINTER(_abi.RelaxResult)]
@@ -114,7 +115,7 @@ def _load():
                  "rh_field_adjoint", "rh_field_adjoint_dev", "rh_relax_adjoint_dev",
                  "rh_relax_adjoint", "rh_model_set_field_form", "rh_model_field_form",
                  "rh_model_mesh_counts", "rh_matrix_assemble_ex", "rh_matrix_layout",
-                 "rh_relax_tab", "rh_relax_tab_dev", "rh_set_fallback"]:
+                 "rh_relax_tab", "rh_relax_tab_dev", "rh_set_fallback", "rh_matrix_format"]:
         getattr(L, name).restype = st
     _RAW = L
     return L
@@ -241,18 +242,28 @@ class Matrix:
     """Device-resident interaction matrix (rh_matrix)."""
 
     def __init__(self, model, prec=RH_MIXED, device_bytes=0, resident_fraction=-1.0,
-                 allow_streaming=False, fp64_integrals=False):
-        """device_bytes / resident_fraction / allow_streaming / fp64_integrals:
-        rh_matrix_opts (rows beyond the device budget are streamed from pinned
-        host memory when allowed; allow_streaming="policy" follows
+                 allow_streaming=False, fp64_integrals=False, dense_rows=False):
+        """device_bytes / resident_fraction / allow_streaming / fp64_integrals /
+        dense_rows: rh_matrix_opts (rows beyond the device budget are streamed
+        from pinned host memory when allowed; allow_streaming="policy" follows
         rh_get_fallback; fp64_integrals integrates every block of an RH_MIXED
-        matrix in FP64 instead of the far ones in FP32)."""
+        matrix in FP64 instead of the far ones in FP32; dense_rows stores
+        row-major 3x3 blocks even where the symmetric-block layout applies)."""
         self.model = model
         self.n = model.n
         self.h = _vp()
         stream = _abi.RH_STREAM_BY_POLICY if allow_streaming == "policy" else (1 if allow_streaming else 0)
-        o = _abi.MatrixOpts(int(device_bytes), float(resident_fraction), stream, 1 if fp64_integrals else 0)
+        o = _abi.MatrixOpts(int(device_bytes), float(resident_fraction), stream, 1 if fp64_integrals else 0,
+                            1 if dense_rows else 0, 0)
         check(lib().rh_matrix_assemble_ex(model.h, int(prec), ctypes.byref(o), ctypes.byref(self.h)))
+
+    def format(self):
+        """(planes, ld): 0 for row-major 3x3 blocks (ld values per row), 5 or
+        6 for the symmetric-block planes (ld values per plane)."""
+        p = np.zeros(1, np.int32)
+        ld = np.zeros(1, np.int64)
+        check(lib().rh_matrix_format(self.h, p.ctypes.data_as(_i32), ld.ctypes.data_as(_i64)))
+        return int(p[0]), int(ld[0])
 
     def layout(self):
         """(rows resident on the device, rows streamed from host memory)."""

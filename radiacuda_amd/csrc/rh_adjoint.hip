@@ -207,6 +207,37 @@ __global__ __launch_bounds__(256) void k_matvec_t(const T* __restrict__ Q, int64
     }
 }
 
+// Transposed pass over the symmetric-block layout: every stored block is
+// symmetric, so (Q^T x)_j = sum_i Q_ij x_i; part[rc][3j + a] sums the element
+// rows of chunk rc.  Each lane owns two consecutive column elements.
+template <class T, int P>
+__global__ __launch_bounds__(256) void k_matvec_t_sym(const T* __restrict__ Q, int64_t nel, int64_t ldp,
+                                                      int64_t el_per_chunk, const double* __restrict__ x,
+                                                      double* __restrict__ part, int64_t ld)
+{
+    const int64_t j0 = (int64_t(blockIdx.x) * 256 + threadIdx.x) * 2;
+    const int64_t i0 = int64_t(blockIdx.y) * el_per_chunk;
+    const int64_t i1 = (i0 + el_per_chunk < nel) ? (i0 + el_per_chunk) : nel;
+    if (j0 >= ldp) return;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = i0; i < i1; i++) {
+        const T* b = Q + i * P * ldp + j0;
+        const double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const double xx = double(b[e]), xy = double(b[ldp + e]), xz = double(b[2 * ldp + e]);
+            const double yy = double(b[3 * ldp + e]), yz = double(b[4 * ldp + e]);
+            const double zz = (P == 6) ? double(b[5 * ldp + e]) : -(xx + yy);
+            acc[3 * e] += xx * x0 + xy * x1 + xz * x2;
+            acc[3 * e + 1] += xy * x0 + yy * x1 + yz * x2;
+            acc[3 * e + 2] += xz * x0 + yz * x1 + zz * x2;
+        }
+    }
+    double* o = part + int64_t(blockIdx.y) * ld + 3 * j0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) o[k] = acc[k];
+}
+
 // mu_i = K_i^T lam_i (linear susceptibility tensor of element i, element frame)
 // followed by rowT_i^T when the rows carry a transform.
 __device__ __forceinline__ void lin_tensor(const double* mat, double K[9])
@@ -393,6 +424,31 @@ hipError_t launch_matvec_t(const void* Q, int elem_bytes, int64_t nrows, int64_t
         hipLaunchKernelGGL(k_matvec_t<double>, grid, dim3(256), 0, s, static_cast<const double*>(Q),
                            nrows, ld, rpc, x, part);
     hipLaunchKernelGGL(k_sum_parts, dim3(unsigned((nrows + 255) / 256)), dim3(256), 0, s, nrows,
+                       chunks, ld, part, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_matvec_t_sym(const void* Q, int elem_bytes, int planes, int64_t nel, int64_t ldp,
+                               const double* x, double* part, int64_t chunks, double* y, hipStream_t s)
+{
+    if ((planes != 5 && planes != 6) || ldp % 256 != 0 || chunks < 1) return hipErrorInvalidValue;
+    const int64_t epc = (nel + chunks - 1) / chunks;
+    const int64_t ld = 3 * ldp;
+    const dim3 grid{unsigned((ldp / 2 + 255) / 256), unsigned(chunks), 1u};
+    if (elem_bytes == 4) {
+        const float* q = static_cast<const float*>(Q);
+        if (planes == 5)
+            hipLaunchKernelGGL((k_matvec_t_sym<float, 5>), grid, dim3(256), 0, s, q, nel, ldp, epc, x, part, ld);
+        else
+            hipLaunchKernelGGL((k_matvec_t_sym<float, 6>), grid, dim3(256), 0, s, q, nel, ldp, epc, x, part, ld);
+    } else {
+        const double* q = static_cast<const double*>(Q);
+        if (planes == 5)
+            hipLaunchKernelGGL((k_matvec_t_sym<double, 5>), grid, dim3(256), 0, s, q, nel, ldp, epc, x, part, ld);
+        else
+            hipLaunchKernelGGL((k_matvec_t_sym<double, 6>), grid, dim3(256), 0, s, q, nel, ldp, epc, x, part, ld);
+    }
+    hipLaunchKernelGGL(k_sum_parts, dim3(unsigned((3 * nel + 255) / 256)), dim3(256), 0, s, 3 * nel,
                        chunks, ld, part, y);
     return hipGetLastError();
 }

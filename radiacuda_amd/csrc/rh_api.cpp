@@ -252,7 +252,13 @@ struct rh_matrix {
     rh_precision prec = RH_MIXED;
     bool far32 = false;  // far blocks integrated in FP32 (RH_MIXED)
     int elem_bytes = 4;
-    int64_t n3 = 0, ld = 0;
+    // Storage of an element row: planes == 0, its 3 matrix rows of ld values;
+    // planes == 5 or 6, the symmetric-block planes of ldp values (ld = 3*ldp
+    // is then the padded length of the iterate).
+    int planes = 0;
+    int64_t n3 = 0, ld = 0, ldp = 0;
+    int32_t* d_flag = nullptr;  // assembly: a traced block met in the 5-plane layout
+    rh_matrix_opts opts{};
     void* d_Q = nullptr;  // rows of elements [0, res_elems)
     double* d_qdiag = nullptr;
     // out-of-core part: rows of elements [res_elems, n) in pinned host memory,
@@ -266,6 +272,11 @@ struct rh_matrix {
     hipEvent_t ev_free[2] = {nullptr, nullptr};   // tile b may be overwritten
     RelaxWork* rw = nullptr;  // relaxation work, kept between solves
     size_t row_bytes() const { return size_t(ld) * size_t(elem_bytes); }
+    // bytes of one element row's storage
+    size_t elem_stride() const
+    {
+        return planes ? size_t(planes) * size_t(ldp) * size_t(elem_bytes) : 3 * row_bytes();
+    }
     bool streamed() const { return res_elems < n3 / 3; }
 };
 
@@ -502,15 +513,31 @@ rh_status rh_model_counts(const rh_model* m, int64_t* counts8)
 }
 
 // ---------------------------------------------------------------- assembly
-static rh_status assemble_into(rh_matrix* q)
+// y = Q x over element rows [e0, e1) held at Qp (the first of them).
+static hipError_t matvec_rows(const rh_matrix* q, const void* Qp, int64_t e0, int64_t e1, const double* x,
+                              double* y, hipStream_t s, const int32_t* skip = nullptr)
+{
+    if (q->planes)
+        return rh::launch_matvec_sym(Qp, q->elem_bytes, q->planes, e1 - e0, q->ldp, x, nullptr, y + 3 * e0, s,
+                                     skip);
+    return rh::launch_matvec(Qp, q->elem_bytes, 3 * (e1 - e0), q->ld, x, nullptr, y + 3 * e0, s, skip);
+}
+
+// Assembles every element row; *traced is set when the 5-plane layout met a
+// block with a non-zero trace (the matrix must then be stored with 6 planes).
+static rh_status assemble_rows(rh_matrix* q, bool* traced)
 {
     rh_model* m = q->m;
     RH_HIP(enter_device(m->device));
     hipStream_t s = m->stream;
     const int64_t n = m->host.n;
+    const int64_t lds = q->planes ? q->ldp : q->ld;
+    *traced = false;
+    if (q->planes == 5) RH_HIP(hipMemsetAsync(q->d_flag, 0, sizeof(int32_t), s));
     Timer t("asm", s);
     t.begin();
-    RH_HIP(rh::launch_assemble(m->dev, q->d_Q, q->elem_bytes, q->ld, q->d_qdiag, 0, q->res_elems, s, q->far32));
+    RH_HIP(rh::launch_assemble(m->dev, q->d_Q, q->elem_bytes, lds, q->d_qdiag, 0, q->res_elems, s, q->far32,
+                               q->planes, q->d_flag));
     // streamed rows: assemble a tile on the device, copy it to pinned host
     // memory on the copy stream while the next tile is assembled
     int64_t tile = 0;
@@ -518,16 +545,23 @@ static rh_status assemble_into(rh_matrix* q)
         const int b = int(tile & 1);
         const int64_t e1 = std::min(n, e0 + q->tile_elems);
         RH_HIP(hipStreamWaitEvent(s, q->ev_free[b], 0));
-        RH_HIP(rh::launch_assemble(m->dev, q->stage[b], q->elem_bytes, q->ld, q->d_qdiag, e0, e1, s, q->far32));
+        RH_HIP(rh::launch_assemble(m->dev, q->stage[b], q->elem_bytes, lds, q->d_qdiag, e0, e1, s, q->far32,
+                                   q->planes, q->d_flag));
         RH_HIP(hipEventRecord(q->ev_ready[b], s));
         RH_HIP(hipStreamWaitEvent(q->copy, q->ev_ready[b], 0));
-        char* dst = static_cast<char*>(q->h_Q) + size_t(3 * (e0 - q->res_elems)) * q->row_bytes();
-        RH_HIP(hipMemcpyAsync(dst, q->stage[b], size_t(3 * (e1 - e0)) * q->row_bytes(),
-                              hipMemcpyDeviceToHost, q->copy));
+        char* dst = static_cast<char*>(q->h_Q) + size_t(e0 - q->res_elems) * q->elem_stride();
+        RH_HIP(hipMemcpyAsync(dst, q->stage[b], size_t(e1 - e0) * q->elem_stride(), hipMemcpyDeviceToHost,
+                              q->copy));
         RH_HIP(hipEventRecord(q->ev_free[b], q->copy));
     }
     t.end();
     if (q->copy) RH_HIP(hipStreamSynchronize(q->copy));
+    if (q->planes == 5) {
+        int32_t f = 0;
+        RH_HIP(hipMemcpyAsync(&f, q->d_flag, sizeof(f), hipMemcpyDeviceToHost, s));
+        RH_HIP(hipStreamSynchronize(s));
+        *traced = f != 0;
+    }
     RH_HIP(hipStreamSynchronize(s));
     return RH_OK;
 }
@@ -537,69 +571,71 @@ static rh_status assemble_into(rh_matrix* q)
 static hipError_t matrix_pass(rh_matrix* q, const double* x, double* y, hipStream_t s)
 {
     const int64_t n = q->m->host.n;
-    hipError_t e = rh::launch_matvec(q->d_Q, q->elem_bytes, 3 * q->res_elems, q->ld, x, nullptr, y, s);
+    hipError_t e = matvec_rows(q, q->d_Q, 0, q->res_elems, x, y, s);
     int64_t tile = 0;
     for (int64_t e0 = q->res_elems; e == hipSuccess && e0 < n; e0 += q->tile_elems, tile++) {
         const int b = int(tile & 1);
         const int64_t e1 = std::min(n, e0 + q->tile_elems);
-        const char* src = static_cast<const char*>(q->h_Q) + size_t(3 * (e0 - q->res_elems)) * q->row_bytes();
+        const char* src = static_cast<const char*>(q->h_Q) + size_t(e0 - q->res_elems) * q->elem_stride();
         e = hipStreamWaitEvent(q->copy, q->ev_free[b], 0);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(q->stage[b], src, size_t(3 * (e1 - e0)) * q->row_bytes(),
-                               hipMemcpyHostToDevice, q->copy);
+            e = hipMemcpyAsync(q->stage[b], src, size_t(e1 - e0) * q->elem_stride(), hipMemcpyHostToDevice, q->copy);
         if (e == hipSuccess) e = hipEventRecord(q->ev_ready[b], q->copy);
         if (e == hipSuccess) e = hipStreamWaitEvent(s, q->ev_ready[b], 0);
-        if (e == hipSuccess)
-            e = rh::launch_matvec(q->stage[b], q->elem_bytes, 3 * (e1 - e0), q->ld, x, nullptr, y + 3 * e0, s);
+        if (e == hipSuccess) e = matvec_rows(q, q->stage[b], e0, e1, x, y, s);
         if (e == hipSuccess) e = hipEventRecord(q->ev_free[b], s);
     }
     return e;
 }
 
-static void free_matrix(rh_matrix* q)
+// Frees the matrix storage (device rows, streamed rows, tiles, copy stream).
+static void release_storage(rh_matrix* q)
 {
-    if (!q) return;
     (void)hipSetDevice(q->device);
     if (q->copy) (void)hipStreamSynchronize(q->copy);
     dfree(q->d_Q);
-    dfree(q->d_qdiag);
+    q->d_Q = nullptr;
     dfree(q->stage[0]);
     dfree(q->stage[1]);
+    q->stage[0] = q->stage[1] = nullptr;
     if (q->h_Q) (void)hipHostFree(q->h_Q);
+    q->h_Q = nullptr;
     for (int b = 0; b < 2; b++) {
         if (q->ev_ready[b]) (void)hipEventDestroy(q->ev_ready[b]);
         if (q->ev_free[b]) (void)hipEventDestroy(q->ev_free[b]);
+        q->ev_ready[b] = q->ev_free[b] = nullptr;
     }
     if (q->copy) (void)hipStreamDestroy(q->copy);
+    q->copy = nullptr;
+    q->tile_elems = 0;
+}
+
+static void free_matrix(rh_matrix* q)
+{
+    if (!q) return;
+    release_storage(q);
+    dfree(q->d_qdiag);
+    dfree(q->d_flag);
     delete q->rw;
     delete q;
 }
 
-rh_status rh_matrix_assemble_ex(rh_model* m, rh_precision prec, const rh_matrix_opts* opts,
-                                rh_matrix** out)
+// Places and allocates the storage of q for q->planes (q->opts as validated):
+// the resident element rows (a whole number of 64-element groups when some
+// are streamed) and, beyond the budget, pinned host rows and two tiles.
+static rh_status alloc_storage(rh_matrix* q)
 {
-    if (!m || !out) return fail(RH_ERR_ARG, "null argument");
-    *out = nullptr;
-    if (prec != RH_FP64 && prec != RH_MIXED) return fail(RH_ERR_ARG, "unknown precision");
-    rh_matrix_opts o{};
-    o.resident_fraction = -1.0;
-    if (opts) o = *opts;
-    if (o.resident_fraction > 1.0) return fail(RH_ERR_ARG, "resident_fraction above 1");
-    if (o.allow_streaming != 0 && o.allow_streaming != 1 && o.allow_streaming != RH_STREAM_BY_POLICY)
-        return fail(RH_ERR_ARG, "allow_streaming must be 0, 1 or RH_STREAM_BY_POLICY");
-    if (o.allow_streaming == RH_STREAM_BY_POLICY) o.allow_streaming = (g_fallback == RH_FALLBACK_BREAK) ? 0 : 1;
-    if (o.fp64_integrals != 0 && o.fp64_integrals != 1) return fail(RH_ERR_ARG, "fp64_integrals must be 0 or 1");
-    RH_HIP(enter_device(m->device));
+    rh_model* m = q->m;
+    const rh_matrix_opts& o = q->opts;
     const int64_t n = m->host.n;
-    rh_matrix* q = new rh_matrix();
-    q->m = m;
-    q->device = m->device;
-    q->prec = prec;
-    q->far32 = (prec == RH_MIXED) && o.fp64_integrals == 0;
-    q->elem_bytes = (prec == RH_FP64) ? 8 : 4;
-    q->n3 = 3 * n;
-    q->ld = std::max<int64_t>(256, (q->n3 + 255) / 256 * 256);
-    const size_t elem_rows = 3 * q->row_bytes();  // one element's 3 rows
+    if (q->planes) {
+        q->ldp = std::max<int64_t>(256, (n + 255) / 256 * 256);
+        q->ld = 3 * q->ldp;
+    } else {
+        q->ldp = 0;
+        q->ld = std::max<int64_t>(256, (q->n3 + 255) / 256 * 256);
+    }
+    const size_t elem_rows = q->elem_stride();  // one element row's storage
     const size_t total = size_t(n) * elem_rows;
     size_t budget = size_t(o.device_bytes);
     if (o.device_bytes <= 0) {
@@ -611,19 +647,15 @@ rh_status rh_matrix_assemble_ex(rh_model* m, rh_precision prec, const rh_matrix_
     if (o.resident_fraction >= 0.0) {
         res = std::min<int64_t>(n, int64_t(o.resident_fraction * double(n)));
     } else if (total > budget) {
-        if (!o.allow_streaming) {
-            delete q;
+        if (!o.allow_streaming)
             return fail(RH_ERR_NOMEM, "the " + std::to_string(total) + "-byte matrix exceeds the device budget of " +
                                           std::to_string(budget) + " bytes and streaming is not allowed");
-        }
         const size_t stage_bytes = size_t(2) << 26;  // two 64 MiB tiles
         res = (budget > stage_bytes) ? int64_t((budget - stage_bytes) / elem_rows) : 0;
         res = std::min(res, n);
     }
-    if (res < n && !o.allow_streaming && o.resident_fraction < 0.0) {
-        delete q;
+    if (res < n && !o.allow_streaming && o.resident_fraction < 0.0)
         return fail(RH_ERR_NOMEM, "matrix does not fit and streaming is not allowed");
-    }
     // the resident rows and every streamed tile start at a multiple of the
     // assembly's 64-row groups, so each block is integrated with the same
     // wave-uniform choices (FP32 or FP64 integrals) wherever its row lives
@@ -632,7 +664,6 @@ rh_status rh_matrix_assemble_ex(rh_model* m, rh_precision prec, const rh_matrix_
     q->res_elems = res;
     const size_t dev_bytes = size_t(res) * elem_rows;
     hipError_t e = hipMalloc(&q->d_Q, std::max<size_t>(dev_bytes, 16));
-    if (e == hipSuccess) e = dalloc(&q->d_qdiag, size_t(9 * n));
     // padded columns must read as zero in the matrix pass
     if (e == hipSuccess && dev_bytes) e = hipMemsetAsync(q->d_Q, 0, dev_bytes, m->stream);
     if (e == hipSuccess && res < n) {
@@ -655,14 +686,62 @@ rh_status rh_matrix_assemble_ex(rh_model* m, rh_precision prec, const rh_matrix_
         }
     }
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_HIP,
+                    std::string("matrix allocation (") + std::to_string(dev_bytes) + " device bytes, " +
+                        std::to_string(size_t(n - res) * elem_rows) + " pinned host bytes): " + hipGetErrorString(e));
+    return RH_OK;
+}
+
+// Assembles q; a 5-plane matrix that meets a traced block is stored again
+// with 6 planes.
+static rh_status assemble_into(rh_matrix* q)
+{
+    bool traced = false;
+    rh_status st = assemble_rows(q, &traced);
+    if (st != RH_OK || !traced) return st;
+    release_storage(q);
+    q->planes = 6;
+    st = alloc_storage(q);
+    if (st == RH_OK) st = assemble_rows(q, &traced);
+    return st;
+}
+
+rh_status rh_matrix_assemble_ex(rh_model* m, rh_precision prec, const rh_matrix_opts* opts,
+                                rh_matrix** out)
+{
+    if (!m || !out) return fail(RH_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (prec != RH_FP64 && prec != RH_MIXED) return fail(RH_ERR_ARG, "unknown precision");
+    rh_matrix_opts o{};
+    o.resident_fraction = -1.0;
+    if (opts) o = *opts;
+    if (o.resident_fraction > 1.0) return fail(RH_ERR_ARG, "resident_fraction above 1");
+    if (o.allow_streaming != 0 && o.allow_streaming != 1 && o.allow_streaming != RH_STREAM_BY_POLICY)
+        return fail(RH_ERR_ARG, "allow_streaming must be 0, 1 or RH_STREAM_BY_POLICY");
+    if (o.allow_streaming == RH_STREAM_BY_POLICY) o.allow_streaming = (g_fallback == RH_FALLBACK_BREAK) ? 0 : 1;
+    if (o.fp64_integrals != 0 && o.fp64_integrals != 1) return fail(RH_ERR_ARG, "fp64_integrals must be 0 or 1");
+    if (o.dense_rows != 0 && o.dense_rows != 1) return fail(RH_ERR_ARG, "dense_rows must be 0 or 1");
+    RH_HIP(enter_device(m->device));
+    const int64_t n = m->host.n;
+    rh_matrix* q = new rh_matrix();
+    q->m = m;
+    q->device = m->device;
+    q->prec = prec;
+    q->opts = o;
+    q->far32 = (prec == RH_MIXED) && o.fp64_integrals == 0;
+    q->elem_bytes = (prec == RH_FP64) ? 8 : 4;
+    q->n3 = 3 * n;
+    // lab-frame blocks (no transform copies, no row transforms) are symmetric
+    q->planes = (o.dense_rows == 0 && !m->host.sym && m->host.rowt_identity) ? 5 : 0;
+    hipError_t e = dalloc(&q->d_qdiag, size_t(9 * n));
+    if (e == hipSuccess) e = hipMalloc(&q->d_flag, sizeof(int32_t));
     if (e != hipSuccess) {
-        std::string msg = std::string("matrix allocation (") + std::to_string(dev_bytes) +
-                          " device bytes, " + std::to_string(size_t(n - res) * elem_rows) +
-                          " pinned host bytes): " + hipGetErrorString(e);
         free_matrix(q);
-        return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_HIP, msg);
+        return fail(e == hipErrorOutOfMemory ? RH_ERR_NOMEM : RH_ERR_HIP, hipGetErrorString(e));
     }
-    rh_status st = assemble_into(q);
+    rh_status st = alloc_storage(q);
+    if (st == RH_OK) st = assemble_into(q);
     if (st != RH_OK) {
         free_matrix(q);
         return st;
@@ -709,29 +788,56 @@ rh_status rh_matrix_device_ptr(const rh_matrix* q, void** dptr, int32_t* elem_by
     return RH_OK;
 }
 
+rh_status rh_matrix_format(const rh_matrix* q, int32_t* planes, int64_t* ld)
+{
+    if (!q || !planes || !ld) return fail(RH_ERR_ARG, "null argument");
+    *planes = q->planes;
+    *ld = q->planes ? q->ldp : q->ld;
+    return RH_OK;
+}
+
 rh_status rh_matrix_export(const rh_matrix* q, int64_t row0, int64_t nrows, double* out)
 {
     if (!q || !out) return fail(RH_ERR_ARG, "null argument");
     if (row0 < 0 || nrows < 0 || row0 + nrows > q->n3) return fail(RH_ERR_ARG, "row range out of bounds");
     rh_model* m = q->m;
     RH_HIP(enter_device(m->device));
-    const int64_t n3 = q->n3;
-    std::vector<char> buf(size_t(q->ld) * size_t(q->elem_bytes));
+    const int64_t n3 = q->n3, n = m->host.n;
+    // one element row's storage, read from the device or the pinned rows
+    std::vector<char> buf(q->elem_stride());
+    int64_t have = -1;
+    auto load = [&](int64_t e) -> hipError_t {
+        if (e == have) return hipSuccess;
+        have = e;
+        if (e < q->res_elems)
+            return hipMemcpy(buf.data(), static_cast<const char*>(q->d_Q) + size_t(e) * q->elem_stride(),
+                             buf.size(), hipMemcpyDeviceToHost);
+        std::memcpy(buf.data(), static_cast<const char*>(q->h_Q) + size_t(e - q->res_elems) * q->elem_stride(),
+                    buf.size());
+        return hipSuccess;
+    };
+    auto val = [&](int64_t k) -> double {
+        return (q->elem_bytes == 4) ? double(reinterpret_cast<const float*>(buf.data())[k])
+                                    : reinterpret_cast<const double*>(buf.data())[k];
+    };
+    // plane of entry (a, b) of a symmetric block
+    static const int kPlane[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
     for (int64_t r = row0; r < row0 + nrows; r++) {
-        if (r < 3 * q->res_elems) {
-            const char* src = static_cast<const char*>(q->d_Q) + size_t(r) * q->row_bytes();
-            RH_HIP(hipMemcpy(buf.data(), src, size_t(n3) * q->elem_bytes, hipMemcpyDeviceToHost));
-        } else {
-            const char* src = static_cast<const char*>(q->h_Q) + size_t(r - 3 * q->res_elems) * q->row_bytes();
-            std::memcpy(buf.data(), src, size_t(n3) * q->elem_bytes);
-        }
+        const int64_t e = r / 3, a = r % 3;
+        RH_HIP(load(e));
         double* o = out + (r - row0) * n3;
-        for (int64_t c = 0; c < n3; c++)
-            o[c] = (q->elem_bytes == 4) ? double(reinterpret_cast<float*>(buf.data())[c])
-                                        : reinterpret_cast<double*>(buf.data())[c];
+        if (!q->planes) {
+            for (int64_t c = 0; c < n3; c++) o[c] = val(a * q->ld + c);
+            continue;
+        }
+        for (int64_t j = 0; j < n; j++)
+            for (int b = 0; b < 3; b++) {
+                const int k = kPlane[3 * a + b];
+                o[3 * j + b] = (k < q->planes) ? val(k * q->ldp + j) : -(val(j) + val(3 * q->ldp + j));
+            }
     }
     // diagonal blocks are kept in FP64 beside the matrix
-    std::vector<double> qd(size_t(9 * m->host.n));
+    std::vector<double> qd(size_t(9 * n));
     RH_HIP(hipMemcpy(qd.data(), q->d_qdiag, qd.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int64_t r = row0; r < row0 + nrows; r++) {
         const int64_t e = r / 3, a = r % 3;
@@ -796,8 +902,8 @@ struct AdjointPass {
 };
 
 // Device-driven form of the forward iteration below: the matrix pass, the
-// relaxation step, the Anderson step and the update all read the loop state
-// on the device.  For a resident matrix the host enqueues iterations in
+// relaxation step (whose last block runs the Anderson step) and the update
+// all read the loop state on the device.  For a resident matrix the host enqueues iterations in
 // batches without waiting for any of them, copies the state back after each
 // batch and looks at it one batch late; iterations enqueued after the loop
 // finished return at once.  The batch after the first few is sized from the
@@ -839,13 +945,12 @@ rh_status relax_run_dev(rh_matrix* q, const double* d_mats, const double* d_tab,
         for (int j = 0; j < cnt; j++) {
             tm.begin();
             if (lagged)
-                RH_HIP(rh::launch_matvec(q->d_Q, q->elem_bytes, n3, q->ld, w.x, nullptr, w.y, s, &w.st->done));
+                RH_HIP(matvec_rows(q, q->d_Q, 0, n, w.x, w.y, s, &w.st->done));
             else
                 RH_HIP(matrix_pass(q, w.x, w.y, s));
             tm.end();
             RH_HIP(rh::launch_relax_step_dev(n, d_mats, d_tab, d_hext, q->d_qdiag, w.y, w.x, w.F, d_H, w.part, hb,
                                              w.st, s));
-            RH_HIP(rh::launch_anderson_dev(w.nb, w.part, w.st, s));
             RH_HIP(rh::launch_update_dev(n3, w.x, w.F, hb, d_M, w.st, s));
         }
         enq += cnt;
@@ -913,8 +1018,12 @@ rh_status relax_run(rh_matrix* q, const double* d_mats, const double* d_tab, con
         tm.begin();
         if (adj) {
             RH_HIP(rh::launch_adj_scale(n, d_mats, nullptr, w.x, adj->mu, s));
-            RH_HIP(rh::launch_matvec_t(q->d_Q, q->elem_bytes, n3, q->ld, adj->mu, adj->tpart,
-                                       adj->chunks, w.y, s));
+            if (q->planes)
+                RH_HIP(rh::launch_matvec_t_sym(q->d_Q, q->elem_bytes, q->planes, n, q->ldp, adj->mu, adj->tpart,
+                                               adj->chunks, w.y, s));
+            else
+                RH_HIP(rh::launch_matvec_t(q->d_Q, q->elem_bytes, n3, q->ld, adj->mu, adj->tpart,
+                                           adj->chunks, w.y, s));
         } else {
             RH_HIP(matrix_pass(q, w.x, w.y, s));
         }

@@ -56,16 +56,36 @@ __device__ __forceinline__ void store_out(T* p, double v) { *p = T(v); }
 // unstaged (FP64), 2.41 against 3.03 ms (FP32); wider tiles cost occupancy.
 // Sources whose columns are not consecutive (mixed kinds) are stored
 // directly.
+//
+// Symmetric-block layout (PK = 5 or 6): without transform copies and row
+// transforms every block is the symmetric tensor Q(x_i) of source j, so an
+// element row is stored as PK planes (xx, xy, xz, yy, yz[, zz]) of ld values,
+// plane k holding entry k of the blocks (i, 0 .. n-1); with PK = 5 the zz
+// entry is -(xx + yy), exact for a collocation point outside the source.  A
+// block with a trace beyond 1/4 (a collocation point inside another element)
+// sets *flag, and the caller stores the matrix with 6 planes instead.
 #ifndef RH_ASM_S64
 #define RH_ASM_S64 4
 #endif
 #ifndef RH_ASM_S32
 #define RH_ASM_S32 8
 #endif
-template <class OUT>
+// symmetric-block tiles: 8 (FP64) or 16 (FP32) sources, 64-byte plane
+// segments; on the 40,000-tetrahedron yoke 49.2 / 38.5 ms against 55.1 / 40.4
+// with 4 / 8 sources and 80.7 / 48.1 with 2 / 4
+#ifndef RH_ASM_P64
+#define RH_ASM_P64 8
+#endif
+#ifndef RH_ASM_P32
+#define RH_ASM_P32 16
+#endif
+template <class OUT, int PK>
 struct AsmTile {
-    static constexpr int S = (sizeof(OUT) == 8) ? RH_ASM_S64 : RH_ASM_S32;  // sources per tile
-    static constexpr int U = 3 * S * int(sizeof(OUT)) / 8;  // 8-byte units per row segment
+    static constexpr int RPE = PK ? PK : 3;  // stored rows per element row
+    static constexpr int VPS = PK ? 1 : 3;   // values per source in a stored row
+    static constexpr int S = PK ? ((sizeof(OUT) == 8) ? RH_ASM_P64 : RH_ASM_P32)
+                                : ((sizeof(OUT) == 8) ? RH_ASM_S64 : RH_ASM_S32);  // sources per tile
+    static constexpr int U = VPS * S * int(sizeof(OUT)) / 8;  // 8-byte units per row segment
     static constexpr int W = (U + 1) * 8 / int(sizeof(OUT));  // LDS row stride (elements)
 };
 
@@ -75,13 +95,17 @@ struct AsmTile {
 #define RH_ASM_ATTR
 #endif
 // FAR32: far blocks in FP32 arithmetic, near ones in FP64 (block_Q_mixed).
-template <class R, bool SYM, bool ROWT, class OUT, int KIND, bool FAR32 = false>
+// PK: 0 row-major 3x3 blocks, 5 or 6 the symmetric-block planes (then ld is
+// the plane stride and SYM, ROWT are false).
+template <class R, bool SYM, bool ROWT, class OUT, int KIND, bool FAR32 = false, int PK = 0>
 __global__ __launch_bounds__(64) RH_ASM_ATTR void k_assemble(DevModel m, OUT* __restrict__ Q, int64_t ld,
                                                  double* __restrict__ qdiag, int64_t jt, int64_t r0,
-                                                 int64_t r1)
+                                                 int64_t r1, int32_t* __restrict__ flag)
 {
-    constexpr int S = AsmTile<OUT>::S, W = AsmTile<OUT>::W;
-    __shared__ OUT tile[3 * 64 * W];
+    static_assert(PK == 0 || (!SYM && !ROWT), "the symmetric-block layout needs lab-frame blocks");
+    using TL = AsmTile<OUT, PK>;
+    constexpr int S = TL::S, W = TL::W, RPE = TL::RPE, VPS = TL::VPS;
+    __shared__ OUT tile[RPE * 64 * W];
     const int lane = threadIdx.x;
     const int64_t i0 = r0 + int64_t(blockIdx.x) * 64;
     const int64_t i = i0 + lane;
@@ -96,7 +120,8 @@ __global__ __launch_bounds__(64) RH_ASM_ATTR void k_assemble(DevModel m, OUT* __
     double rt[9];
     if (ROWT)
         for (int k = 0; k < 9; k++) rt[k] = m.rowt[9 * ic + k];
-    OUT* row = Q + (3 * (ic - r0)) * ld;
+    OUT* row = Q + (RPE * (ic - r0)) * ld;
+    bool traced = false;
     for (int64_t kb = k0; kb < k1; kb += S) {
         const int cnt = int((k1 - kb < S) ? (k1 - kb) : S);
         const int64_t jb = m.kidx[kb];
@@ -121,25 +146,40 @@ __global__ __launch_bounds__(64) RH_ASM_ATTR void k_assemble(DevModel m, OUT* __
                     q[k] = 0.0;
                 }
             }
+            // stored values: the 3x3 block row by row, or the planes
+            double v[RPE * VPS];
+            if constexpr (PK == 0) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) v[k] = q[k];
+            } else {
+                if (PK == 5) traced |= fabs(q[0] + q[4] + q[8]) > 0.25;
+                v[0] = q[0];
+                v[1] = 0.5 * (q[1] + q[3]);
+                v[2] = 0.5 * (q[2] + q[6]);
+                v[3] = q[4];
+                v[4] = 0.5 * (q[5] + q[7]);
+                if constexpr (PK == 6) v[5] = q[8];
+            }
             if (run) {
-                OUT* t = tile + 3 * lane * W + 3 * s;
+                OUT* t = tile + RPE * lane * W + VPS * s;
 #pragma unroll
-                for (int a = 0; a < 3; a++)
+                for (int a = 0; a < RPE; a++)
 #pragma unroll
-                    for (int b = 0; b < 3; b++) t[a * W + b] = OUT(q[3 * a + b]);
+                    for (int b = 0; b < VPS; b++) t[a * W + b] = OUT(v[VPS * a + b]);
             } else if (live) {
-                OUT* o = row + 3 * j;
+                OUT* o = row + VPS * j;
 #pragma unroll
-                for (int a = 0; a < 3; a++)
+                for (int a = 0; a < RPE; a++)
 #pragma unroll
-                    for (int b = 0; b < 3; b++) store_out(o + a * ld + b, q[3 * a + b]);
+                    for (int b = 0; b < VPS; b++) store_out(o + a * ld + b, v[VPS * a + b]);
             }
         }
         if (run) {
             __syncthreads();
-            OUT* base = Q + (3 * (i0 - r0)) * ld + 3 * jb;
-            constexpr int U = AsmTile<OUT>::U, RP = 64 / U;  // rows per pass
-            if (cnt == S && nrow == 64 && ((3 * jb * int64_t(sizeof(OUT))) & 7) == 0) {
+            OUT* base = Q + (RPE * (i0 - r0)) * ld + VPS * jb;
+            constexpr int U = TL::U, RP = 64 / U;  // rows per pass
+            constexpr int TR = RPE * 64;            // stored rows of a full tile
+            if (cnt == S && nrow == 64 && ((VPS * jb * int64_t(sizeof(OUT))) & 7) == 0) {
                 // full tile: 8-byte units, RP rows of U units per pass, offsets
                 // in 32 bits from a uniform base (the tile spans < 2^32 bytes)
                 const int lr = lane / U, lc = lane - U * lr;
@@ -149,17 +189,17 @@ __global__ __launch_bounds__(64) RH_ASM_ATTR void k_assemble(DevModel m, OUT* __
                     char* dst = reinterpret_cast<char*>(base);  // uniform, advanced per pass
                     const int64_t step = int64_t(RP) * ld * int64_t(sizeof(OUT));
                     const uint32_t off = uint32_t(lr) * uint32_t(ld * sizeof(OUT)) + uint32_t(8 * lc);
-                    constexpr int full = 192 / RP;  // passes with every lane's row in the tile
+                    constexpr int full = TR / RP;  // passes with every lane's row in the tile
 #pragma unroll 4
                     for (int it = 0; it < full; it++) {
                         *reinterpret_cast<uint64_t*>(dst + off) = src[RP * it * WU];
                         dst += step;
                     }
-                    if (RP * full + lr < 192) *reinterpret_cast<uint64_t*>(dst + off) = src[RP * full * WU];
+                    if (RP * full + lr < TR) *reinterpret_cast<uint64_t*>(dst + off) = src[RP * full * WU];
                 }
             } else {
-                // partial tile: the 3*nrow matrix rows, 3*cnt values each
-                const int cols = 3 * cnt, total = 3 * nrow * cols;
+                // partial tile: the RPE*nrow stored rows, VPS*cnt values each
+                const int cols = VPS * cnt, total = RPE * nrow * cols;
                 const int step_r = 64 / cols, step_c = 64 - step_r * cols;
                 int r = lane / cols, c = lane - r * cols;
                 for (int e = lane; e < total; e += 64) {
@@ -175,34 +215,37 @@ __global__ __launch_bounds__(64) RH_ASM_ATTR void k_assemble(DevModel m, OUT* __
             __syncthreads();
         }
     }
+    if (PK == 5 && live && traced) *flag = 1;
 }
 
-template <bool SYM, bool ROWT, class OUT>
+template <bool SYM, bool ROWT, class OUT, int PK = 0>
 hipError_t asm_dispatch(const DevModel& m, OUT* Q, int64_t ld, double* qdiag, int64_t r0, int64_t r1,
-                        hipStream_t s, bool far32)
+                        hipStream_t s, bool far32, int32_t* flag)
 {
     if (r1 <= r0) return hipSuccess;
     const int64_t gx = (r1 - r0 + 63) / 64;
     for (int slot = 0; slot < 3; slot++) {
         const int64_t cnt = m.koff[slot + 1] - m.koff[slot];
         if (cnt == 0) continue;
-        // enough workgroups to fill 256 CUs several times over
+        // enough workgroups to fill 256 CUs several times over; column tiles
+        // are whole LDS tiles
+        constexpr int S = AsmTile<OUT, PK>::S;
         const int64_t gy_target = std::max<int64_t>(1, (8192 + gx - 1) / gx);
         int64_t jt = std::max<int64_t>(4, (cnt + gy_target - 1) / gy_target);
-        jt = std::min<int64_t>(jt, 256);
+        jt = std::min<int64_t>((jt + S - 1) / S * S, 256);
         const dim3 grid{unsigned(gx), unsigned((cnt + jt - 1) / jt), 1u};
         // FP32 storage: far blocks in FP32 arithmetic (general polyhedra stay FP64)
         constexpr bool mixed = sizeof(OUT) == 4;
         if (slot == 0 && mixed && far32)
-            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 0, mixed>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1);
+            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 0, mixed, PK>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1, flag);
         else if (slot == 0)
-            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 0>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1);
+            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 0, false, PK>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1, flag);
         else if (slot == 1 && mixed && far32)
-            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 1, mixed>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1);
+            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 1, mixed, PK>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1, flag);
         else if (slot == 1)
-            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 1>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1);
+            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 1, false, PK>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1, flag);
         else
-            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 2>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1);
+            hipLaunchKernelGGL((k_assemble<double, SYM, ROWT, OUT, 2, false, PK>), grid, dim3(64), 0, s, m, Q, ld, qdiag, jt, r0, r1, flag);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -719,6 +762,135 @@ __global__ __launch_bounds__(256) void k_matvec_f64(const double* __restrict__ Q
     for (int r = 0; r < RW; r++) {
         const double v = wave_sum(acc[r]);
         if (lane == 0 && r0 + r < nrows) y[r0 + r] = v + (add ? add[r0 + r] : 0.0);
+    }
+}
+
+// Matrix pass over the symmetric-block layout: ER element rows per wave, their
+// P planes (xx, xy, xz, yy, yz[, zz]) streamed with 16-byte non-temporal loads
+// (two column elements per lane in FP64, four in FP32), the iterate read once
+// per column chunk as the interleaved (x, y, z) triples of those columns and
+// shared by the ER rows; FP64 accumulation.
+// element rows per wave: on the 40,000-tetrahedron yoke (5 planes) 4 rows
+// measured 9.57 ms FP64 / 4.87 ms FP32 per pass, 2 rows 9.75 / 5.04, 1 row
+// 10.70 / 6.18 (the iterate's reads per matrix byte fall as 1/rows); a
+// twofold unroll of the column loop gained nothing
+#ifndef RH_MVS_ER
+#define RH_MVS_ER 4
+#endif
+#ifndef RH_MVS_UNROLL
+#define RH_MVS_UNROLL 1
+#endif
+template <int P, int ER>
+__global__ __launch_bounds__(256) void k_matvec_sym_f64(const double* __restrict__ Q, int64_t nel,
+                                                        int64_t ldp, const double* __restrict__ x,
+                                                        const double* __restrict__ add,
+                                                        double* __restrict__ y,
+                                                        const int32_t* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t i0 = (int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * ER;
+    if (i0 >= nel) return;
+    const nd2* pl[ER][P];
+#pragma unroll
+    for (int r = 0; r < ER; r++) {
+        const int64_t i = (i0 + r < nel) ? (i0 + r) : i0;  // rows past the end repeat row i0, unstored
+#pragma unroll
+        for (int k = 0; k < P; k++) pl[r][k] = reinterpret_cast<const nd2*>(Q + (i * P + k) * ldp);
+    }
+    const nd2* x2 = reinterpret_cast<const nd2*>(x);
+    double h[ER][3];
+#pragma unroll
+    for (int r = 0; r < ER; r++) h[r][0] = h[r][1] = h[r][2] = 0.0;
+    const int64_t nv = ldp / 2;
+#pragma unroll RH_MVS_UNROLL
+    for (int64_t c = lane; c < nv; c += 64) {
+        nd2 q[ER][P];
+#pragma unroll
+        for (int r = 0; r < ER; r++)
+#pragma unroll
+            for (int k = 0; k < P; k++) q[r][k] = __builtin_nontemporal_load(pl[r][k] + c);
+        // columns 2c and 2c+1: (a.x, a.y, b.x) and (b.y, d.x, d.y)
+        const nd2 a = x2[3 * c], b = x2[3 * c + 1], d = x2[3 * c + 2];
+#pragma unroll
+        for (int r = 0; r < ER; r++) {
+            const nd2* v = q[r];
+            const nd2 zz = (P == 6) ? v[P - 1] : -(v[0] + v[3]);
+            h[r][0] += v[0].x * a.x + v[1].x * a.y + v[2].x * b.x + v[0].y * b.y + v[1].y * d.x + v[2].y * d.y;
+            h[r][1] += v[1].x * a.x + v[3].x * a.y + v[4].x * b.x + v[1].y * b.y + v[3].y * d.x + v[4].y * d.y;
+            h[r][2] += v[2].x * a.x + v[4].x * a.y + zz.x * b.x + v[2].y * b.y + v[4].y * d.x + zz.y * d.y;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < ER; r++) {
+        const double hx = wave_sum(h[r][0]), hy = wave_sum(h[r][1]), hz = wave_sum(h[r][2]);
+        const int64_t i = i0 + r;
+        if (lane == 0 && i < nel) {
+            y[3 * i] = hx + (add ? add[3 * i] : 0.0);
+            y[3 * i + 1] = hy + (add ? add[3 * i + 1] : 0.0);
+            y[3 * i + 2] = hz + (add ? add[3 * i + 2] : 0.0);
+        }
+    }
+}
+
+template <int P, int ER>
+__global__ __launch_bounds__(256) void k_matvec_sym_f32(const float* __restrict__ Q, int64_t nel,
+                                                        int64_t ldp, const double* __restrict__ x,
+                                                        const double* __restrict__ add,
+                                                        double* __restrict__ y,
+                                                        const int32_t* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t i0 = (int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6)) * ER;
+    if (i0 >= nel) return;
+    const nf4* pl[ER][P];
+#pragma unroll
+    for (int r = 0; r < ER; r++) {
+        const int64_t i = (i0 + r < nel) ? (i0 + r) : i0;  // rows past the end repeat row i0, unstored
+#pragma unroll
+        for (int k = 0; k < P; k++) pl[r][k] = reinterpret_cast<const nf4*>(Q + (i * P + k) * ldp);
+    }
+    const nd2* x2 = reinterpret_cast<const nd2*>(x);
+    double h[ER][3];
+#pragma unroll
+    for (int r = 0; r < ER; r++) h[r][0] = h[r][1] = h[r][2] = 0.0;
+    const int64_t nv = ldp / 4;
+#pragma unroll RH_MVS_UNROLL
+    for (int64_t c = lane; c < nv; c += 64) {
+        nf4 q[ER][P];
+#pragma unroll
+        for (int r = 0; r < ER; r++)
+#pragma unroll
+            for (int k = 0; k < P; k++) q[r][k] = __builtin_nontemporal_load(pl[r][k] + c);
+        // columns 4c .. 4c+3: twelve doubles of x
+        nd2 m[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) m[k] = x2[6 * c + k];
+        const double mv[12] = {m[0].x, m[0].y, m[1].x, m[1].y, m[2].x, m[2].y,
+                               m[3].x, m[3].y, m[4].x, m[4].y, m[5].x, m[5].y};
+#pragma unroll
+        for (int r = 0; r < ER; r++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const double xx = q[r][0][e], xy = q[r][1][e], xz = q[r][2][e], yy = q[r][3][e],
+                             yz = q[r][4][e];
+                const double zz = (P == 6) ? double(q[r][P - 1][e]) : -(xx + yy);
+                const double mx = mv[3 * e], my = mv[3 * e + 1], mz = mv[3 * e + 2];
+                h[r][0] += xx * mx + xy * my + xz * mz;
+                h[r][1] += xy * mx + yy * my + yz * mz;
+                h[r][2] += xz * mx + yz * my + zz * mz;
+            }
+    }
+#pragma unroll
+    for (int r = 0; r < ER; r++) {
+        const double hx = wave_sum(h[r][0]), hy = wave_sum(h[r][1]), hz = wave_sum(h[r][2]);
+        const int64_t i = i0 + r;
+        if (lane == 0 && i < nel) {
+            y[3 * i] = hx + (add ? add[3 * i] : 0.0);
+            y[3 * i + 1] = hy + (add ? add[3 * i + 1] : 0.0);
+            y[3 * i + 2] = hz + (add ? add[3 * i + 2] : 0.0);
+        }
     }
 }
 
@@ -1332,67 +1504,36 @@ __global__ __launch_bounds__(kRelaxLanes) void k_relax_step(int64_t n, const dou
                      hp.nh);
 }
 
-// The same step in the device-driven loop: history slots from the state.
-__global__ __launch_bounds__(kRelaxLanes) void k_relax_step_dev(int64_t n, const double* __restrict__ mats,
-                                                                const double* __restrict__ tab,
-                                                                const double* __restrict__ hext,
-                                                                const double* __restrict__ qdiag,
-                                                                const double* __restrict__ y,
-                                                                const double* __restrict__ M,
-                                                                double* __restrict__ F,
-                                                                double* __restrict__ Hout,
-                                                                double* __restrict__ partials, HistBufs hb,
-                                                                const RelaxDevState* __restrict__ st)
+// The Anderson step of the device-driven loop on one wave: the step partials
+// reduced across the wave in a fixed order, then on lane 0 the residual and
+// convergence test, the restart safeguard, the Gram matrix of the differences
+// in use (kept oldest first, the newest row from this step), the regularized
+// Cholesky solve of the least-squares problem and the update coefficients,
+// and the history extension for the next iteration.  The small arrays are
+// unrolled over kHistMax with the unused part padded by the identity, so they
+// stay in registers; the arithmetic on the live part is the host-driven
+// loop's.
+__device__ void anderson_wave(int64_t nb, const double* __restrict__ partials, RelaxDevState* __restrict__ st)
 {
-    if (st->done) return;
-    const int nh = st->hist, head = st->head;
-    const double* dF[kHistMax];
+    const int lane = threadIdx.x & 63;
+    double v[kRelaxPart];
 #pragma unroll
-    for (int k = 0; k < kHistMax; k++) dF[k] = hb.dF[(head - nh + k + 2 * kHistMax) % kHistMax];
-    const int w = (head - 1 + kHistMax) % kHistMax;
-    relax_step_block(n, mats, tab, hext, qdiag, y, M, F, Hout, partials, hb.Fp, hb.xp,
-                     st->ext ? hb.dF[w] : nullptr, st->ext ? hb.dX[w] : nullptr, dF, nh);
-}
-
-// The Anderson step of the device-driven loop (one workgroup of 256 lanes):
-// the step partials reduced in a fixed order, then on lane 0 the residual
-// and convergence test, the restart safeguard, the history Gram matrix, the
-// regularized Cholesky solve of the least-squares problem and the update
-// coefficients, and the history extension for the next iteration.  The same
-// arithmetic as the host-driven loop.
-__global__ __launch_bounds__(256) void k_anderson_dev(int64_t nb, const double* __restrict__ partials,
-                                                      RelaxDevState* __restrict__ st)
-{
-    if (st->done) return;
-    __shared__ double red[256][kRelaxPart];
-    double acc[kRelaxPart];
-#pragma unroll
-    for (int k = 0; k < kRelaxPart; k++) acc[k] = 0.0;
-    for (int64_t b = threadIdx.x; b < nb; b += 256) {
+    for (int k = 0; k < kRelaxPart; k++) v[k] = 0.0;
+    for (int64_t b = lane; b < nb; b += 64) {
         const double* o = partials + int64_t(kRelaxPart) * b;
-        acc[0] += o[0];
-        acc[1] = fmax(acc[1], o[1]);
-        acc[2] = fmax(acc[2], o[2]);
+        v[0] += o[0];
+        v[1] = fmax(v[1], o[1]);
+        v[2] = fmax(v[2], o[2]);
 #pragma unroll
-        for (int k = 3; k < kRelaxPart; k++) acc[k] += o[k];
+        for (int k = 3; k < kRelaxPart; k++) v[k] += o[k];
     }
+    v[0] = wave_sum(v[0]);
+    v[1] = wave_max(v[1]);
+    v[2] = wave_max(v[2]);
 #pragma unroll
-    for (int k = 0; k < kRelaxPart; k++) red[threadIdx.x][k] = acc[k];
-    __syncthreads();
-    for (int stp = 128; stp > 0; stp >>= 1) {
-        if (int(threadIdx.x) < stp) {
-            red[threadIdx.x][0] += red[threadIdx.x + stp][0];
-            red[threadIdx.x][1] = fmax(red[threadIdx.x][1], red[threadIdx.x + stp][1]);
-            red[threadIdx.x][2] = fmax(red[threadIdx.x][2], red[threadIdx.x + stp][2]);
-            for (int k = 3; k < kRelaxPart; k++) red[threadIdx.x][k] += red[threadIdx.x + stp][k];
-        }
-        __syncthreads();
-    }
-    // lane 0 alone from here; its small arrays live in LDS
-    __shared__ double A[kHistMax * kHistMax], L[kHistMax * kHistMax], z[kHistMax], gam[kHistMax];
-    __shared__ int slot[kHistMax];
-    if (threadIdx.x != 0) return;
-    const double* v = red[0];
+    for (int k = 3; k < kRelaxPart; k++) v[k] = wave_sum(v[k]);
+    if (lane != 0) return;
+    st->arrive = 0;  // every block of this step has arrived
     const int hist = st->hist;
     const int head = st->head;
     st->it += 1;
@@ -1413,7 +1554,7 @@ __global__ __launch_bounds__(256) void k_anderson_dev(int64_t nb, const double* 
         st->done = 1;
         return;
     }
-    for (int k = 0; k < hist; k++) slot[k] = (head - hist + k + 2 * kHistMax) % kHistMax;
+    const int r0 = (head - hist + 2 * kHistMax) % kHistMax;  // buffer slot of the oldest difference
     bool plain = (hist == 0);
     int nh = hist, hd = head;
     if (misfit > 1e4 * st->best) {
@@ -1424,45 +1565,72 @@ __global__ __launch_bounds__(256) void k_anderson_dev(int64_t nb, const double* 
     }
     st->best = fmin(st->best, misfit);
     const double beta = st->beta;
+    double gam[kHistMax];
+#pragma unroll
+    for (int a = 0; a < kHistMax; a++) gam[a] = 0.0;
     if (!plain) {
-        const int newest = slot[hist - 1];
-        for (int k = 0; k < hist; k++) {
-            st->G[newest * kHistMax + slot[k]] = v[3 + k];
-            st->G[slot[k] * kHistMax + newest] = v[3 + k];
-        }
-        // (G + lambda I) gam = r, G the Gram matrix of the differences in use
+        // Gram matrix, oldest first; row and column hist-1 are the newest
+        double A[kHistMax][kHistMax];
+#pragma unroll
+        for (int a = 0; a < kHistMax; a++)
+#pragma unroll
+            for (int b = 0; b < kHistMax; b++) A[a][b] = st->G[a * kHistMax + b];
+#pragma unroll
+        for (int a = 0; a < kHistMax; a++)
+#pragma unroll
+            for (int k = 0; k < kHistMax; k++)
+                if (a == hist - 1 && k < hist) {
+                    A[a][k] = v[3 + k];
+                    A[k][a] = v[3 + k];
+                }
+        // (A + lambda I) gam = r on the live part, identity beyond it
         double tr = 0.0;
-        for (int a = 0; a < hist; a++)
-            for (int b = 0; b < hist; b++) A[a * kHistMax + b] = st->G[slot[a] * kHistMax + slot[b]];
-        for (int a = 0; a < hist; a++) tr += A[a * kHistMax + a];
+#pragma unroll
+        for (int a = 0; a < kHistMax; a++)
+            if (a < hist) tr += A[a][a];
         const double lam = 1e-10 * (tr > 0 ? tr / hist : 1.0);
-        for (int a = 0; a < hist; a++) A[a * kHistMax + a] += lam;
+        double L[kHistMax][kHistMax];
         bool ok = true;
-        for (int a = 0; a < hist && ok; a++)
+#pragma unroll
+        for (int a = 0; a < kHistMax; a++)
+#pragma unroll
             for (int b = 0; b <= a; b++) {
-                double sm = A[a * kHistMax + b];
-                for (int k = 0; k < b; k++) sm -= L[a * kHistMax + k] * L[b * kHistMax + k];
+                const bool live = a < hist && b < hist;
+                double sm = live ? A[a][b] + ((a == b) ? lam : 0.0) : ((a == b) ? 1.0 : 0.0);
+#pragma unroll
+                for (int k = 0; k < b; k++) sm -= L[a][k] * L[b][k];
                 if (a == b) {
-                    if (!(sm > 0.0)) {
-                        ok = false;
-                        break;
-                    }
-                    L[a * kHistMax + a] = sqrt(sm);
+                    ok = ok && (sm > 0.0);
+                    L[a][a] = sqrt(sm);
                 } else {
-                    L[a * kHistMax + b] = sm / L[b * kHistMax + b];
+                    L[a][b] = sm / L[b][b];
                 }
             }
         if (ok) {
-            for (int a = 0; a < hist; a++) {
-                double sm = v[3 + kHistMax + a];
-                for (int k = 0; k < a; k++) sm -= L[a * kHistMax + k] * z[k];
-                z[a] = sm / L[a * kHistMax + a];
+            double z[kHistMax];
+#pragma unroll
+            for (int a = 0; a < kHistMax; a++) {
+                double sm = (a < hist) ? v[3 + kHistMax + a] : 0.0;
+#pragma unroll
+                for (int k = 0; k < a; k++) sm -= L[a][k] * z[k];
+                z[a] = sm / L[a][a];
             }
-            for (int a = hist - 1; a >= 0; a--) {
+#pragma unroll
+            for (int a = kHistMax - 1; a >= 0; a--) {
                 double sm = z[a];
-                for (int k = a + 1; k < hist; k++) sm -= L[k * kHistMax + a] * gam[k];
-                gam[a] = sm / L[a * kHistMax + a];
+#pragma unroll
+                for (int k = a + 1; k < kHistMax; k++) sm -= L[k][a] * gam[k];
+                gam[a] = sm / L[a][a];
             }
+            // the Gram matrix of the next iteration: the oldest difference
+            // drops out when the history is full
+            const bool shift = (nh == kHistMax);
+#pragma unroll
+            for (int a = 0; a < kHistMax; a++)
+#pragma unroll
+                for (int b = 0; b < kHistMax; b++)
+                    st->G[a * kHistMax + b] =
+                        shift ? ((a + 1 < kHistMax && b + 1 < kHistMax) ? A[a + 1][b + 1] : 0.0) : A[a][b];
         } else {
             nh = 0;
             hd = 0;
@@ -1470,20 +1638,57 @@ __global__ __launch_bounds__(256) void k_anderson_dev(int64_t nb, const double* 
         }
     }
     st->cF = beta;
+#pragma unroll
     for (int b = 0; b < kHistMax; b++) {
         st->cdX[b] = 0.0;
         st->cdF[b] = 0.0;
     }
     if (!plain) {
-        for (int k = 0; k < hist; k++) {
-            st->cdX[slot[k]] = -gam[k];
-            st->cdF[slot[k]] = -beta * gam[k];
-        }
+#pragma unroll
+        for (int k = 0; k < kHistMax; k++)
+            if (k < hist) {
+                const int p = (r0 + k) % kHistMax;
+                st->cdX[p] = -gam[k];
+                st->cdF[p] = -beta * gam[k];
+            }
     }
     // the next iteration writes a new difference into slot hd
     st->ext = 1;
     st->head = (hd + 1) % kHistMax;
     st->hist = nh + 1 < kHistMax ? nh + 1 : kHistMax;
+}
+
+// The same step in the device-driven loop (history slots from the state); the
+// last block to finish runs the Anderson step on the partials of all blocks.
+__global__ __launch_bounds__(kRelaxLanes) void k_relax_step_dev(int64_t n, const double* __restrict__ mats,
+                                                                const double* __restrict__ tab,
+                                                                const double* __restrict__ hext,
+                                                                const double* __restrict__ qdiag,
+                                                                const double* __restrict__ y,
+                                                                const double* __restrict__ M,
+                                                                double* __restrict__ F,
+                                                                double* __restrict__ Hout,
+                                                                double* __restrict__ partials, HistBufs hb,
+                                                                RelaxDevState* __restrict__ st)
+{
+    if (st->done) return;
+    const int nh = st->hist, head = st->head;
+    const double* dF[kHistMax];
+#pragma unroll
+    for (int k = 0; k < kHistMax; k++) dF[k] = hb.dF[(head - nh + k + 2 * kHistMax) % kHistMax];
+    const int w = (head - 1 + kHistMax) % kHistMax;
+    relax_step_block(n, mats, tab, hext, qdiag, y, M, F, Hout, partials, hb.Fp, hb.xp,
+                     st->ext ? hb.dF[w] : nullptr, st->ext ? hb.dX[w] : nullptr, dF, nh);
+    // release this block's partials, then count it in
+    int arrived = 0;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        arrived = atomicAdd(&st->arrive, 1);
+    }
+    arrived = __shfl(arrived, 0, 64);
+    if (arrived != int(gridDim.x) - 1) return;
+    __threadfence();  // the other blocks' partials are visible to the last one
+    anderson_wave(int64_t(gridDim.x), partials, st);
 }
 
 // The update of the device-driven loop (grid-stride over the 3n components).
@@ -1534,19 +1739,25 @@ __global__ __launch_bounds__(256) void k_current(int64_t ns, const int32_t* __re
 
 // ---------------------------------------------------------------- launchers
 hipError_t launch_assemble(const DevModel& m, void* Q, int elem_bytes, int64_t ld, double* qdiag,
-                           int64_t r0, int64_t r1, hipStream_t s, bool far32)
+                           int64_t r0, int64_t r1, hipStream_t s, bool far32, int planes, int32_t* flag)
 {
     if (m.n == 0 || r1 <= r0) return hipSuccess;
+    if (planes != 0 && (m.sym || !m.rowt_identity || (planes != 5 && planes != 6) || (planes == 5 && !flag)))
+        return hipErrorInvalidValue;
     if (elem_bytes == 4) {
         float* q = static_cast<float*>(Q);
-        if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, r0, r1, s, far32);
-        if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, r0, r1, s, far32);
-        return asm_dispatch<true, true>(m, q, ld, qdiag, r0, r1, s, far32);
+        if (planes == 5) return asm_dispatch<false, false, float, 5>(m, q, ld, qdiag, r0, r1, s, far32, flag);
+        if (planes == 6) return asm_dispatch<false, false, float, 6>(m, q, ld, qdiag, r0, r1, s, far32, flag);
+        if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, r0, r1, s, far32, flag);
+        if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, r0, r1, s, far32, flag);
+        return asm_dispatch<true, true>(m, q, ld, qdiag, r0, r1, s, far32, flag);
     }
     double* q = static_cast<double*>(Q);
-    if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, r0, r1, s, false);
-    if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, r0, r1, s, false);
-    return asm_dispatch<true, true>(m, q, ld, qdiag, r0, r1, s, false);
+    if (planes == 5) return asm_dispatch<false, false, double, 5>(m, q, ld, qdiag, r0, r1, s, false, flag);
+    if (planes == 6) return asm_dispatch<false, false, double, 6>(m, q, ld, qdiag, r0, r1, s, false, flag);
+    if (!m.sym && m.rowt_identity) return asm_dispatch<false, false>(m, q, ld, qdiag, r0, r1, s, false, flag);
+    if (m.rowt_identity) return asm_dispatch<true, false>(m, q, ld, qdiag, r0, r1, s, false, flag);
+    return asm_dispatch<true, true>(m, q, ld, qdiag, r0, r1, s, false, flag);
 }
 
 namespace {
@@ -1699,6 +1910,31 @@ hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t l
     return hipGetLastError();
 }
 
+hipError_t launch_matvec_sym(const void* Q, int elem_bytes, int planes, int64_t nel, int64_t ldp,
+                             const double* x, const double* add, double* y, hipStream_t s,
+                             const int32_t* skip)
+{
+    if (nel <= 0) return hipSuccess;
+    if ((planes != 5 && planes != 6) || ldp % 256 != 0) return hipErrorInvalidValue;
+    constexpr int ER = RH_MVS_ER;
+    const int64_t waves = (nel + ER - 1) / ER;
+    const dim3 grid{unsigned((waves + 3) / 4), 1u, 1u};
+    if (elem_bytes == 4) {
+        const float* q = static_cast<const float*>(Q);
+        if (planes == 5)
+            hipLaunchKernelGGL((k_matvec_sym_f32<5, ER>), grid, dim3(256), 0, s, q, nel, ldp, x, add, y, skip);
+        else
+            hipLaunchKernelGGL((k_matvec_sym_f32<6, ER>), grid, dim3(256), 0, s, q, nel, ldp, x, add, y, skip);
+    } else {
+        const double* q = static_cast<const double*>(Q);
+        if (planes == 5)
+            hipLaunchKernelGGL((k_matvec_sym_f64<5, ER>), grid, dim3(256), 0, s, q, nel, ldp, x, add, y, skip);
+        else
+            hipLaunchKernelGGL((k_matvec_sym_f64<6, ER>), grid, dim3(256), 0, s, q, nel, ldp, x, add, y, skip);
+    }
+    return hipGetLastError();
+}
+
 int64_t relax_blocks(int64_t n) { return (n + kRelaxLanes - 1) / kRelaxLanes; }
 
 hipError_t launch_relax_update(int64_t n, const double* mats, const double* tab, const double* hext,
@@ -1722,17 +1958,11 @@ hipError_t launch_relax_step(int64_t n, const double* mats, const double* tab, c
 
 hipError_t launch_relax_step_dev(int64_t n, const double* mats, const double* tab, const double* hext,
                                  const double* qdiag, const double* y, const double* M, double* F,
-                                 double* Hout, double* partials, const HistBufs& hb, const RelaxDevState* st,
+                                 double* Hout, double* partials, const HistBufs& hb, RelaxDevState* st,
                                  hipStream_t s)
 {
     hipLaunchKernelGGL(k_relax_step_dev, dim3(unsigned(relax_blocks(n))), dim3(kRelaxLanes), 0, s, n, mats, tab,
                        hext, qdiag, y, M, F, Hout, partials, hb, st);
-    return hipGetLastError();
-}
-
-hipError_t launch_anderson_dev(int64_t nb, const double* partials, RelaxDevState* st, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_anderson_dev, dim3(1), dim3(256), 0, s, nb, partials, st);
     return hipGetLastError();
 }
 

@@ -59,8 +59,16 @@ struct DevModel {
 // far32 (FP32 storage only): blocks and tetrahedra far from the collocation
 // point (beyond kNearRadii circumradii) are integrated in FP32, near ones and
 // general polyhedra in FP64.
+//
+// planes = 5 or 6 (models without transform copies or row transforms only):
+// the symmetric-block layout instead, element row i as planes (xx, xy, xz,
+// yy, yz[, zz]) of ld values from Q + planes*(i - r0)*ld, plane k holding
+// entry k of the blocks (i, 0 .. n-1).  With 5 planes zz = -(xx + yy), and a
+// block whose trace exceeds 1/4 in magnitude (a collocation point inside
+// another element) sets *flag to 1: the caller must then use 6 planes.
 hipError_t launch_assemble(const DevModel& m, void* Q, int elem_bytes, int64_t ld, double* qdiag,
-                           int64_t r0, int64_t r1, hipStream_t s, bool far32 = false);
+                           int64_t r0, int64_t r1, hipStream_t s, bool far32 = false, int planes = 0,
+                           int32_t* flag = nullptr);
 
 // Number of partial-sum slices the field kernels use for np points (>= 1);
 // the work buffer needs 6 * np * slices doubles when slices > 1.  work_cap >= 0
@@ -81,6 +89,11 @@ hipError_t launch_field(const DevModel& m, const double* M, const double* pts, i
 // when *skip is non-zero (a device-driven loop that has finished).
 hipError_t launch_matvec(const void* Q, int elem_bytes, int64_t nrows, int64_t ld, const double* x,
                          const double* add, double* y, hipStream_t s, const int32_t* skip = nullptr);
+// y = Q x (+ add) over nel element rows of the symmetric-block layout (planes
+// 5 or 6, plane stride ldp a multiple of 256, x padded with zeros to 3*ldp).
+hipError_t launch_matvec_sym(const void* Q, int elem_bytes, int planes, int64_t nel, int64_t ldp,
+                             const double* x, const double* add, double* y, hipStream_t s,
+                             const int32_t* skip = nullptr);
 
 // Per-element implicit update: proposal M_prop = M_mat(H) with
 // H = y + hext + Qii M_prop solved locally; F = M_prop - M.  partials gets
@@ -115,10 +128,11 @@ hipError_t launch_relax_step(int64_t n, const double* mats, const double* tab, c
 // ------------------------------------------------------------ device-driven loop
 // State of the Anderson relaxation loop kept on the device, so that the host
 // enqueues iterations without waiting for each one: a pass of the matrix, the
-// relaxation step, the Anderson step (k_anderson_dev: residual, convergence
-// test, history Gram matrix, small least-squares solve, next coefficients)
-// and the update.  Once the Anderson step finds the loop finished it sets
-// done, and the kernels of iterations enqueued after that return at once.
+// relaxation step with the Anderson step in its last block (residual,
+// convergence test, history Gram matrix, small least-squares solve, next
+// coefficients) and the update.  Once the Anderson step finds the loop
+// finished it sets done, and the kernels of iterations enqueued after that
+// return at once.
 struct RelaxDevState {
     int32_t it;         // iterations completed
     int32_t done;       // 0 running; 1 finished, the update writes M = x + F; 2 finished, M kept
@@ -128,8 +142,10 @@ struct RelaxDevState {
     int32_t ext;        // the current iteration writes a difference into slot head - 1
     int32_t max_iter;
     int32_t keep_if_converged;  // a first iterate that meets prec is kept as it is
+    int32_t arrive;             // blocks of the current relaxation step that have finished
+    int32_t pad_;
     double prec, beta, best, misfit, maxM, maxH, n;
-    double G[kHistMax * kHistMax];  // dF_a . dF_b by physical slot
+    double G[kHistMax * kHistMax];  // dF_a . dF_b of the differences in use, oldest first
     double cF;                      // update: x += cF F + sum_b cdX[b] dX[b] + cdF[b] dF[b]
     double cdX[kHistMax], cdF[kHistMax];
 };
@@ -139,13 +155,14 @@ struct HistBufs {
     double* dF[kHistMax];
     double* dX[kHistMax];
 };
-// relaxation step of the device-driven loop (slots and history from st)
+// relaxation step of the device-driven loop (slots and history from st); the
+// last of its blocks to finish runs the Anderson step (reduction of the nb
+// block partials, residual, convergence test, history Gram matrix, small
+// least-squares solve, next coefficients) and advances st.
 hipError_t launch_relax_step_dev(int64_t n, const double* mats, const double* tab, const double* hext,
                                  const double* qdiag, const double* y, const double* M, double* F,
-                                 double* Hout, double* partials, const HistBufs& hb, const RelaxDevState* st,
+                                 double* Hout, double* partials, const HistBufs& hb, RelaxDevState* st,
                                  hipStream_t s);
-// Anderson step: one workgroup reduces the nb step partials and advances st.
-hipError_t launch_anderson_dev(int64_t nb, const double* partials, RelaxDevState* st, hipStream_t s);
 // x += cF F + sum_b (cdX[b] dX[b] + cdF[b] dF[b]); when st->done == 1, M_out = x + F.
 hipError_t launch_update_dev(int64_t n3, double* x, const double* F, const HistBufs& hb, double* M_out,
                              const RelaxDevState* st, hipStream_t s);
@@ -216,6 +233,10 @@ hipError_t launch_adj_geom(const DevModel& m, const double* adj, const int64_t* 
 int64_t matvec_t_chunks(int64_t nrows, int64_t ld);
 hipError_t launch_matvec_t(const void* Q, int elem_bytes, int64_t nrows, int64_t ld, const double* x,
                            double* part, int64_t chunks, double* y, hipStream_t s);
+// The same over nel element rows of the symmetric-block layout (part: chunks *
+// 3*ldp doubles).
+hipError_t launch_matvec_t_sym(const void* Q, int elem_bytes, int planes, int64_t nel, int64_t ldp,
+                               const double* x, double* part, int64_t chunks, double* y, hipStream_t s);
 // mu_i = rowT_i^T K_i^T lam_i (rowt may be null).
 hipError_t launch_adj_scale(int64_t n, const double* mats, const double* rowt, const double* lam,
                             double* mu, hipStream_t s);
