@@ -1010,9 +1010,12 @@ __device__ __forceinline__ void law_f_df(const double* mat, const double* tab, d
         const double ms = mat[2 + t], ks = mat[5 + t];
         if (ms == 0.0) continue;
         const double u = ks * h / ms;
-        f += ms * tanh(u);
-        const double c = cosh(u);
-        df += ks / (c * c);
+        // tanh u and sech^2 u from one expm1: with e = exp(-2|u|) - 1,
+        // tanh|u| = -e / (2 + e) and sech^2 u = 4 (1 + e) / (2 + e)^2
+        const double e = expm1(-2.0 * fabs(u));
+        const double d = 2.0 + e;
+        f += ms * copysign(-e / d, u);
+        df += ks * (4.0 * (1.0 + e) / (d * d));
     }
 }
 
