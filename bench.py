@@ -180,11 +180,18 @@ def run_large_yoke(ctx, args, cpu):
         # field of the relaxed state at sampled gap points (for the errors)
         N.check(ctx.L.rh_field_dev(model.h, _vp(d_M), _vp(d_pts), len(pts), _vp(d_B), None, prec))
         ctx.sync()
+        B = d_B.cpu().numpy()
+        B32 = None
+        if prec == N.RH_FP64:
+            # the FP32 field integrals on the same (FP64) magnetization
+            N.check(ctx.L.rh_field_dev(model.h, _vp(d_M), _vp(d_pts), len(pts), _vp(d_B), None, N.RH_MIXED))
+            ctx.sync()
+            B32 = d_B.cpu().numpy()
         r = dict(ms_per_step=1e3 * wall / K, asm_ms=1e3 * phase["asm"] / K,
                  solve_s=phase["solve"] / K, iters=res.iters, misfit=res.misfit,
                  converged=bool(res.converged), refined=res.refined,
                  kernel_ms={f: kt[f][0] for f in kt}, kernel_counts={f: kt[f][1] for f in kt},
-                 M=d_M.cpu().numpy(), B=d_B.cpu().numpy(), planes=planes, mld=mld)
+                 M=d_M.cpu().numpy(), B=B, B32=B32, planes=planes, mld=mld)
         Q.close()
         out[label] = r
         log("large yoke %s: %.1f ms/step (asm %.1f ms, solve %.3f s in %d it, misfit %.2e)"
@@ -218,6 +225,13 @@ def run_large_yoke(ctx, args, cpu):
         sec["mixed"]["speedup_vs_fp64"] = r64["ms_per_step"] / rmx["ms_per_step"]
         sec["mixed"]["M_err_vs_fp64"] = float(np.abs(rmx["M"] - r64["M"]).max() / np.abs(r64["M"]).max())
         sec["mixed"]["B_err_vs_fp64"] = float(np.abs(rmx["B"] - r64["B"]).max() / np.abs(r64["B"]).max())
+        sec["mixed"]["fp32_field_err_same_M"] = float(np.abs(r64["B32"] - r64["B"]).max() / np.abs(r64["B"]).max())
+        sec["mixed"]["error_note"] = (
+            "M_err and B_err compare two solves that each stop at misfit <= prec (%.0e) on different paths: "
+            "at chi ~ 2000 the stopping tolerance, not the precision, sets their difference (the GPU tests "
+            "take both on to prec 1e-9, where the refined mixed state's gap field is the FP64 one to 1e-6); "
+            "fp32_field_err_same_M is the FP32 field integrals against the FP64 ones on one magnetization"
+            % args.prec_large)
     if cpu and "fp64" in out:
         from oracle import oracle as OR
         OR.set_randomization(False)
@@ -228,8 +242,10 @@ def run_large_yoke(ctx, args, cpu):
             sec["mixed"]["B_err_vs_oracle_sampled"] = float(np.abs(out["mixed"]["B"] - Bo).max()
                                                             / np.abs(Bo).max())
         sec["error_note"] = ("B of the relaxed state at %d gap points vs the oracle's field of the same "
-                             "FP64 magnetization; the oracle restates the reference's forms, which are "
-                             "off by 1.7e-7 of this field (pinned by 50-digit sums in the tests)" % len(pts))
+                             "FP64 magnetization; the oracle restates the reference's forms, which lose "
+                             "accuracy on this mesh's 14 tetrahedra with a face within 1e-7 of -z and on "
+                             "jittered faces (the GPU tests pin those elements and the whole model at 50 "
+                             "digits: the device FP64 field is within 1e-14 of the 50-digit sums)" % len(pts))
     return sec, out, P, mats
 
 

@@ -143,8 +143,7 @@ void dfree(void* p)
 // iteration on M = G(M), G = per-element implicit material update given the
 // field of all other elements (the fixed point of radGPU_RelaxAuto,
 // cpp/src/core/radgpurlx.cu:799-1349, with an acceleration of our own).
-constexpr int kHist = 8;
-static_assert(kHist == rh::kHistMax, "the relaxation step holds kHistMax differences");
+constexpr int kHist = rh::kHistMax;
 
 // Work arrays of the relaxation: kept with the matrix and reused by every
 // solve on it (allocated by the first one).
@@ -203,13 +202,13 @@ struct RelaxWork {
             if (e == hipSuccess) e = dalloc(&dF[k], size_t(n3));
         }
         if (e == hipSuccess) e = dalloc(&part, size_t(rh::kRelaxPart * nb));
-        if (e == hipSuccess) e = dalloc(&dots, size_t(17 * db));
+        if (e == hipSuccess) e = dalloc(&dots, size_t(rh::kDotMax * db));
         if (e == hipSuccess) e = dalloc(&st, 1);
         if (e == hipSuccess) e = dalloc(&H64, size_t(n3));
         if (e == hipSuccess) e = dalloc(&hx, size_t(n3));
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&hst), 3 * sizeof(rh::RelaxDevState));
         if (e == hipSuccess)
-            e = hipHostMalloc(reinterpret_cast<void**>(&hbuf), size_t(rh::kRelaxPart * nb + 17 * db) * sizeof(double));
+            e = hipHostMalloc(reinterpret_cast<void**>(&hbuf), size_t(rh::kRelaxPart * nb + rh::kDotMax * db) * sizeof(double));
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
         return e;
@@ -1052,8 +1051,8 @@ rh_status relax_run(rh_matrix* q, const double* d_mats, const double* d_tab, con
             for (int k = 0; k < hist; k++) slot[k] = (head - hist + k + kHist) % kHist;
             int nd = 0;
             if (hist > 0) {
-                const double* a[17];
-                const double* b[17];
+                const double* a[rh::kDotMax];
+                const double* b[rh::kDotMax];
                 const int newest = slot[hist - 1];
                 for (int k = 0; k < hist; k++) {
                     a[nd] = w.dF[newest];
@@ -1158,8 +1157,8 @@ rh_status relax_run(rh_matrix* q, const double* d_mats, const double* d_tab, con
             continue;
         }
         // x <- x + beta F - sum_k gam_k (dX_k + beta dF_k)
-        const double* v[34];
-        double c[34];
+        const double* v[rh::kCombMax];
+        double c[rh::kCombMax];
         int nv = 0;
         v[nv] = w.x;
         c[nv++] = 1.0;
@@ -1891,7 +1890,7 @@ rh_status rh_relax_adjoint_dev(rh_matrix* q, const double* d_mats, const double*
     const int64_t nb = rh::relax_blocks(n);
     const int64_t db = rh::dot_blocks(n3);
     RH_HIP(dalloc(&w.part, size_t(rh::kRelaxPart * nb)));
-    RH_HIP(dalloc(&w.dots, size_t(17 * db)));
+    RH_HIP(dalloc(&w.dots, size_t(rh::kDotMax * db)));
     AdjointPass ap;
     ap.chunks = rh::matvec_t_chunks(n3, q->ld);
     struct Free2 {
@@ -1909,7 +1908,7 @@ rh_status rh_relax_adjoint_dev(rh_matrix* q, const double* d_mats, const double*
     if (opts->keep_m)
         RH_HIP(hipMemcpyAsync(w.x, d_lam, size_t(n3) * sizeof(double), hipMemcpyDeviceToDevice, s));
     double* hbuf = nullptr;
-    RH_HIP(hipHostMalloc(reinterpret_cast<void**>(&hbuf), size_t(rh::kRelaxPart * nb + 17 * db) * sizeof(double)));
+    RH_HIP(hipHostMalloc(reinterpret_cast<void**>(&hbuf), size_t(rh::kRelaxPart * nb + rh::kDotMax * db) * sizeof(double)));
     struct PinnedFree {
         double* p;
         ~PinnedFree() { (void)hipHostFree(p); }

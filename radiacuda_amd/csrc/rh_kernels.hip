@@ -1362,8 +1362,8 @@ __global__ __launch_bounds__(256) void k_vcomb(int nv, const double* __restrict_
 
 // ---------------------------------------------------------------- vector helpers
 struct DotPack {
-    const double* a[17];
-    const double* b[17];
+    const double* a[kDotMax];
+    const double* b[kDotMax];
 };
 
 // All nd dot products in one pass: every lane keeps nd running sums, waves
@@ -1371,19 +1371,19 @@ struct DotPack {
 __global__ __launch_bounds__(256) void k_dots(int nd, DotPack pk, int64_t len,
                                               double* __restrict__ partials)
 {
-    __shared__ double red[4][17];
-    double acc[17];
+    __shared__ double red[4][kDotMax];
+    double acc[kDotMax];
 #pragma unroll
-    for (int d = 0; d < 17; d++) acc[d] = 0.0;
+    for (int d = 0; d < kDotMax; d++) acc[d] = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < len;
          i += int64_t(gridDim.x) * 256) {
 #pragma unroll
-        for (int d = 0; d < 17; d++)
+        for (int d = 0; d < kDotMax; d++)
             if (d < nd) acc[d] += pk.a[d][i] * pk.b[d][i];
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int d = 0; d < 17; d++) {
+    for (int d = 0; d < kDotMax; d++) {
         if (d < nd) {
             const double v = wave_sum(acc[d]);
             if (lane == 0) red[wave][d] = v;
@@ -1396,8 +1396,8 @@ __global__ __launch_bounds__(256) void k_dots(int nd, DotPack pk, int64_t len,
 }
 
 struct CombPack {
-    const double* v[34];
-    double c[34];
+    const double* v[kCombMax];
+    double c[kCombMax];
 };
 
 __global__ __launch_bounds__(256) void k_lincomb(int nv, CombPack pk, int64_t len,
@@ -1508,16 +1508,18 @@ __global__ __launch_bounds__(kRelaxLanes) void k_relax_step(int64_t n, const dou
 }
 
 // The Anderson step of the device-driven loop on one wave: the step partials
-// reduced across the wave in a fixed order, then on lane 0 the residual and
-// convergence test, the restart safeguard, the Gram matrix of the differences
-// in use (kept oldest first, the newest row from this step), the regularized
+// reduced across the wave in a fixed order, then the residual and convergence
+// test, the restart safeguard, the Gram matrix of the differences in use
+// (kept oldest first, the newest row from this step), the regularized
 // Cholesky solve of the least-squares problem and the update coefficients,
-// and the history extension for the next iteration.  The small arrays are
-// unrolled over kHistMax with the unused part padded by the identity, so they
-// stay in registers; the arithmetic on the live part is the host-driven
-// loop's.
+// and the history extension for the next iteration.  The Gram matrix lives in
+// LDS; the factorization runs column by column with the lanes updating the
+// trailing entries (the same operations in the same order as a row-by-row
+// factorization), and the two triangular solves column by column.
 __device__ void anderson_wave(int64_t nb, const double* __restrict__ partials, RelaxDevState* __restrict__ st)
 {
+    constexpr int H = kHistMax, HP = kHistMax + 1;  // LDS row stride
+    __shared__ double sA[H * HP], sL[H * HP], sz[H], sv[kRelaxPart];
     const int lane = threadIdx.x & 63;
     double v[kRelaxPart];
 #pragma unroll
@@ -1535,130 +1537,130 @@ __device__ void anderson_wave(int64_t nb, const double* __restrict__ partials, R
     v[2] = wave_max(v[2]);
 #pragma unroll
     for (int k = 3; k < kRelaxPart; k++) v[k] = wave_sum(v[k]);
-    if (lane != 0) return;
-    st->arrive = 0;  // every block of this step has arrived
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < kRelaxPart; k++) sv[k] = v[k];
+    }
+    __syncthreads();
+    // wave-uniform decisions (every lane reads the same state)
     const int hist = st->hist;
     const int head = st->head;
-    st->it += 1;
+    const int it = st->it + 1;
     const double misfit = sqrt(v[0] / st->n);
-    st->misfit = misfit;
-    st->maxM = fmax(1e-17, v[1]);
-    st->maxH = fmax(1e-17, v[2]);
-    if (!isfinite(misfit)) {
-        st->fail = 1;
-        st->done = 2;
-        return;
-    }
-    if (misfit <= st->prec && st->it == 1 && st->keep_if_converged) {
-        st->done = 2;
-        return;
-    }
-    if (misfit <= st->prec || st->it >= st->max_iter) {
-        st->done = 1;
-        return;
-    }
-    const int r0 = (head - hist + 2 * kHistMax) % kHistMax;  // buffer slot of the oldest difference
-    bool plain = (hist == 0);
-    int nh = hist, hd = head;
-    if (misfit > 1e4 * st->best) {
-        nh = 0;
-        hd = 0;
-        st->beta = fmax(0.05, 0.5 * st->beta);
-        plain = true;
-    }
-    st->best = fmin(st->best, misfit);
-    const double beta = st->beta;
-    double gam[kHistMax];
-#pragma unroll
-    for (int a = 0; a < kHistMax; a++) gam[a] = 0.0;
-    if (!plain) {
-        // Gram matrix, oldest first; row and column hist-1 are the newest
-        double A[kHistMax][kHistMax];
-#pragma unroll
-        for (int a = 0; a < kHistMax; a++)
-#pragma unroll
-            for (int b = 0; b < kHistMax; b++) A[a][b] = st->G[a * kHistMax + b];
-#pragma unroll
-        for (int a = 0; a < kHistMax; a++)
-#pragma unroll
-            for (int k = 0; k < kHistMax; k++)
-                if (a == hist - 1 && k < hist) {
-                    A[a][k] = v[3 + k];
-                    A[k][a] = v[3 + k];
-                }
-        // (A + lambda I) gam = r on the live part, identity beyond it
+    const bool finite = isfinite(misfit);
+    const bool finish = !finite || misfit <= st->prec || it >= st->max_iter;
+    const bool restart = !finish && misfit > 1e4 * st->best;
+    const double best = fmin(st->best, misfit);
+    const double beta = restart ? fmax(0.05, 0.5 * st->beta) : st->beta;
+    bool ok = !finish && !restart && hist > 0;
+    if (ok) {
+        // Gram matrix: live part from the state with the newest row and
+        // column from this step; its regularized copy to factor in sL
+        for (int e = lane; e < H * H; e += 64) {
+            const int a = e / H, c = e - H * (e / H);
+            double g = 0.0;
+            if (a < hist && c < hist) {
+                if (a == hist - 1) g = sv[3 + c];
+                else if (c == hist - 1) g = sv[3 + a];
+                else g = st->G[a * H + c];
+            }
+            sA[a * HP + c] = g;
+        }
+        __syncthreads();
         double tr = 0.0;
-#pragma unroll
-        for (int a = 0; a < kHistMax; a++)
-            if (a < hist) tr += A[a][a];
+        for (int a = 0; a < hist; a++) tr += sA[a * HP + a];
         const double lam = 1e-10 * (tr > 0 ? tr / hist : 1.0);
-        double L[kHistMax][kHistMax];
-        bool ok = true;
-#pragma unroll
-        for (int a = 0; a < kHistMax; a++)
-#pragma unroll
-            for (int b = 0; b <= a; b++) {
-                const bool live = a < hist && b < hist;
-                double sm = live ? A[a][b] + ((a == b) ? lam : 0.0) : ((a == b) ? 1.0 : 0.0);
-#pragma unroll
-                for (int k = 0; k < b; k++) sm -= L[a][k] * L[b][k];
-                if (a == b) {
-                    ok = ok && (sm > 0.0);
-                    L[a][a] = sqrt(sm);
-                } else {
-                    L[a][b] = sm / L[b][b];
-                }
+        for (int e = lane; e < H * H; e += 64) {
+            const int a = e / H, c = e - H * (e / H);
+            sL[a * HP + c] = sA[a * HP + c] + ((a == c && a < hist) ? lam : 0.0);
+        }
+        __syncthreads();
+        // Cholesky, column k at a time: pivot, the column below it, then the
+        // trailing entries (i, j), k < j <= i, lose L_ik L_jk
+        for (int k = 0; k < hist && ok; k++) {
+            const double dkk = sL[k * HP + k];
+            if (!(dkk > 0.0)) {
+                ok = false;
+                break;
             }
-        if (ok) {
-            double z[kHistMax];
-#pragma unroll
-            for (int a = 0; a < kHistMax; a++) {
-                double sm = (a < hist) ? v[3 + kHistMax + a] : 0.0;
-#pragma unroll
-                for (int k = 0; k < a; k++) sm -= L[a][k] * z[k];
-                z[a] = sm / L[a][a];
+            const double lkk = sqrt(dkk);
+            __syncthreads();
+            if (lane == k) sL[k * HP + k] = lkk;
+            if (lane > k && lane < hist) sL[lane * HP + k] = sL[lane * HP + k] / lkk;
+            __syncthreads();
+            for (int e = lane; e < H * H; e += 64) {
+                const int i = e / H, j = e - H * (e / H);
+                if (j > k && i >= j && i < hist) sL[i * HP + j] -= sL[i * HP + k] * sL[j * HP + k];
             }
-#pragma unroll
-            for (int a = kHistMax - 1; a >= 0; a--) {
-                double sm = z[a];
-#pragma unroll
-                for (int k = a + 1; k < kHistMax; k++) sm -= L[k][a] * gam[k];
-                gam[a] = sm / L[a][a];
-            }
-            // the Gram matrix of the next iteration: the oldest difference
-            // drops out when the history is full
-            const bool shift = (nh == kHistMax);
-#pragma unroll
-            for (int a = 0; a < kHistMax; a++)
-#pragma unroll
-                for (int b = 0; b < kHistMax; b++)
-                    st->G[a * kHistMax + b] =
-                        shift ? ((a + 1 < kHistMax && b + 1 < kHistMax) ? A[a + 1][b + 1] : 0.0) : A[a][b];
-        } else {
-            nh = 0;
-            hd = 0;
-            plain = true;
+            __syncthreads();
         }
     }
-    st->cF = beta;
-#pragma unroll
-    for (int b = 0; b < kHistMax; b++) {
-        st->cdX[b] = 0.0;
-        st->cdF[b] = 0.0;
+    if (ok) {
+        // L z = r, then L^T gam = z, column by column (gam overwrites z)
+        if (lane < hist) sz[lane] = sv[3 + kHistMax + lane];
+        __syncthreads();
+        for (int k = 0; k < hist; k++) {
+            const double zk = sz[k] / sL[k * HP + k];
+            __syncthreads();
+            if (lane == k) sz[k] = zk;
+            if (lane > k && lane < hist) sz[lane] -= sL[lane * HP + k] * zk;
+            __syncthreads();
+        }
+        for (int k = hist - 1; k >= 0; k--) {
+            const double gk = sz[k] / sL[k * HP + k];
+            __syncthreads();
+            if (lane == k) sz[k] = gk;
+            if (lane < k) sz[lane] -= sL[k * HP + lane] * gk;
+            __syncthreads();
+        }
     }
-    if (!plain) {
-#pragma unroll
-        for (int k = 0; k < kHistMax; k++)
-            if (k < hist) {
-                const int p = (r0 + k) % kHistMax;
-                st->cdX[p] = -gam[k];
-                st->cdF[p] = -beta * gam[k];
+    // state of the next iteration
+    if (lane == 0) {
+        st->arrive = 0;  // every block of this step has arrived
+        st->it = it;
+        st->misfit = misfit;
+        st->maxM = fmax(1e-17, v[1]);
+        st->maxH = fmax(1e-17, v[2]);
+        if (!finite) {
+            st->fail = 1;
+            st->done = 2;
+        } else if (misfit <= st->prec && it == 1 && st->keep_if_converged) {
+            st->done = 2;
+        } else if (finish) {
+            st->done = 1;
+        } else {
+            st->best = best;
+            st->beta = beta;
+            const int nh = ok ? hist : 0, hd = ok ? head : 0;
+            st->cF = beta;
+            for (int c = 0; c < H; c++) {
+                st->cdX[c] = 0.0;
+                st->cdF[c] = 0.0;
             }
+            if (ok) {
+                const int r0 = (head - hist + 2 * H) % H;  // buffer slot of the oldest difference
+                for (int k = 0; k < hist; k++) {
+                    const int p = (r0 + k) % H;
+                    st->cdX[p] = -sz[k];
+                    st->cdF[p] = -beta * sz[k];
+                }
+            }
+            // the next iteration writes a new difference into slot hd
+            st->ext = 1;
+            st->head = (hd + 1) % H;
+            st->hist = nh + 1 < H ? nh + 1 : H;
+        }
     }
-    // the next iteration writes a new difference into slot hd
-    st->ext = 1;
-    st->head = (hd + 1) % kHistMax;
-    st->hist = nh + 1 < kHistMax ? nh + 1 : kHistMax;
+    // the Gram matrix of the next iteration: the oldest difference drops out
+    // when the history is full
+    if (ok) {
+        const int sh = (hist == H) ? 1 : 0;
+        for (int e = lane; e < H * H; e += 64) {
+            const int a = e / H, c = e - H * (e / H);
+            const int a2 = a + sh, c2 = c + sh;
+            st->G[e] = (a2 < H && c2 < H) ? sA[a2 * HP + c2] : 0.0;
+        }
+    }
 }
 
 // The same step in the device-driven loop (history slots from the state); the
@@ -2033,7 +2035,7 @@ int64_t dot_blocks(int64_t len) { return std::max<int64_t>(1, std::min<int64_t>(
 hipError_t launch_dots(int nd, const double* const* a, const double* const* b, int64_t len,
                        double* partials, hipStream_t s)
 {
-    if (nd < 1 || nd > 17) return hipErrorInvalidValue;
+    if (nd < 1 || nd > kDotMax) return hipErrorInvalidValue;
     DotPack pk;
     for (int d = 0; d < nd; d++) {
         pk.a[d] = a[d];
@@ -2046,7 +2048,7 @@ hipError_t launch_dots(int nd, const double* const* a, const double* const* b, i
 hipError_t launch_lincomb(int nv, const double* const* v, const double* c, int64_t len, double* y,
                           hipStream_t s)
 {
-    if (nv < 1 || nv > 34) return hipErrorInvalidValue;
+    if (nv < 1 || nv > kCombMax) return hipErrorInvalidValue;
     CombPack pk;
     for (int k = 0; k < nv; k++) {
         pk.v[k] = v[k];

@@ -111,8 +111,17 @@ hipError_t launch_relax_update(int64_t n, const double* mats, const double* tab,
 // which is dFw) with every dF[k] and of every dF[k] with F.  partials holds
 // kRelaxPart doubles per block: [sum |F|^2, max |M_prop|, max |H|, nh dots
 // with the newest difference, padding to kHistMax, nh dots with F, padding].
-constexpr int kHistMax = 8;
+// Anderson depth: on the 40,000-tetrahedron yoke (prec 1e-5) 16 differences
+// took 55-59 iterations where 8 took 60-85 (over mixing factors 0.3-0.9)
+#ifndef RH_HIST
+#define RH_HIST 16
+#endif
+constexpr int kHistMax = RH_HIST;
 constexpr int kRelaxPart = 3 + 2 * kHistMax;
+// dot products and combined vectors of one host-driven Anderson iteration
+// (launch_dots, launch_lincomb)
+constexpr int kDotMax = 2 * kHistMax + 1;
+constexpr int kCombMax = 2 * kHistMax + 2;
 struct HistPack {
     double* Fp = nullptr;
     double* xp = nullptr;

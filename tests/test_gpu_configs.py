@@ -86,11 +86,14 @@ def test_iron_cube_full_size():
     Qo, _jit = models.oracle_assemble(P)
     assert models.block_rel(Qs, Qo, n) <= 1e-10
     t = time.perf_counter()
-    res, M, _H = Q.relax(mats, hext, np.zeros((n, 3)), 1e-10, 2000)
+    res, M, _H = Q.relax(mats, hext, np.zeros((n, 3)), 1e-11, 3000)
     t_solve = time.perf_counter() - t
     assert res.converged
-    # FP64 fixed point of the oracle's system (Newton polish from the GPU result)
+    # FP64 fixed point of the oracle's system (Newton polish from the GPU
+    # result): the RMS residual 1e-11 T times the iron's conditioning
     Mn = models.fp64_fixed_point(Qo, mats, hext, M0=M)
+    print("iron cube: %d iterations to misfit %.2e, vs FP64 fixed point %.2e"
+          % (res.iters, res.misfit, np.abs(M - Mn).max() / np.abs(Mn).max()), flush=True)
     assert np.abs(M - Mn).max() <= 1e-7 * np.abs(Mn).max()
     # field of the relaxed state at the 10,000 line points: elements + coil
     t = time.perf_counter()
@@ -141,14 +144,21 @@ def test_large_yoke_120k_matrix():
         print("prec %d: assembly %.3f s, solve %.3f s, %d iterations, misfit %.2e, refined %d"
               % (prec, t_asm, t_solve, res.iters, res.misfit, res.refined), flush=True)
         assert res.converged and res.misfit <= 1e-5
-        out[prec] = (R, M)
+        # on to a tight prec from there (the mixed matrix with its FP64
+        # refinement): both reach the FP64 system's fixed point
+        t = time.perf_counter()
+        rt, Mt, _H = Q.relax(mats, hext, M, 1e-9, 4000, keep_m=True, refine=3)
+        print("prec %d: on to misfit %.2e in %d more iterations (%.2f s), refined %d"
+              % (prec, rt.misfit, rt.iters, time.perf_counter() - t, rt.refined), flush=True)
+        assert rt.converged
+        out[prec] = (R, M, Mt)
         Q.close()
     # sampled rows: against the host build of the same integrals, against the
     # oracle on every column but the tilted-face elements (a face within 1e-7
     # of -z: the reference's face rotation drops its tilt, radvlpgn.cpp:357-371),
     # and those columns against 50-digit values (tests/golden/mp50_yoke.json)
-    R64, M64 = out[N.RH_FP64]
-    R32, M32 = out[N.RH_MIXED]
+    R64, M64, M64t = out[N.RH_FP64]
+    R32, M32, M32t = out[N.RH_MIXED]
     Rh = np.concatenate([_host_rows(P, i) for i in rows])
     om = OR.prepare_model(P)
     Ro = np.concatenate([OR.assemble(om, rows=(i, i + 1)) for i in rows])
@@ -182,8 +192,16 @@ def test_large_yoke_120k_matrix():
     # agree to the stopping tolerance's reach (max |dM| ~ 4e-4 of max |M| at
     # chi ~ 2000), not to the precision of either matrix
     dM = np.abs(M32 - M64).max() / np.abs(M64).max()
-    print("large yoke: refined mixed vs FP64 solve, max |dM|/max|M| = %.2e" % dM, flush=True)
+    # at prec 1e-9 the refined mixed state is the FP64 one to the precision
+    # target: magnetization and the field it makes in the gap
+    dMt = np.abs(M32t - M64t).max() / np.abs(M64t).max()
+    gpts = W.gap_grid(info, 20)[::97]
+    B64t = model.field(M64t, gpts)[0]
+    dBt = np.abs(model.field(M32t, gpts)[0] - B64t).max() / np.abs(B64t).max()
+    print("large yoke: refined mixed vs FP64 solve, max |dM|/max|M| = %.2e at prec 1e-5; at prec 1e-9 "
+          "%.2e, gap field %.2e" % (dM, dMt, dBt), flush=True)
     assert dM <= 1e-3
+    assert dMt <= 1e-5 and dBt <= 1e-6
     # field at gap points: host build; oracle without the tilted-face
     # elements; those elements against 50-digit values
     pts = W.gap_grid(info, 20)[::97]
