@@ -3,7 +3,7 @@ kernel math itself: the host build instantiates the FP64 integrals with an
 op-counting scalar (radiacuda_amd/csrc/rh_opcount.hpp, rhc_opcount in
 _rhcheck.so) and counts every operation on the branches the points take.
 
-Weights (fixed): + - x / sqrt count 1, a fused
+Weights (fixed in advance of any measurement): + - x / sqrt count 1, a fused
 multiply-add 2, a log 20, an atan2 30.  The counting scalar sees the
 arithmetic done in the integrals' own precision; the few FP64 operations the
 kernels add around them are counted here by hand and listed next to each
