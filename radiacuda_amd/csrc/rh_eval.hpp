@@ -7,6 +7,8 @@
 //           x' = Minv_c (x - V_c)   (cpp/src/core/radg3d.cpp:57-115)
 #pragma once
 
+#include <type_traits>
+
 #include "rh_math.hpp"
 
 namespace rh {
@@ -146,9 +148,85 @@ RH_HD void block_Q_mixed(const ModelView& mv, int64_t j, const double x[3], doub
     }
 }
 
+// Near test of the FP32 field sweeps (element frame): blocks and tetrahedra
+// as the assembly's; general polyhedra: a vertex within kNear32 longest edges
+// of the point.
+template <int KIND>
+RH_HD bool near_field(int kind, const double* __restrict__ rec, const double x[3])
+{
+    const int K = (KIND >= 0) ? KIND : (kind == EK_REC ? 0 : (kind == EK_TET ? 1 : 2));
+    if (K != 2) return near_element<KIND>(kind, rec, x);
+    const int nv = int(rec[0]), ne = int(rec[1]);
+    const double* vt = rec + kGenHdr;
+    const double* ed = vt + 3 * nv;
+    double d2 = 1e300, l = 0.0;
+    for (int v = 0; v < nv; v++) {
+        const double a = vt[3 * v] - x[0], b = vt[3 * v + 1] - x[1], c = vt[3 * v + 2] - x[2];
+        d2 = fmin(d2, a * a + b * b + c * c);
+    }
+    for (int e = 0; e < ne; e++) l = fmax(l, ed[kGenEdge * e + 5]);
+    const double t = double(kNear32) * l;
+    return d2 < t * t;
+}
+
+// Passes of a field sweep over a range of elements: every element at R
+// (kPassAll; FP32 takes FP64 for the near elements inline), the FP32 sweep's
+// far elements only (kPassFar), or its near elements in FP64 (kPassNear), run
+// after kPassFar over the same elements and waves and added to its result.
+// Split in two kernels, the FP64 code does not weigh on the FP32 sweep's
+// registers.
+enum : int { kPassAll = 0, kPassFar = 1, kPassNear = 2 };
+
+// The near test of the FP32 sweeps for one element at x (element frame):
+// wave-uniform on the device.
+template <int KIND>
+RH_HD bool near32(int kind, const double* __restrict__ rec, const double* __restrict__ crec,
+                  const double x[3])
+{
+    if (crec) return tet_near32(crec, x);
+    bool near = near_field<KIND>(kind, rec, x);
+#ifdef __HIP_DEVICE_COMPILE__
+    near = __any(near);
+#endif
+    return near;
+}
+
+// One element's H at x (element frame) from its record (or its charged
+// record crec) and magnetization m; returns the inside flag.
+template <class R, int KIND>
+RH_HD bool elem_H_at(int kind, const double* __restrict__ rec, const double* __restrict__ crec,
+                     const double m[3], const double x[3], double hl[3])
+{
+    if (crec) return field_tet<R>(crec, x, hl);
+    double q6[6];
+    const bool in = eval_k<R, KIND>(kind, rec, x, q6);
+    hl[0] = q6[0] * m[0] + q6[3] * m[1] + q6[4] * m[2];
+    hl[1] = q6[3] * m[0] + q6[1] * m[1] + q6[5] * m[2];
+    hl[2] = q6[4] * m[0] + q6[5] * m[1] + q6[2] * m[2];
+    return in;
+}
+
+// elem_H_at for pass PASS: false when the pass skips the element.
+template <class R, int KIND, int PASS>
+RH_HD bool elem_H(int kind, const double* __restrict__ rec, const double* __restrict__ crec,
+                  const double m[3], const double x[3], double hl[3], bool& in)
+{
+    if constexpr (PASS == kPassAll && std::is_same<R, float>::value) {
+        in = near32<KIND>(kind, rec, crec, x) ? elem_H_at<double, KIND>(kind, rec, crec, m, x, hl)
+                                               : elem_H_at<float, KIND>(kind, rec, crec, m, x, hl);
+        return true;
+    }
+    if constexpr (PASS != kPassAll) {
+        if (near32<KIND>(kind, rec, crec, x) != (PASS == kPassNear)) return false;
+    }
+    in = elem_H_at<R, KIND>(kind, rec, crec, m, x, hl);
+    return true;
+}
+
 // Adds the B and H of the elements idx[k0..k1) (all of kind slot KIND, or of
-// any kind when KIND < 0; idx == nullptr means the identity list) at x.
-template <class R, bool SYM, int KIND = -1>
+// any kind when KIND < 0; idx == nullptr means the identity list) at x, for
+// pass PASS.
+template <class R, bool SYM, int KIND = -1, int PASS = kPassAll>
 RH_HD void field_add(const ModelView& mv, const double* M, const int32_t* idx, int64_t k0,
                      int64_t k1, const double x[3], double B[3], double H[3])
 {
@@ -162,15 +240,7 @@ RH_HD void field_add(const ModelView& mv, const double* M, const int32_t* idx, i
         if (!SYM) {
             double hl[3];
             bool in;
-            if (charged) {
-                in = field_tet<R>(crec, x, hl);
-            } else {
-                double q6[6];
-                in = eval_k<R, KIND>(kind, rec, x, q6);
-                hl[0] = q6[0] * m[0] + q6[3] * m[1] + q6[4] * m[2];
-                hl[1] = q6[3] * m[0] + q6[1] * m[1] + q6[5] * m[2];
-                hl[2] = q6[4] * m[0] + q6[5] * m[1] + q6[2] * m[2];
-            }
+            if (!elem_H<R, KIND, PASS>(kind, rec, crec, m, x, hl, in)) continue;
             H[0] += hl[0]; H[1] += hl[1]; H[2] += hl[2];
             B[0] += hl[0] + (in ? m[0] : 0.0);
             B[1] += hl[1] + (in ? m[1] : 0.0);
@@ -182,15 +252,7 @@ RH_HD void field_add(const ModelView& mv, const double* M, const int32_t* idx, i
             double xl[3], hl[3];
             apply_inv(cp, x, xl);
             bool in;
-            if (charged) {
-                in = field_tet<R>(crec, xl, hl);
-            } else {
-                double q6[6];
-                in = eval_k<R, KIND>(kind, rec, xl, q6);
-                hl[0] = q6[0] * m[0] + q6[3] * m[1] + q6[4] * m[2];
-                hl[1] = q6[3] * m[0] + q6[1] * m[1] + q6[5] * m[2];
-                hl[2] = q6[4] * m[0] + q6[5] * m[1] + q6[2] * m[2];
-            }
+            if (!elem_H<R, KIND, PASS>(kind, rec, crec, m, xl, hl, in)) continue;
             double bl[3] = {hl[0] + (in ? m[0] : 0.0), hl[1] + (in ? m[1] : 0.0),
                             hl[2] + (in ? m[2] : 0.0)};
             double hg[3], bg[3];

@@ -255,7 +255,33 @@ hipError_t asm_dispatch(const DevModel& m, OUT* Q, int64_t ld, double* qdiag, in
 // ---------------------------------------------------------------- field
 // Sums the elements kidx[koff[KIND] + y*js ...] of one kind slot; slice
 // `slice0 + blockIdx.y` of the partial buffer (or B/H directly).
-template <class R, bool SYM, int KIND>
+// The FP32 sweep runs as two launches over the same grid: kPassFar (FP32,
+// the elements no lane of the wave is near) stores the sums, kPassNear (FP64,
+// the others) adds to them.
+template <int PASS>
+__device__ __forceinline__ void field_store(double* __restrict__ o, const double b[3], const double h[3])
+{
+    for (int k = 0; k < 3; k++) {
+        if (PASS == kPassNear) {
+            o[k] += b[k];
+            o[3 + k] += h[k];
+        } else {
+            o[k] = b[k];
+            o[3 + k] = h[k];
+        }
+    }
+}
+template <int PASS>
+__device__ __forceinline__ void field_store(double* __restrict__ B, double* __restrict__ H, int64_t p,
+                                            const double b[3], const double h[3])
+{
+    for (int k = 0; k < 3; k++) {
+        if (B) B[3 * p + k] = (PASS == kPassNear) ? B[3 * p + k] + b[k] : b[k];
+        if (H) H[3 * p + k] = (PASS == kPassNear) ? H[3 * p + k] + h[k] : h[k];
+    }
+}
+
+template <class R, bool SYM, int KIND, int PASS>
 __global__ __launch_bounds__(256) void k_field(DevModel m, const double* __restrict__ M,
                                                const double* __restrict__ pts, int64_t np,
                                                int64_t js, int64_t slice0, double* __restrict__ B,
@@ -268,19 +294,11 @@ __global__ __launch_bounds__(256) void k_field(DevModel m, const double* __restr
     const int64_t k1 = (k0 + js < kend) ? (k0 + js) : kend;
     const double x[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
     double b[3] = {0, 0, 0}, h[3] = {0, 0, 0};
-    field_add<R, SYM, KIND>(view(m), M, m.kidx, k0, k1, x, b, h);
-    if (part) {
-        double* o = part + ((slice0 + int64_t(blockIdx.y)) * np + p) * 6;
-        for (int k = 0; k < 3; k++) {
-            o[k] = b[k];
-            o[3 + k] = h[k];
-        }
-    } else {
-        for (int k = 0; k < 3; k++) {
-            if (B) B[3 * p + k] = b[k];
-            if (H) H[3 * p + k] = h[k];
-        }
-    }
+    field_add<R, SYM, KIND, PASS>(view(m), M, m.kidx, k0, k1, x, b, h);
+    if (part)
+        field_store<PASS>(part + ((slice0 + int64_t(blockIdx.y)) * np + p) * 6, b, h);
+    else
+        field_store<PASS>(B, H, p, b, h);
 }
 
 // Tetrahedron field sweep over charged records staged in LDS: the workgroup
@@ -298,6 +316,9 @@ constexpr int kTetTile = 32;
 // staged per LDS tile.  FP64: 5 waves (96 VGPRs, 16 bytes of spill) with
 // 64-record tiles measured 53.3 ms against 54.5 ms at 4 waves with 128-record
 // tiles on the C-yoke; FP32 runs 7 waves at 67 VGPRs.
+#ifndef RH_MESH_WAVES32
+#define RH_MESH_WAVES32 7
+#endif
 #ifndef RH_MESH_WAVES
 #define RH_MESH_WAVES 5
 #endif
@@ -308,7 +329,7 @@ constexpr int kTetTile = 32;
 #define RH_MESH_STAGE 64
 #endif
 
-template <class R, bool SYM>
+template <class R, bool SYM, int PASS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH_TET_WAVES))) void k_field_tet(DevModel m, const double* __restrict__ pts,
                                                    int64_t np, int64_t js, int64_t slice0,
                                                    double* __restrict__ B, double* __restrict__ H,
@@ -341,6 +362,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH_TET_WAVE
             const double mm[3] = {rec[kChargedM], rec[kChargedM + 1], rec[kChargedM + 2]};
             if (!SYM) {
                 double hl[3];
+                if (PASS != kPassAll && tet_near32(rec, x) != (PASS == kPassNear)) continue;
                 const bool in = field_tet<R>(rec, x, hl);
                 for (int k = 0; k < 3; k++) {
                     h[k] += hl[k];
@@ -353,6 +375,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH_TET_WAVE
                 const double* cp = m.copies + 24 * c;
                 double xl[3], hl[3], hg[3], bg[3];
                 apply_inv(cp, x, xl);
+                if (PASS != kPassAll && tet_near32(rec, xl) != (PASS == kPassNear)) continue;
                 const bool in = field_tet<R>(rec, xl, hl);
                 const double bl[3] = {hl[0] + (in ? mm[0] : 0.0), hl[1] + (in ? mm[1] : 0.0),
                                       hl[2] + (in ? mm[2] : 0.0)};
@@ -366,18 +389,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH_TET_WAVE
         }
     }
     if (!live) return;
-    if (part) {
-        double* o = part + ((slice0 + int64_t(blockIdx.y)) * np + p) * 6;
-        for (int k = 0; k < 3; k++) {
-            o[k] = b[k];
-            o[3 + k] = h[k];
-        }
-    } else {
-        for (int k = 0; k < 3; k++) {
-            if (B) B[3 * p + k] = b[k];
-            if (H) H[3 * p + k] = h[k];
-        }
-    }
+    if (part)
+        field_store<PASS>(part + ((slice0 + int64_t(blockIdx.y)) * np + p) * 6, b, h);
+    else
+        field_store<PASS>(B, H, p, b, h);
 }
 
 // ---------------------------------------------------------------- mesh form
@@ -409,6 +424,9 @@ __global__ __launch_bounds__(256) void k_mesh_charge_edges(MeshDev md, const dou
         f[1] = t1;
         f[2] = t2;
         f[3] = len;
+        // slot 8: the near distance of the FP32 sweep (kNear32 lengths)
+        f[4] = kNear32 * len;
+        f[5] = 0.0f;
     }
 }
 
@@ -452,6 +470,18 @@ __global__ __launch_bounds__(256) void k_mesh_charge_faces(MeshDev md, const dou
     mk[1] = 0;
     o[kMeshFaceEdges + 13] = 0.0;
     if (pack32) {
+        // slot 35: the near distance of the FP32 sweep (kNear32 longest sides)
+        const int nv = g[17] > 3.5 ? 4 : 3;
+        double lmax = 0.0;
+        for (int k = 0; k < nv; k++) {
+            const double* a = g + 3 * k;
+            const double* b = g + 3 * ((k + 1) % nv);
+            const double d0 = b[0] - a[0], d1 = b[1] - a[1], d2 = b[2] - a[2];
+            lmax = fmax(lmax, sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+        }
+        reinterpret_cast<float*>(o + kMeshFaceEdges + 13)[0] = float(double(kNear32) * lmax);
+    }
+    if (pack32) {
         float* f = reinterpret_cast<float*>(o + 12);
         const float a1 = float(g[15]), a2 = float(g[16]);
         f[0] = float(n0);
@@ -484,7 +514,7 @@ __global__ __launch_bounds__(256) void k_mesh_charge_in(MeshDev md, const double
 // records read at wave-uniform addresses.  Inside tiles whose bounding box
 // holds no point of the wave are skipped (wave-uniform vote).
 template <class R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RH_MESH_WAVES))) void
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? RH_MESH_WAVES32 : RH_MESH_WAVES))) void
 k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, int64_t slice0,
              double* __restrict__ B, double* __restrict__ H, double* __restrict__ part)
 {
@@ -508,19 +538,37 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
     // offset and distance from the previous edge.  The loop is unrolled by
     // two with the vertex slots P, Q swapping roles, so the carried values
     // stay in place (no register copies per edge).
+    // FP32: the records of a tile that the wave is near (mesh32_near) are
+    // marked in `defer` and evaluated in FP64 after the tile's loop, where
+    // the loop's carried values are dead, so the FP64 code does not add to
+    // the loop's registers
+    static_assert(kStage <= 64, "one bit per staged record");
+    uint64_t defer = 0;
     {
         MeshVtx<R> P{}, Q{};
         const int64_t e0 = md.ne_free * y / S, e1 = md.ne_free * (y + 1) / S;
-        auto edge = [&](int64_t g, const double* rec, MeshVtx<R>& a, MeshVtx<R>& b) {
+        auto edge = [&](int64_t g, int q, MeshVtx<R>& a, MeshVtx<R>& b) {
+            const double* rec = tile + kMeshEdge * q;
             if (!(mesh_code(rec + 13) == 1 && g != e0)) a = mesh_vtx<R>(rec, x);
             b = mesh_vtx<R>(rec + 3, x);
             // L_e from the end offsets, distances and the length (the
-            // FP32 records carry the length as a float in slot 7)
-            const R len = (sizeof(R) == 4) ? R(mesh32(rec, 6)[3]) : R(rec[9]);
-            const double L = double(edge_L(a.r, a.d, b.r, b.d, len));
-            h[0] += L * rec[10];
-            h[1] += L * rec[11];
-            h[2] += L * rec[12];
+            // FP32 records carry the length as a float in slot 7, and keep
+            // it in FP64 in slot 9 for the waves near the edge)
+            if constexpr (sizeof(R) == 4) {
+                if (mesh32_near(fminf(a.d, b.d), mesh32(rec, 8)[0])) {
+                    defer |= uint64_t(1) << q;
+                } else {
+                    const double L = double(edge_L(a.r, a.d, b.r, b.d, mesh32(rec, 6)[3]));
+                    h[0] += L * rec[10];
+                    h[1] += L * rec[11];
+                    h[2] += L * rec[12];
+                }
+            } else {
+                const double L = double(edge_L(a.r, a.d, b.r, b.d, R(rec[9])));
+                h[0] += L * rec[10];
+                h[1] += L * rec[11];
+                h[2] += L * rec[12];
+            }
         };
         for (int64_t base = e0; base < e1; base += kStage) {
             const int cnt = int((e1 - base < kStage) ? (e1 - base) : kStage);
@@ -532,12 +580,21 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
             int q = 0;
 #pragma unroll 1
             for (; q + 2 <= cnt; q += 2) {
-                edge(base + q, tile + kMeshEdge * q, P, Q);
-                edge(base + q + 1, tile + kMeshEdge * (q + 1), Q, P);
+                edge(base + q, q, P, Q);
+                edge(base + q + 1, q + 1, Q, P);
             }
             if (q < cnt) {
-                edge(base + q, tile + kMeshEdge * q, P, Q);
+                edge(base + q, q, P, Q);
                 P = Q;
+            }
+            if constexpr (sizeof(R) == 4) {
+                for (; defer; defer &= defer - 1) {
+                    const double* rec = tile + kMeshEdge * __builtin_ctzll(defer);
+                    const double L = mesh_edge_L_near(rec, x);
+                    h[0] += L * rec[10];
+                    h[1] += L * rec[11];
+                    h[2] += L * rec[12];
+                }
             }
         }
     }
@@ -558,36 +615,63 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
             h[1] += L * e[2];
             h[2] += L * e[3];
         };
-        auto face = [&](int64_t g, const double* rec, MeshVtx<R>& a, MeshVtx<R>& b, MeshVtx<R>& c, R& dab,
-                        R& dbc) {
+        auto face = [&](int64_t g, int q, MeshVtx<R>& a, MeshVtx<R>& b, MeshVtx<R>& c, R& dab, R& dbc) {
+            const double* rec = tile + kMeshFace * q;
             const int code = mesh_code(rec + 21);
-            double om;
-            if (code == 2) {
-                if constexpr (sizeof(R) == 4)
-                    om = double(mesh32_quad_omega(rec, x));
-                else
-                    om = mesh_face_omega<R>(rec, x);
-            } else {
-                if (!(code == 1 && g != f0)) {
-                    a = mesh_vtx<R>(rec, x);
-                    b = mesh_vtx<R>(rec + 3, x);
-                    dab = dot(a.r, b.r);
+            if constexpr (sizeof(R) == 4) {
+                // a near record only sets its bit; the carried values
+                // (b, c, dbc) are in place either way
+                float om;
+                if (code == 2) {
+                    if (!mesh32_quad_omega(rec, x, om)) {
+                        defer |= uint64_t(1) << q;
+                        return;
+                    }
+                } else {
+                    if (!(code == 1 && g != f0)) {
+                        a = mesh_vtx<R>(rec, x);
+                        b = mesh_vtx<R>(rec + 3, x);
+                        dab = dot(a.r, b.r);
+                    }
+                    c = mesh_vtx<R>(rec + 6, x);
+                    dbc = dot(b.r, c.r);
+                    if (mesh32_near(fminf(fminf(a.d, b.d), c.d), mesh32(rec, kMeshFaceEdges + 13)[0])) {
+                        defer |= uint64_t(1) << q;
+                        return;
+                    }
+                    const R dca = dot(c.r, a.r);
+                    const int mask = mesh_mask(rec + kMeshFaceEdges + 12);
+                    om = mesh32_tri_omega(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, dca, rec);
+                    if (mask & 1) owned(a, b, dab, rec + kMeshFaceEdges);
+                    if (mask & 2) owned(b, c, dbc, rec + kMeshFaceEdges + 4);
+                    if (mask & 4) owned(c, a, dca, rec + kMeshFaceEdges + 8);
                 }
-                c = mesh_vtx<R>(rec + 6, x);
-                dbc = dot(b.r, c.r);
-                const R dca = dot(c.r, a.r);
-                if constexpr (sizeof(R) == 4)
-                    om = double(mesh32_tri_omega(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, dca, rec));
-                else
+                h[0] -= double(om) * rec[18];
+                h[1] -= double(om) * rec[19];
+                h[2] -= double(om) * rec[20];
+            } else {
+                double om;
+                if (code == 2) {
+                    om = mesh_face_omega<R>(rec, x);
+                } else {
+                    if (!(code == 1 && g != f0)) {
+                        a = mesh_vtx<R>(rec, x);
+                        b = mesh_vtx<R>(rec + 3, x);
+                        dab = dot(a.r, b.r);
+                    }
+                    c = mesh_vtx<R>(rec + 6, x);
+                    dbc = dot(b.r, c.r);
+                    const R dca = dot(c.r, a.r);
+                    const int mask = mesh_mask(rec + kMeshFaceEdges + 12);
                     om = mesh_tri_omega_from<R>(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, dca, rec);
-                const int mask = mesh_mask(rec + kMeshFaceEdges + 12);
-                if (mask & 1) owned(a, b, dab, rec + kMeshFaceEdges);
-                if (mask & 2) owned(b, c, dbc, rec + kMeshFaceEdges + 4);
-                if (mask & 4) owned(c, a, dca, rec + kMeshFaceEdges + 8);
+                    if (mask & 1) owned(a, b, dab, rec + kMeshFaceEdges);
+                    if (mask & 2) owned(b, c, dbc, rec + kMeshFaceEdges + 4);
+                    if (mask & 4) owned(c, a, dca, rec + kMeshFaceEdges + 8);
+                }
+                h[0] -= om * rec[18];
+                h[1] -= om * rec[19];
+                h[2] -= om * rec[20];
             }
-            h[0] -= om * rec[18];
-            h[1] -= om * rec[19];
-            h[2] -= om * rec[20];
         };
         for (int64_t base = f0; base < f1; base += kStage) {
             const int cnt = int((f1 - base < kStage) ? (f1 - base) : kStage);
@@ -599,15 +683,15 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
             int q = 0;
 #pragma unroll 1
             for (; q + 3 <= cnt; q += 3) {
-                face(base + q, tile + kMeshFace * q, A, Bv, C, dAB, dBC);
-                face(base + q + 1, tile + kMeshFace * (q + 1), Bv, C, A, dBC, dCA);
-                face(base + q + 2, tile + kMeshFace * (q + 2), C, A, Bv, dCA, dAB);
+                face(base + q, q, A, Bv, C, dAB, dBC);
+                face(base + q + 1, q + 1, Bv, C, A, dBC, dCA);
+                face(base + q + 2, q + 2, C, A, Bv, dCA, dAB);
             }
             // remainder, then the carried pair back into (A, B)
             if (q < cnt) {
-                face(base + q, tile + kMeshFace * q, A, Bv, C, dAB, dBC);
+                face(base + q, q, A, Bv, C, dAB, dBC);
                 if (q + 1 < cnt) {
-                    face(base + q + 1, tile + kMeshFace * (q + 1), Bv, C, A, dBC, dCA);
+                    face(base + q + 1, q + 1, Bv, C, A, dBC, dCA);
                     Bv = A;
                     A = C;
                     dAB = dCA;
@@ -615,6 +699,24 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
                     A = Bv;
                     Bv = C;
                     dAB = dBC;
+                }
+            }
+            if constexpr (sizeof(R) == 4) {
+                for (; defer; defer &= defer - 1) {
+                    const double* rec = tile + kMeshFace * __builtin_ctzll(defer);
+                    double om;
+                    if (mesh_code(rec + 21) == 2) {
+                        om = mesh_face_omega_near(rec, x);
+                    } else {
+                        const MeshTri64 t = mesh_tri_near(rec, mesh_mask(rec + kMeshFaceEdges + 12), x);
+                        om = t.om;
+                        h[0] += t.h0;
+                        h[1] += t.h1;
+                        h[2] += t.h2;
+                    }
+                    h[0] -= om * rec[18];
+                    h[1] -= om * rec[19];
+                    h[2] -= om * rec[20];
                 }
             }
         }
@@ -1784,14 +1886,29 @@ void field_slot(const DevModel& m, const double* M, const double* pts, int64_t n
     const int64_t cnt = m.koff[KIND + 1] - m.koff[KIND];
     int64_t js = (cnt + splits - 1) / splits;
     const dim3 grid{unsigned((np + 255) / 256), unsigned(splits), 1u};
+    constexpr bool f32 = sizeof(R) == 4;
     if (KIND == 1 && m.charged) {
         js = (js + kTetTile - 1) / kTetTile * kTetTile;
-        hipLaunchKernelGGL((k_field_tet<R, SYM>), grid, dim3(256), 0, s, m, pts, np, js, slice0, B,
-                           H, part);
+        if constexpr (f32) {
+            hipLaunchKernelGGL((k_field_tet<float, SYM, kPassFar>), grid, dim3(256), 0, s, m, pts, np, js,
+                               slice0, B, H, part);
+            hipLaunchKernelGGL((k_field_tet<double, SYM, kPassNear>), grid, dim3(256), 0, s, m, pts, np, js,
+                               slice0, B, H, part);
+        } else {
+            hipLaunchKernelGGL((k_field_tet<double, SYM, kPassAll>), grid, dim3(256), 0, s, m, pts, np, js,
+                               slice0, B, H, part);
+        }
         return;
     }
-    hipLaunchKernelGGL((k_field<R, SYM, KIND>), grid, dim3(256), 0, s, m, M, pts, np, js, slice0, B,
-                       H, part);
+    if constexpr (f32) {
+        hipLaunchKernelGGL((k_field<float, SYM, KIND, kPassFar>), grid, dim3(256), 0, s, m, M, pts, np, js,
+                           slice0, B, H, part);
+        hipLaunchKernelGGL((k_field<double, SYM, KIND, kPassNear>), grid, dim3(256), 0, s, m, M, pts, np, js,
+                           slice0, B, H, part);
+    } else {
+        hipLaunchKernelGGL((k_field<double, SYM, KIND, kPassAll>), grid, dim3(256), 0, s, m, M, pts, np, js,
+                           slice0, B, H, part);
+    }
 }
 
 bool mesh_on(const DevModel& m) { return m.use_mesh && m.mesh.active; }

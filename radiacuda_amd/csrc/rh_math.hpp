@@ -651,7 +651,7 @@ RH_HD bool eval_tet(const double* __restrict__ rec, const double p[3], double q6
 // ---------------------------------------------------------------- charged tetrahedron
 // Field-only record for a given magnetization M (element frame), built per
 // field call: verts(12) | edges x6: t(3), len, w = W_e M (3) | faces x4: n(3),
-// 2*area, g = (n.M) n (3) | M(3), pad.  H = 1/(4pi) [sum L_e w_e - sum Omega_f g_f].
+// 2*area, g = (n.M) n (3) | M(3), longest edge.  H = 1/(4pi) [sum L_e w_e - sum Omega_f g_f].
 // 86 doubles = 43 x 16 bytes.
 constexpr int kChargedM = 12 + 6 * 7 + 4 * 7;  // offset of M
 constexpr int kChargedTet = kChargedM + 4;     // 86
@@ -678,7 +678,33 @@ RH_HD void charge_tet(const double* __restrict__ rec, const double M[3], double*
     out[kChargedM] = M[0];
     out[kChargedM + 1] = M[1];
     out[kChargedM + 2] = M[2];
-    out[kChargedM + 3] = 0.0;
+    // the longest edge: the FP32 sweep's near test
+    double lmax = 0.0;
+    for (int e = 0; e < 6; e++) lmax = fmax(lmax, rec[12 + kTetEdge * e + 3]);
+    out[kChargedM + 3] = lmax;
+}
+
+// Near test of the FP32 sweeps: FP32 offsets carry eps32 of the distance, so
+// a term's relative error grows like eps32 * size / distance.  A wave with a
+// point within 1.5 longest edges of a vertex evaluates the element in FP64.
+constexpr float kNear32 = 1.5f;
+
+// The near test of a charged tetrahedron for the FP32 sweep (the distances as
+// field_tet<float> forms them); on the device true for the whole wave when
+// any lane is near.
+RH_HD bool tet_near32(const double* __restrict__ rec, const double p[3])
+{
+    float d = 3.0e38f;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const V3<float> r = rel<float>(rec + 3 * v, p);
+        d = fminf(d, rh_sqrt(dot(r, r)));
+    }
+    bool near = d < kNear32 * float(rec[kChargedM + 3]);
+#ifdef __HIP_DEVICE_COMPILE__
+    near = __any(near);
+#endif
+    return near;
 }
 
 template <class R>

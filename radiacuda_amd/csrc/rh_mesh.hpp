@@ -137,18 +137,103 @@ RH_HD float mesh32_tri_omega(const V3<float>& r0, float R0, const V3<float>& r1,
     return 2.0f * rh_atan2_wave(im, re);
 }
 
-RH_HD float mesh32_quad_omega(const double* __restrict__ rec, const double p[3])
+// FP64 solid angle of a face of an FP32 sweep record, whose normal and areas
+// are stored as floats: the fan triangles' triple products r0 . (r1 x r2)
+// are taken as r0 . ((v1 - v0) x (v2 - v0)) from the FP64 vertices, signed by
+// the stored normal (the same quantity as area * (r0 . n) of the FP64 sweep).
+RH_HD double mesh_face_omega_near(const double* __restrict__ rec, const double p[3])
+{
+    const float* f = mesh32(rec, 12);
+    const V3<double> e1 = v3(rec[3] - rec[0], rec[4] - rec[1], rec[5] - rec[2]);
+    const V3<double> e2 = v3(rec[6] - rec[0], rec[7] - rec[1], rec[8] - rec[2]);
+    const V3<double> c1 = cross(e1, e2);
+    const double sg = (c1.x * f[0] + c1.y * f[1] + c1.z * f[2] >= 0.0) ? 1.0 : -1.0;
+    const V3<double> r0 = rel<double>(rec, p), r1 = rel<double>(rec + 3, p), r2 = rel<double>(rec + 6, p);
+    const double R0 = rh_sqrt(dot(r0, r0)), R1 = rh_sqrt(dot(r1, r1)), R2 = rh_sqrt(dot(r2, r2));
+    double im = sg * dot(r0, c1);
+    double re = R0 * R1 * R2 + dot(r0, r1) * R2 + dot(r0, r2) * R1 + dot(r1, r2) * R0;
+    if (rec[17] > 3.5) {
+        const V3<double> r3 = rel<double>(rec + 9, p);
+        const double R3 = rh_sqrt(dot(r3, r3));
+        const V3<double> c2 = cross(e2, v3(rec[9] - rec[0], rec[10] - rec[1], rec[11] - rec[2]));
+        const double im2 = sg * dot(r0, c2);
+        const double re2 = R0 * R2 * R3 + dot(r0, r2) * R3 + dot(r0, r3) * R2 + dot(r2, r3) * R0;
+        const double pr = re * re2 - im * im2;
+        const double pi = re * im2 + im * re2;
+        re = pr;
+        im = pi;
+    }
+    return 2.0 * rh_atan2_wave(im, re);
+}
+
+// Near test of the FP32 mesh sweep: FP32 offsets carry eps32 of the distance,
+// so a term's relative error grows like eps32 * size / distance.  A wave with
+// a point within kNear32 entity sizes of a vertex (faces: the longest side;
+// edges: the length) evaluates that entity in FP64.  thr: kNear32 sizes, as
+// the record carries it (faces: slot 35, edges: slot 8).
+RH_HD bool mesh32_near(float dmin, float thr)
+{
+    bool near = dmin < thr;
+#ifdef __HIP_DEVICE_COMPILE__
+    near = __any(near);
+#endif
+    return near;
+}
+
+// L_e of an FP32 sweep edge record in FP64 (vertices in slots 0-5, the FP64
+// length in slot 9).
+RH_HD double mesh_edge_L_near(const double* __restrict__ rec, const double p[3])
+{
+    const MeshVtx<double> a = mesh_vtx<double>(rec, p), b = mesh_vtx<double>(rec + 3, p);
+    return edge_L(a.r, a.d, b.r, b.d, rec[9]);
+}
+
+// A triangle of an FP32 sweep face record in FP64: its solid angle and the
+// sum of L_e w_e over the edges it owns.
+struct MeshTri64 {
+    double om, h0, h1, h2;
+};
+RH_HD MeshTri64 mesh_tri_near(const double* __restrict__ rec, int mask, const double p[3])
+{
+    const MeshVtx<double> v[3] = {mesh_vtx<double>(rec, p), mesh_vtx<double>(rec + 3, p),
+                                  mesh_vtx<double>(rec + 6, p)};
+    // r0 . (r1 x r2) = r0 . ((v1 - v0) x (v2 - v0)), signed by the stored normal
+    const float* f = mesh32(rec, 12);
+    const V3<double> c1 = cross(v3(rec[3] - rec[0], rec[4] - rec[1], rec[5] - rec[2]),
+                                v3(rec[6] - rec[0], rec[7] - rec[1], rec[8] - rec[2]));
+    const double im = (c1.x * f[0] + c1.y * f[1] + c1.z * f[2] >= 0.0) ? dot(v[0].r, c1) : -dot(v[0].r, c1);
+    const double re = v[0].d * v[1].d * v[2].d + dot(v[0].r, v[1].r) * v[2].d + dot(v[0].r, v[2].r) * v[1].d +
+                      dot(v[1].r, v[2].r) * v[0].d;
+    MeshTri64 t{2.0 * rh_atan2_wave(im, re), 0.0, 0.0, 0.0};
+    for (int k = 0; k < 3; k++) {
+        if (!(mask & (1 << k))) continue;
+        const double* e = rec + kMeshFaceEdges + 4 * k;
+        const MeshVtx<double>& a = v[k];
+        const MeshVtx<double>& b = v[(k + 1) % 3];
+        const double L = edge_L(a.r, a.d, b.r, b.d, e[0]);
+        t.h0 += L * e[1];
+        t.h1 += L * e[2];
+        t.h2 += L * e[3];
+    }
+    return t;
+}
+
+// Solid angle of a quadrilateral in FP32; false (om untouched) when the wave
+// is near it.
+RH_HD bool mesh32_quad_omega(const double* __restrict__ rec, const double p[3], float& om)
 {
     const V3<float> r0 = rel<float>(rec, p), r1 = rel<float>(rec + 3, p), r2 = rel<float>(rec + 6, p),
                     r3 = rel<float>(rec + 9, p);
     const float R0 = rh_sqrt(dot(r0, r0)), R1 = rh_sqrt(dot(r1, r1)), R2 = rh_sqrt(dot(r2, r2)),
                 R3 = rh_sqrt(dot(r3, r3));
+    if (mesh32_near(fminf(fminf(R0, R1), fminf(R2, R3)), mesh32(rec, kMeshFaceEdges + 13)[0])) return false;
     const float* f = mesh32(rec, 12);
     const float h = dot(r0, v3(f[0], f[1], f[2]));
     const float im = f[3] * h, im2 = f[4] * h;
     const float re = R0 * R1 * R2 + dot(r0, r1) * R2 + dot(r0, r2) * R1 + dot(r1, r2) * R0;
     const float re2 = R0 * R2 * R3 + dot(r0, r2) * R3 + dot(r0, r3) * R2 + dot(r2, r3) * R0;
-    return 2.0f * rh_atan2_wave(re * im2 + im * re2, re * re2 - im * im2);
+    om = 2.0f * rh_atan2_wave(re * im2 + im * re2, re * re2 - im * im2);
+    return true;
 }
 
 // Plane test of one inside record.

@@ -278,6 +278,56 @@ def test_field_mesh_form(oracle_off):
         model.close()
 
 
+def _near_points(items, d_rel):
+    """Points d_rel element sizes off every face centroid (both sides, along
+    the face normal) and off the midpoint of every edge of each element."""
+    pts = []
+    for (o, _k, _s) in items:
+        V = o.verts
+        size = np.linalg.norm(V - V.mean(0), axis=1).max()
+        for f in o.faces:
+            c = V[f].mean(0)
+            nrm = np.cross(V[f[1]] - V[f[0]], V[f[2]] - V[f[0]])
+            nrm /= np.linalg.norm(nrm)
+            pts += [c + d_rel * size * nrm, c - d_rel * size * nrm]
+            for a, b in zip(f, f[1:] + f[:1]):
+                u = np.cross(V[b] - V[a], [0.3, 0.7, 0.1])
+                pts.append(0.5 * (V[a] + V[b]) + d_rel * size * u / np.linalg.norm(u))
+    return np.array(pts)
+
+
+@pytest.mark.parametrize("form", [2, 1])
+def test_fp32_field_near_elements(oracle_off, form):
+    """The FP32 field sweep evaluates an element (or a mesh edge / face) in
+    FP64 for every wave that holds a point within 1.5 element sizes of it, so
+    the FP32 result stays within 2e-6 of the FP64 one (relative to the field
+    scale) from 1e-1 down to 1e-6 element sizes off faces and edges, where
+    FP32 cancellation alone loses every digit.  Tetrahedra (mesh triangles)
+    and hexahedra (mesh quads), mesh form and per-element form."""
+    print("case: fp32 near elements, form %d" % form, flush=True)
+    rng = np.random.default_rng(77)
+    ids = models.tet_mesh((3, 2, 2), 4.0, jitter=0.1, seed=78)
+    for c in ([16, 2, 3], [19, 2, 3], [16, 6, 3]):
+        ids.append(models.hex_block(c, [3, 4, 6], rng.uniform(-1, 1, 3)))
+    top = rad.ObjCnt(ids)
+    P, items = models.packed(top)
+    M = rng.uniform(-1.5, 1.5, (len(items), 3))
+    model = N.Model(P)
+    if form == 1:
+        model.set_field_form(1)
+    assert model.field_form() == form
+    for d_rel in (1e-1, 1e-3, 1e-5, 1e-6):
+        pts = _near_points(items, d_rel)
+        B64, H64 = model.field(M, pts, precision=N.RH_FP64)
+        B32, H32 = model.field(M, pts, precision=N.RH_MIXED)
+        eb = np.abs(B32 - B64).max() / np.abs(B64).max()
+        eh = np.abs(H32 - H64).max() / np.abs(H64).max()
+        print("distance %.0e sizes, %d points: FP32 error B %.2e, H %.2e" % (d_rel, len(pts), eb, eh),
+              flush=True)
+        assert eb <= 2e-6 and eh <= 2e-6
+    model.close()
+
+
 @pytest.mark.parametrize("t", [1.0, 1e-2, 1e-4, 1e-6, 1e-8])
 def test_sliver_tetrahedron_on_device(oracle_off, t):
     """The reference's flatness sweep (tests/test_sliver_tet_asm.py) on the
