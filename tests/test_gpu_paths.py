@@ -278,14 +278,22 @@ def test_field_mesh_form(oracle_off):
         model.close()
 
 
+def _elem_geometry(o):
+    """Vertices and 0-based faces of a polyhedron or a block."""
+    if isinstance(o, O.RecMag):
+        c = np.array([[x, y, z] for x in (-.5, .5) for y in (-.5, .5) for z in (-.5, .5)])
+        return o.center + c * o.dims, [[v - 1 for v in f] for f in models.HEX_FACES]
+    return o.verts, o.faces
+
+
 def _near_points(items, d_rel):
     """Points d_rel element sizes off every face centroid (both sides, along
     the face normal) and off the midpoint of every edge of each element."""
     pts = []
     for (o, _k, _s) in items:
-        V = o.verts
+        V, faces = _elem_geometry(o)
         size = np.linalg.norm(V - V.mean(0), axis=1).max()
-        for f in o.faces:
+        for f in faces:
             c = V[f].mean(0)
             nrm = np.cross(V[f[1]] - V[f[0]], V[f[2]] - V[f[0]])
             nrm /= np.linalg.norm(nrm)
@@ -296,20 +304,25 @@ def _near_points(items, d_rel):
     return np.array(pts)
 
 
+@pytest.mark.parametrize("mirror", [False, True])
 @pytest.mark.parametrize("form", [2, 1])
-def test_fp32_field_near_elements(oracle_off, form):
+def test_fp32_field_near_elements(oracle_off, form, mirror):
     """The FP32 field sweep evaluates an element (or a mesh edge / face) in
     FP64 for every wave that holds a point within 1.5 element sizes of it, so
     the FP32 result stays within 2e-6 of the FP64 one (relative to the field
     scale) from 1e-1 down to 1e-6 element sizes off faces and edges, where
-    FP32 cancellation alone loses every digit.  Tetrahedra (mesh triangles)
-    and hexahedra (mesh quads), mesh form and per-element form."""
-    print("case: fp32 near elements, form %d" % form, flush=True)
+    FP32 cancellation alone loses every digit.  Tetrahedra (mesh triangles),
+    hexahedra (mesh quads) and a block; mesh form and per-element form;
+    with a mirror symmetry, also next to the mirrored copies."""
+    print("case: fp32 near elements, form %d, mirror %s" % (form, mirror), flush=True)
     rng = np.random.default_rng(77)
     ids = models.tet_mesh((3, 2, 2), 4.0, jitter=0.1, seed=78)
     for c in ([16, 2, 3], [19, 2, 3], [16, 6, 3]):
         ids.append(models.hex_block(c, [3, 4, 6], rng.uniform(-1, 1, 3)))
+    ids.append(rad.ObjRecMag([3, 13, 4], [4, 3, 5], list(rng.uniform(-1, 1, 3))))
     top = rad.ObjCnt(ids)
+    if mirror:
+        rad.TrfZerPara(top, [0, 0, -1], [0, 0, 1])  # copies in z' = -2 - z
     P, items = models.packed(top)
     M = rng.uniform(-1.5, 1.5, (len(items), 3))
     model = N.Model(P)
@@ -318,6 +331,8 @@ def test_fp32_field_near_elements(oracle_off, form):
     assert model.field_form() == form
     for d_rel in (1e-1, 1e-3, 1e-5, 1e-6):
         pts = _near_points(items, d_rel)
+        if mirror:
+            pts = np.concatenate([pts, pts * [1, 1, -1] - [0, 0, 2]])
         B64, H64 = model.field(M, pts, precision=N.RH_FP64)
         B32, H32 = model.field(M, pts, precision=N.RH_MIXED)
         eb = np.abs(B32 - B64).max() / np.abs(B64).max()
