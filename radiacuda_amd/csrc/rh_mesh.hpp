@@ -18,15 +18,17 @@
 namespace rh {
 
 // Charged edge record: a(3), b(3), t(3), len, w(3), cont (1: a is the
-// previous record's b).
+// previous record's b).  The FP32 sweep's records hold t(3), len and the
+// near distance as floats in slots 6-8.
 constexpr int kMeshEdge = 14;
 // Charged face record: v0..v3 (12), n(3), 2*area(v0,v1,v2), 2*area(v0,v2,v3),
 // nv, g(3), code (0 triangle, 1 triangle whose v0, v1 are the previous
 // record's v1, v2, 2 quadrilateral), then the edges the triangle owns: for
 // the sides (v0,v1), (v1,v2), (v2,v0) in turn len and w(3) (slots 22-33), and
-// the mask of owned sides as an integer in the low word of slot 34.  A
-// quadrilateral is CCW about n; a triangle's vertices may come in either
-// order (its solid angle below uses n and the area, not the order).
+// the mask of owned sides as an integer in the low word of slot 34; slot 35:
+// the FP32 sweep's near distance (a float).  A quadrilateral is CCW about n;
+// a triangle's vertices may come in either order (its solid angle below uses
+// n and the area, not the order).
 constexpr int kMeshFace = 36;
 constexpr int kMeshFaceEdges = 22;
 // Inside-test record: nplanes, M(3), up to 8 planes (n(3), d) with outward n:
