@@ -81,6 +81,9 @@ RH_HD void block_Q(const ModelView& mv, int64_t j, const double x[3], double q9[
 // within kNearRadii circumradii of the element's centre (blocks: the half
 // diagonal about the centre; tetrahedra: the farthest vertex from the vertex
 // mean).  General polyhedra always count as near.
+// The near tests spell out their fused multiply-adds (sq3), so that two
+// kernels that must take the same decision (the FP32 field sweep's far and
+// near passes) compute the same bits whatever the compiler contracts.
 constexpr double kNearRadii = 2.0;
 template <int KIND>
 RH_HD bool near_element(int kind, const double* __restrict__ rec, const double x[3])
@@ -88,19 +91,19 @@ RH_HD bool near_element(int kind, const double* __restrict__ rec, const double x
     const int K = (KIND >= 0) ? KIND : (kind == EK_REC ? 0 : (kind == EK_TET ? 1 : 2));
     if (K == 0) {
         const double dx = x[0] - rec[0], dy = x[1] - rec[1], dz = x[2] - rec[2];
-        const double r2 = rec[3] * rec[3] + rec[4] * rec[4] + rec[5] * rec[5];
-        return dx * dx + dy * dy + dz * dz < kNearRadii * kNearRadii * r2;
+        const double r2 = sq3(rec[3], rec[4], rec[5]);
+        return sq3(dx, dy, dz) < kNearRadii * kNearRadii * r2;
     }
     if (K == 1) {
-        double c[3];
-        for (int k = 0; k < 3; k++) c[k] = 0.25 * (rec[k] + rec[3 + k] + rec[6 + k] + rec[9 + k]);
+        // offsets from the vertex mean 0.25 s as fma(-0.25, s, .)
+        double s4[3];
+        for (int k = 0; k < 3; k++) s4[k] = (rec[k] + rec[3 + k]) + (rec[6 + k] + rec[9 + k]);
         double r2 = 0.0;
-        for (int v = 0; v < 4; v++) {
-            const double a = rec[3 * v] - c[0], b = rec[3 * v + 1] - c[1], d = rec[3 * v + 2] - c[2];
-            r2 = fmax(r2, a * a + b * b + d * d);
-        }
-        const double dx = x[0] - c[0], dy = x[1] - c[1], dz = x[2] - c[2];
-        return dx * dx + dy * dy + dz * dz < kNearRadii * kNearRadii * r2;
+        for (int v = 0; v < 4; v++)
+            r2 = fmax(r2, sq3(fma(-0.25, s4[0], rec[3 * v]), fma(-0.25, s4[1], rec[3 * v + 1]),
+                              fma(-0.25, s4[2], rec[3 * v + 2])));
+        return sq3(fma(-0.25, s4[0], x[0]), fma(-0.25, s4[1], x[1]), fma(-0.25, s4[2], x[2])) <
+               kNearRadii * kNearRadii * r2;
     }
     return true;
 }
@@ -160,10 +163,7 @@ RH_HD bool near_field(int kind, const double* __restrict__ rec, const double x[3
     const double* vt = rec + kGenHdr;
     const double* ed = vt + 3 * nv;
     double d2 = 1e300, l = 0.0;
-    for (int v = 0; v < nv; v++) {
-        const double a = vt[3 * v] - x[0], b = vt[3 * v + 1] - x[1], c = vt[3 * v + 2] - x[2];
-        d2 = fmin(d2, a * a + b * b + c * c);
-    }
+    for (int v = 0; v < nv; v++) d2 = fmin(d2, sq3(vt[3 * v] - x[0], vt[3 * v + 1] - x[1], vt[3 * v + 2] - x[2]));
     for (int e = 0; e < ne; e++) l = fmax(l, ed[kGenEdge * e + 5]);
     const double t = double(kNear32) * l;
     return d2 < t * t;

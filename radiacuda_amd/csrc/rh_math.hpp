@@ -71,6 +71,10 @@ template <class R>
 RH_HD V3<R> v3(R x, R y, R z) { return V3<R>{x, y, z}; }
 template <class R>
 RH_HD R dot(const V3<R>& a, const V3<R>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// a^2 + b^2 + c^2 with its fused multiply-adds written out (the same bits in
+// every kernel)
+RH_HD double sq3(double a, double b, double c) { return fma(a, a, fma(b, b, c * c)); }
+RH_HD float sq3(float a, float b, float c) { return fmaf(a, a, fmaf(b, b, c * c)); }
 template <class R>
 RH_HD V3<R> cross(const V3<R>& a, const V3<R>& b)
 {
@@ -698,7 +702,7 @@ RH_HD bool tet_near32(const double* __restrict__ rec, const double p[3])
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         const V3<float> r = rel<float>(rec + 3 * v, p);
-        d = fminf(d, rh_sqrt(dot(r, r)));
+        d = fminf(d, rh_sqrt(sq3(r.x, r.y, r.z)));
     }
     bool near = d < kNear32 * float(rec[kChargedM + 3]);
 #ifdef __HIP_DEVICE_COMPILE__
