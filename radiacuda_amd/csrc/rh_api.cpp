@@ -447,6 +447,7 @@ rh_status rh_model_create(const rh_model_desc* desc, int device, void* stream, r
         if (e == hipSuccess) e = up(&md.face_cont, F.face_cont);
         if (e == hipSuccess) e = up(&md.face_edge, F.face_edge);
         md.ne_free = F.ne_free;
+        md.nf_tri = F.nf_tri;
         double* ch = nullptr;
         if (e == hipSuccess) e = dalloc(&ch, size_t(rh::kMeshEdge * md.ne + rh::kMeshFace * md.nf +
                                                     rh::kMeshIn * md.ni) + 2);

@@ -661,6 +661,7 @@ void build_mesh_form(PackedModel& P)
                 q = r;
             }
         }
+        M.nf_tri = int64_t(forder.size());
         for (size_t f = 0; f < floop.size(); f++)
             if (floop[f].size() != 3) {
                 forder.push_back(int32_t(f));
@@ -668,16 +669,17 @@ void build_mesh_form(PackedModel& P)
                 fcont.push_back(2);  // quadrilateral
             }
     }
-    // Each unique edge that is a side of a triangle is owned by the first
-    // triangle in sweep order that has it: that triangle's sweep step already
+    // Each unique edge that is a side of a face is owned by the first face in
+    // sweep order that has it (a triangle any of its sides, a quadrilateral
+    // its sides (v0,v1), (v1,v2), (v2,v3)): that face's sweep step already
     // holds both vertex offsets, distances and their dot product.
     std::vector<std::pair<int32_t, int32_t>> ekeys(einc.size());
     for (const auto& kv : eid) ekeys[size_t(kv.second)] = kv.first;
     std::vector<int64_t> owner(einc.size(), -1);  // 3 * face position + side
     for (size_t o = 0; o < forder.size(); o++) {
-        if (fcont[o] == 2) continue;
+        const size_t nv = fverts[o].size();
         for (int k = 0; k < 3; k++) {
-            const int32_t a = fverts[o][size_t(k)], b = fverts[o][size_t((k + 1) % 3)];
+            const int32_t a = fverts[o][size_t(k)], b = fverts[o][size_t(k + 1) % nv];
             const auto it = eid.find({std::min(a, b), std::max(a, b)});
             if (it == eid.end()) continue;  // a diagonal of a fanned polygon
             if (owner[size_t(it->second)] < 0) owner[size_t(it->second)] = int64_t(3 * o + k);
@@ -699,7 +701,7 @@ void build_mesh_form(PackedModel& P)
             return next[size_t(v)] < a.size() ? a[next[size_t(v)]] : -1;
         };
         for (size_t e = 0; e < ekeys.size(); e++)
-            if (owner[e] >= 0) used[e] = 1;  // evaluated by its triangle
+            if (owner[e] >= 0) used[e] = 1;  // evaluated by its face
         for (size_t e0 = 0; e0 < ekeys.size(); e0++) {
             if (used[e0]) continue;
             int32_t cur = ekeys[e0].first, e = int32_t(e0);
@@ -743,7 +745,7 @@ void build_mesh_form(PackedModel& P)
         }
         M.edge_inc_off.push_back(int64_t(M.edge_inc_elem.size()));
     }
-    // owned edges: record index per triangle side
+    // owned edges: record index per face side
     M.face_edge.assign(3 * forder.size(), -1);
     for (size_t o = 0; o < eorder.size(); o++) {
         const int64_t w = owner[size_t(eorder[o])];

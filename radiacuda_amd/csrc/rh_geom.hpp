@@ -35,12 +35,14 @@ struct MeshForm {
     std::vector<double> edge_cont;       // 1: the edge starts at the previous edge's end
     std::vector<double> face_cont;       // 0: triangle, 1: triangle whose vertices 0, 1 are the
                                          // previous face's 1, 2; 2: quadrilateral
-    // Edges owned by triangles: edge records [ne_free, ne) are evaluated by
-    // the triangle holding both their vertices (face_edge: per face, the
-    // record index of the edge on sides (v0,v1), (v1,v2), (v2,v0), or -1);
-    // records [0, ne_free) are swept on their own.
+    // Edges owned by faces: edge records [ne_free, ne) are evaluated by the
+    // face holding both their vertices (face_edge: per face, the record index
+    // of the edge on sides (v0,v1), (v1,v2), (v2,v0) of a triangle or (v0,v1),
+    // (v1,v2), (v2,v3) of a quadrilateral, or -1); records [0, ne_free) are
+    // swept on their own.
     int64_t ne_free = 0;
     std::vector<int64_t> face_edge;      // 3 per face
+    int64_t nf_tri = 0;                  // faces [0, nf_tri) are triangles, the rest quadrilaterals
     int64_t n_edges() const { return int64_t(edge_inc_off.empty() ? 0 : edge_inc_off.size() - 1); }
     int64_t n_faces() const { return int64_t(face_inc_off.empty() ? 0 : face_inc_off.size() - 1); }
     int64_t n_in() const { return int64_t(in_elem.size()); }

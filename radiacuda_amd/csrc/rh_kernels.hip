@@ -606,75 +606,89 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
     {
         MeshVtx<R> A{}, Bv{}, C{};
         R dAB = R(0), dBC = R(0), dCA = R(0);
+        // this slice's faces: triangles [f0, ft), then quadrilaterals [fq, f1)
         const int64_t f0 = md.nf * y / S, f1 = md.nf * (y + 1) / S;
-        // an edge owned by the current triangle: the offsets and distances of
-        // its ends and their dot product are already at hand
+        const int64_t ft = f1 < md.nf_tri ? f1 : md.nf_tri, fq = f0 > md.nf_tri ? f0 : md.nf_tri;
+        // an edge owned by the current face: the offsets and distances of its
+        // ends and their dot product are already at hand
         auto owned = [&](const MeshVtx<R>& u, const MeshVtx<R>& v, R duv, const double* e) {
             const double L = double(edge_L(u.r, u.d, v.r, v.d, duv, R(e[0])));
             h[0] += L * e[1];
             h[1] += L * e[2];
             h[2] += L * e[3];
         };
-        auto face = [&](int64_t g, int q, MeshVtx<R>& a, MeshVtx<R>& b, MeshVtx<R>& c, R& dab, R& dbc) {
+        // FP64 for the records of the tile the wave is near
+        auto deferred = [&]() {
+            if constexpr (sizeof(R) == 4) {
+                for (; defer; defer &= defer - 1) {
+                    const double* rec = tile + kMeshFace * __builtin_ctzll(defer);
+                    const MeshFace64 t = mesh_face_near(rec, mesh_mask(rec + kMeshFaceEdges + 12), x);
+                    h[0] += t.h0 - t.om * rec[18];
+                    h[1] += t.h1 - t.om * rec[19];
+                    h[2] += t.h2 - t.om * rec[20];
+                }
+            }
+        };
+        auto tri = [&](int64_t g, int q, MeshVtx<R>& a, MeshVtx<R>& b, MeshVtx<R>& c, R& dab, R& dbc) {
             const double* rec = tile + kMeshFace * q;
-            const int code = mesh_code(rec + 21);
+            if (!(mesh_code(rec + 21) == 1 && g != f0)) {
+                a = mesh_vtx<R>(rec, x);
+                b = mesh_vtx<R>(rec + 3, x);
+                dab = dot(a.r, b.r);
+            }
+            c = mesh_vtx<R>(rec + 6, x);
+            dbc = dot(b.r, c.r);
+            double om;
             if constexpr (sizeof(R) == 4) {
                 // a near record only sets its bit; the carried values
                 // (b, c, dbc) are in place either way
-                float om;
-                if (code == 2) {
-                    if (!mesh32_quad_omega(rec, x, om)) {
-                        defer |= uint64_t(1) << q;
-                        return;
-                    }
-                } else {
-                    if (!(code == 1 && g != f0)) {
-                        a = mesh_vtx<R>(rec, x);
-                        b = mesh_vtx<R>(rec + 3, x);
-                        dab = dot(a.r, b.r);
-                    }
-                    c = mesh_vtx<R>(rec + 6, x);
-                    dbc = dot(b.r, c.r);
-                    if (mesh32_near(fminf(fminf(a.d, b.d), c.d), mesh32(rec, kMeshFaceEdges + 13)[0])) {
-                        defer |= uint64_t(1) << q;
-                        return;
-                    }
-                    const R dca = dot(c.r, a.r);
-                    const int mask = mesh_mask(rec + kMeshFaceEdges + 12);
-                    om = mesh32_tri_omega(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, dca, rec);
-                    if (mask & 1) owned(a, b, dab, rec + kMeshFaceEdges);
-                    if (mask & 2) owned(b, c, dbc, rec + kMeshFaceEdges + 4);
-                    if (mask & 4) owned(c, a, dca, rec + kMeshFaceEdges + 8);
+                if (mesh32_near(fminf(fminf(a.d, b.d), c.d), mesh32(rec, kMeshFaceEdges + 13)[0])) {
+                    defer |= uint64_t(1) << q;
+                    return;
                 }
-                h[0] -= double(om) * rec[18];
-                h[1] -= double(om) * rec[19];
-                h[2] -= double(om) * rec[20];
+                const R dca = dot(c.r, a.r);
+                const int mask = mesh_mask(rec + kMeshFaceEdges + 12);
+                om = double(mesh32_tri_omega(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, dca, rec));
+                if (mask & 1) owned(a, b, dab, rec + kMeshFaceEdges);
+                if (mask & 2) owned(b, c, dbc, rec + kMeshFaceEdges + 4);
+                if (mask & 4) owned(c, a, dca, rec + kMeshFaceEdges + 8);
             } else {
-                double om;
-                if (code == 2) {
-                    om = mesh_face_omega<R>(rec, x);
-                } else {
-                    if (!(code == 1 && g != f0)) {
-                        a = mesh_vtx<R>(rec, x);
-                        b = mesh_vtx<R>(rec + 3, x);
-                        dab = dot(a.r, b.r);
-                    }
-                    c = mesh_vtx<R>(rec + 6, x);
-                    dbc = dot(b.r, c.r);
-                    const R dca = dot(c.r, a.r);
-                    const int mask = mesh_mask(rec + kMeshFaceEdges + 12);
-                    om = mesh_tri_omega_from<R>(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, dca, rec);
-                    if (mask & 1) owned(a, b, dab, rec + kMeshFaceEdges);
-                    if (mask & 2) owned(b, c, dbc, rec + kMeshFaceEdges + 4);
-                    if (mask & 4) owned(c, a, dca, rec + kMeshFaceEdges + 8);
-                }
-                h[0] -= om * rec[18];
-                h[1] -= om * rec[19];
-                h[2] -= om * rec[20];
+                const R dca = dot(c.r, a.r);
+                const int mask = mesh_mask(rec + kMeshFaceEdges + 12);
+                om = mesh_tri_omega_from<R>(a.r, a.d, b.r, b.d, c.r, c.d, dab, dbc, dca, rec);
+                if (mask & 1) owned(a, b, dab, rec + kMeshFaceEdges);
+                if (mask & 2) owned(b, c, dbc, rec + kMeshFaceEdges + 4);
+                if (mask & 4) owned(c, a, dca, rec + kMeshFaceEdges + 8);
             }
+            h[0] -= om * rec[18];
+            h[1] -= om * rec[19];
+            h[2] -= om * rec[20];
         };
-        for (int64_t base = f0; base < f1; base += kStage) {
-            const int cnt = int((f1 - base < kStage) ? (f1 - base) : kStage);
+        // a quadrilateral with the sides (v0,v1), (v1,v2), (v2,v3) it owns
+        auto quad = [&](int q) {
+            const double* rec = tile + kMeshFace * q;
+            const MeshVtx<R> v[4] = {mesh_vtx<R>(rec, x), mesh_vtx<R>(rec + 3, x), mesh_vtx<R>(rec + 6, x),
+                                     mesh_vtx<R>(rec + 9, x)};
+            if constexpr (sizeof(R) == 4) {
+                if (mesh32_near(fminf(fminf(v[0].d, v[1].d), fminf(v[2].d, v[3].d)),
+                                mesh32(rec, kMeshFaceEdges + 13)[0])) {
+                    defer |= uint64_t(1) << q;
+                    return;
+                }
+            }
+            const R d01 = dot(v[0].r, v[1].r), d12 = dot(v[1].r, v[2].r), d23 = dot(v[2].r, v[3].r);
+            const double om =
+                double(mesh_quad_omega_from<R>(v, d01, dot(v[0].r, v[2].r), dot(v[0].r, v[3].r), d12, d23, rec));
+            const int mask = mesh_mask(rec + kMeshFaceEdges + 12);
+            if (mask & 1) owned(v[0], v[1], d01, rec + kMeshFaceEdges);
+            if (mask & 2) owned(v[1], v[2], d12, rec + kMeshFaceEdges + 4);
+            if (mask & 4) owned(v[2], v[3], d23, rec + kMeshFaceEdges + 8);
+            h[0] -= om * rec[18];
+            h[1] -= om * rec[19];
+            h[2] -= om * rec[20];
+        };
+        for (int64_t base = f0; base < ft; base += kStage) {
+            const int cnt = int((ft - base < kStage) ? (ft - base) : kStage);
             const double2* src = reinterpret_cast<const double2*>(md.ch_face + kMeshFace * base);
             __syncthreads();
             for (int i = threadIdx.x; i < cnt * (kMeshFace / 2); i += 256) dst[i] = src[i];
@@ -683,15 +697,15 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
             int q = 0;
 #pragma unroll 1
             for (; q + 3 <= cnt; q += 3) {
-                face(base + q, q, A, Bv, C, dAB, dBC);
-                face(base + q + 1, q + 1, Bv, C, A, dBC, dCA);
-                face(base + q + 2, q + 2, C, A, Bv, dCA, dAB);
+                tri(base + q, q, A, Bv, C, dAB, dBC);
+                tri(base + q + 1, q + 1, Bv, C, A, dBC, dCA);
+                tri(base + q + 2, q + 2, C, A, Bv, dCA, dAB);
             }
             // remainder, then the carried pair back into (A, B)
             if (q < cnt) {
-                face(base + q, q, A, Bv, C, dAB, dBC);
+                tri(base + q, q, A, Bv, C, dAB, dBC);
                 if (q + 1 < cnt) {
-                    face(base + q + 1, q + 1, Bv, C, A, dBC, dCA);
+                    tri(base + q + 1, q + 1, Bv, C, A, dBC, dCA);
                     Bv = A;
                     A = C;
                     dAB = dCA;
@@ -701,24 +715,18 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
                     dAB = dBC;
                 }
             }
-            if constexpr (sizeof(R) == 4) {
-                for (; defer; defer &= defer - 1) {
-                    const double* rec = tile + kMeshFace * __builtin_ctzll(defer);
-                    double om;
-                    if (mesh_code(rec + 21) == 2) {
-                        om = mesh_face_omega_near(rec, x);
-                    } else {
-                        const MeshTri64 t = mesh_tri_near(rec, mesh_mask(rec + kMeshFaceEdges + 12), x);
-                        om = t.om;
-                        h[0] += t.h0;
-                        h[1] += t.h1;
-                        h[2] += t.h2;
-                    }
-                    h[0] -= om * rec[18];
-                    h[1] -= om * rec[19];
-                    h[2] -= om * rec[20];
-                }
-            }
+            deferred();
+        }
+        for (int64_t base = fq; base < f1; base += kStage) {
+            const int cnt = int((f1 - base < kStage) ? (f1 - base) : kStage);
+            const double2* src = reinterpret_cast<const double2*>(md.ch_face + kMeshFace * base);
+            __syncthreads();
+            for (int i = threadIdx.x; i < cnt * (kMeshFace / 2); i += 256) dst[i] = src[i];
+            __syncthreads();
+            if (!live) continue;
+#pragma unroll 1
+            for (int q = 0; q < cnt; q++) quad(q);
+            deferred();
         }
     }
     // inside tests: B gets the magnetization of the element holding the point

@@ -29,6 +29,7 @@ struct MeshDev {
     const double* edge_cont = nullptr;
     const double* face_cont = nullptr;
     int64_t ne_free = 0;                // edges [0, ne_free) are swept on their own
+    int64_t nf_tri = 0;                 // faces [0, nf_tri) are triangles, the rest quadrilaterals
     const int64_t* face_edge = nullptr;  // 3 per face: owned edge record or -1
     double* ch_edge = nullptr;
     double* ch_face = nullptr;

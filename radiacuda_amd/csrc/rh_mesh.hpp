@@ -190,29 +190,21 @@ RH_HD double mesh_edge_L_near(const double* __restrict__ rec, const double p[3])
     return edge_L(a.r, a.d, b.r, b.d, rec[9]);
 }
 
-// A triangle of an FP32 sweep face record in FP64: its solid angle and the
-// sum of L_e w_e over the edges it owns.
-struct MeshTri64 {
+// A face of an FP32 sweep record in FP64 (triangle or quadrilateral): its
+// solid angle and the sum of L_e w_e over the sides it owns.
+struct MeshFace64 {
     double om, h0, h1, h2;
 };
-RH_HD MeshTri64 mesh_tri_near(const double* __restrict__ rec, int mask, const double p[3])
+RH_HD MeshFace64 mesh_face_near(const double* __restrict__ rec, int mask, const double p[3])
 {
-    const MeshVtx<double> v[3] = {mesh_vtx<double>(rec, p), mesh_vtx<double>(rec + 3, p),
-                                  mesh_vtx<double>(rec + 6, p)};
-    // r0 . (r1 x r2) = r0 . ((v1 - v0) x (v2 - v0)), signed by the stored normal
-    const float* f = mesh32(rec, 12);
-    const V3<double> c1 = cross(v3(rec[3] - rec[0], rec[4] - rec[1], rec[5] - rec[2]),
-                                v3(rec[6] - rec[0], rec[7] - rec[1], rec[8] - rec[2]));
-    const double im = (c1.x * f[0] + c1.y * f[1] + c1.z * f[2] >= 0.0) ? dot(v[0].r, c1) : -dot(v[0].r, c1);
-    const double re = v[0].d * v[1].d * v[2].d + dot(v[0].r, v[1].r) * v[2].d + dot(v[0].r, v[2].r) * v[1].d +
-                      dot(v[1].r, v[2].r) * v[0].d;
-    MeshTri64 t{2.0 * rh_atan2_wave(im, re), 0.0, 0.0, 0.0};
+    const int nv = rec[17] > 3.5 ? 4 : 3;
+    MeshFace64 t{mesh_face_omega_near(rec, p), 0.0, 0.0, 0.0};
     for (int k = 0; k < 3; k++) {
         if (!(mask & (1 << k))) continue;
         const double* e = rec + kMeshFaceEdges + 4 * k;
-        const MeshVtx<double>& a = v[k];
-        const MeshVtx<double>& b = v[(k + 1) % 3];
-        const double L = edge_L(a.r, a.d, b.r, b.d, e[0]);
+        const int k1 = (k + 1 == nv) ? 0 : k + 1;
+        const MeshVtx<double> a = mesh_vtx<double>(rec + 3 * k, p), b = mesh_vtx<double>(rec + 3 * k1, p);
+        const double L = edge_L(a.r, a.d, b.r, b.d, dot(a.r, b.r), e[0]);
         t.h0 += L * e[1];
         t.h1 += L * e[2];
         t.h2 += L * e[3];
@@ -220,22 +212,26 @@ RH_HD MeshTri64 mesh_tri_near(const double* __restrict__ rec, int mask, const do
     return t;
 }
 
-// Solid angle of a quadrilateral in FP32; false (om untouched) when the wave
-// is near it.
-RH_HD bool mesh32_quad_omega(const double* __restrict__ rec, const double p[3], float& om)
+// A quadrilateral face record at p from its vertex offsets, distances and
+// dot products: the signed solid angle (fans (v0,v1,v2) and (v0,v2,v3)
+// multiplied as complex numbers).  FP64 records hold n and the doubled fan
+// areas in slots 12-16, FP32 records as floats in slots 12-14.
+template <class R>
+RH_HD R mesh_quad_omega_from(const MeshVtx<R> v[4], R d01, R d02, R d03, R d12, R d23,
+                             const double* __restrict__ rec)
 {
-    const V3<float> r0 = rel<float>(rec, p), r1 = rel<float>(rec + 3, p), r2 = rel<float>(rec + 6, p),
-                    r3 = rel<float>(rec + 9, p);
-    const float R0 = rh_sqrt(dot(r0, r0)), R1 = rh_sqrt(dot(r1, r1)), R2 = rh_sqrt(dot(r2, r2)),
-                R3 = rh_sqrt(dot(r3, r3));
-    if (mesh32_near(fminf(fminf(R0, R1), fminf(R2, R3)), mesh32(rec, kMeshFaceEdges + 13)[0])) return false;
-    const float* f = mesh32(rec, 12);
-    const float h = dot(r0, v3(f[0], f[1], f[2]));
-    const float im = f[3] * h, im2 = f[4] * h;
-    const float re = R0 * R1 * R2 + dot(r0, r1) * R2 + dot(r0, r2) * R1 + dot(r1, r2) * R0;
-    const float re2 = R0 * R2 * R3 + dot(r0, r2) * R3 + dot(r0, r3) * R2 + dot(r2, r3) * R0;
-    om = 2.0f * rh_atan2_wave(re * im2 + im * re2, re * re2 - im * im2);
-    return true;
+    R n0, n1, n2, a1, a2;
+    if constexpr (sizeof(R) == 4) {
+        const float* f = mesh32(rec, 12);
+        n0 = f[0], n1 = f[1], n2 = f[2], a1 = f[3], a2 = f[4];
+    } else {
+        n0 = R(rec[12]), n1 = R(rec[13]), n2 = R(rec[14]), a1 = R(rec[15]), a2 = R(rec[16]);
+    }
+    const R h = dot(v[0].r, v3(n0, n1, n2));
+    const R im = a1 * h, im2 = a2 * h;
+    const R re = v[0].d * v[1].d * v[2].d + d01 * v[2].d + d02 * v[1].d + d12 * v[0].d;
+    const R re2 = v[0].d * v[2].d * v[3].d + d02 * v[3].d + d03 * v[2].d + d23 * v[0].d;
+    return R(2) * rh_atan2_wave(re * im2 + im * re2, re * re2 - im * im2);
 }
 
 // Plane test of one inside record.
