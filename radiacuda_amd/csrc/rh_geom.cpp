@@ -662,11 +662,58 @@ void build_mesh_form(PackedModel& P)
             }
         }
         M.nf_tri = int64_t(forder.size());
+        // Quadrilateral strips: where a strip continues, a quadrilateral's
+        // first two vertices are the previous one's last two, its loop read
+        // from there in either direction.
+        std::map<std::pair<int32_t, int32_t>, std::vector<int32_t>> quad_at;
+        for (size_t f = 0; f < floop.size(); f++) {
+            if (floop[f].size() != 4) continue;
+            for (int k = 0; k < 4; k++) {
+                const int32_t a = floop[f][size_t(k)], b = floop[f][size_t((k + 1) % 4)];
+                quad_at[{std::min(a, b), std::max(a, b)}].push_back(int32_t(f));
+            }
+        }
+        auto next_quad = [&](int32_t a, int32_t b) -> int32_t {
+            auto it = quad_at.find({std::min(a, b), std::max(a, b)});
+            if (it == quad_at.end()) return -1;
+            for (int32_t g : it->second)
+                if (!used[size_t(g)]) return g;
+            return -1;
+        };
+        for (size_t f0 = 0; f0 < floop.size(); f0++) {
+            if (used[f0] || floop[f0].size() != 4) continue;
+            std::vector<int32_t> v = floop[f0];
+            used[f0] = 1;
+            // start with the rotation whose side (v2, v3) continues the strip
+            for (int r = 0; r < 4; r++) {
+                if (next_quad(v[2], v[3]) >= 0) break;
+                std::rotate(v.begin(), v.begin() + 1, v.end());
+            }
+            forder.push_back(int32_t(f0));
+            fverts.push_back(v);
+            fcont.push_back(2);  // quadrilateral
+            int32_t p = v[2], q = v[3];
+            for (int32_t g = next_quad(p, q); g >= 0; g = next_quad(p, q)) {
+                used[size_t(g)] = 1;
+                const std::vector<int32_t>& l = floop[size_t(g)];
+                int i = 0;
+                while (l[size_t(i)] != p) i++;
+                // around the loop from p through q
+                const int dir = (l[size_t((i + 1) % 4)] == q) ? 1 : 3;
+                std::vector<int32_t> w(4);
+                for (int k = 0; k < 4; k++) w[size_t(k)] = l[size_t((i + dir * k) % 4)];
+                forder.push_back(g);
+                fverts.push_back(w);
+                fcont.push_back(3);  // quadrilateral continuing the strip
+                p = w[2];
+                q = w[3];
+            }
+        }
         for (size_t f = 0; f < floop.size(); f++)
-            if (floop[f].size() != 3) {
+            if (!used[f]) {
                 forder.push_back(int32_t(f));
                 fverts.push_back(floop[f]);
-                fcont.push_back(2);  // quadrilateral
+                fcont.push_back(2);
             }
     }
     // Each unique edge that is a side of a face is owned by the first face in

@@ -604,8 +604,8 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
     // triangle also evaluates the edges it owns (+ L_e w_e) from the
     // offsets, distances and dot products of its vertices.
     {
-        MeshVtx<R> A{}, Bv{}, C{};
-        R dAB = R(0), dBC = R(0), dCA = R(0);
+        MeshVtx<R> A{}, Bv{}, C{}, QA{}, QB{}, QC{}, QD{};
+        R dAB = R(0), dBC = R(0), dCA = R(0), dQAB = R(0), dQCD = R(0);
         // this slice's faces: triangles [f0, ft), then quadrilaterals [fq, f1)
         const int64_t f0 = md.nf * y / S, f1 = md.nf * (y + 1) / S;
         const int64_t ft = f1 < md.nf_tri ? f1 : md.nf_tri, fq = f0 > md.nf_tri ? f0 : md.nf_tri;
@@ -664,25 +664,34 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
             h[1] -= om * rec[19];
             h[2] -= om * rec[20];
         };
-        // a quadrilateral with the sides (v0,v1), (v1,v2), (v2,v3) it owns
-        auto quad = [&](int q) {
+        // a quadrilateral with the sides (v0,v1), (v1,v2), (v2,v3) it owns,
+        // along strips: a continued quadrilateral takes v0, v1 and their dot
+        // product from the previous one's v2, v3 (slots a, b, dab)
+        auto quad = [&](int64_t g, int q, MeshVtx<R>& a, MeshVtx<R>& b, MeshVtx<R>& c, MeshVtx<R>& d, R& dab,
+                        R& dcd) {
             const double* rec = tile + kMeshFace * q;
-            const MeshVtx<R> v[4] = {mesh_vtx<R>(rec, x), mesh_vtx<R>(rec + 3, x), mesh_vtx<R>(rec + 6, x),
-                                     mesh_vtx<R>(rec + 9, x)};
+            if (!(mesh_code(rec + 21) == 3 && g != fq)) {
+                a = mesh_vtx<R>(rec, x);
+                b = mesh_vtx<R>(rec + 3, x);
+                dab = dot(a.r, b.r);
+            }
+            c = mesh_vtx<R>(rec + 6, x);
+            d = mesh_vtx<R>(rec + 9, x);
+            dcd = dot(c.r, d.r);
             if constexpr (sizeof(R) == 4) {
-                if (mesh32_near(fminf(fminf(v[0].d, v[1].d), fminf(v[2].d, v[3].d)),
-                                mesh32(rec, kMeshFaceEdges + 13)[0])) {
+                if (mesh32_near(fminf(fminf(a.d, b.d), fminf(c.d, d.d)), mesh32(rec, kMeshFaceEdges + 13)[0])) {
                     defer |= uint64_t(1) << q;
                     return;
                 }
             }
-            const R d01 = dot(v[0].r, v[1].r), d12 = dot(v[1].r, v[2].r), d23 = dot(v[2].r, v[3].r);
+            const MeshVtx<R> v[4] = {a, b, c, d};
+            const R dbc = dot(b.r, c.r);
             const double om =
-                double(mesh_quad_omega_from<R>(v, d01, dot(v[0].r, v[2].r), dot(v[0].r, v[3].r), d12, d23, rec));
+                double(mesh_quad_omega_from<R>(v, dab, dot(a.r, c.r), dot(a.r, d.r), dbc, dcd, rec));
             const int mask = mesh_mask(rec + kMeshFaceEdges + 12);
-            if (mask & 1) owned(v[0], v[1], d01, rec + kMeshFaceEdges);
-            if (mask & 2) owned(v[1], v[2], d12, rec + kMeshFaceEdges + 4);
-            if (mask & 4) owned(v[2], v[3], d23, rec + kMeshFaceEdges + 8);
+            if (mask & 1) owned(a, b, dab, rec + kMeshFaceEdges);
+            if (mask & 2) owned(b, c, dbc, rec + kMeshFaceEdges + 4);
+            if (mask & 4) owned(c, d, dcd, rec + kMeshFaceEdges + 8);
             h[0] -= om * rec[18];
             h[1] -= om * rec[19];
             h[2] -= om * rec[20];
@@ -724,8 +733,19 @@ k_field_mesh(DevModel m, const double* __restrict__ pts, int64_t np, int64_t S, 
             for (int i = threadIdx.x; i < cnt * (kMeshFace / 2); i += 256) dst[i] = src[i];
             __syncthreads();
             if (!live) continue;
+            int q = 0;
 #pragma unroll 1
-            for (int q = 0; q < cnt; q++) quad(q);
+            for (; q + 2 <= cnt; q += 2) {
+                quad(base + q, q, QA, QB, QC, QD, dQAB, dQCD);
+                quad(base + q + 1, q + 1, QC, QD, QA, QB, dQCD, dQAB);
+            }
+            // remainder, then the carried pair back into (QA, QB)
+            if (q < cnt) {
+                quad(base + q, q, QA, QB, QC, QD, dQAB, dQCD);
+                QA = QC;
+                QB = QD;
+                dQAB = dQCD;
+            }
             deferred();
         }
     }

@@ -23,12 +23,14 @@ namespace rh {
 constexpr int kMeshEdge = 14;
 // Charged face record: v0..v3 (12), n(3), 2*area(v0,v1,v2), 2*area(v0,v2,v3),
 // nv, g(3), code (0 triangle, 1 triangle whose v0, v1 are the previous
-// record's v1, v2, 2 quadrilateral), then the edges the triangle owns: for
-// the sides (v0,v1), (v1,v2), (v2,v0) in turn len and w(3) (slots 22-33), and
+// record's v1, v2, 2 quadrilateral, 3 quadrilateral whose v0, v1 are the
+// previous record's v2, v3), then the edges the face owns: for the sides
+// (v0,v1), (v1,v2), (v2,v0) of a triangle or (v0,v1), (v1,v2), (v2,v3) of a
+// quadrilateral in turn len and w(3) (slots 22-33), and
 // the mask of owned sides as an integer in the low word of slot 34; slot 35:
-// the FP32 sweep's near distance (a float).  A quadrilateral is CCW about n;
-// a triangle's vertices may come in either order (its solid angle below uses
-// n and the area, not the order).
+// the FP32 sweep's near distance (a float).  A face's vertices may come in
+// either direction of its loop: its solid angle below takes the fan
+// triangles' triple products from n and their areas, not from the order.
 constexpr int kMeshFace = 36;
 constexpr int kMeshFaceEdges = 22;
 // Inside-test record: nplanes, M(3), up to 8 planes (n(3), d) with outward n:
@@ -72,8 +74,8 @@ RH_HD double mesh_face_omega(const double* __restrict__ rec, const double p[3])
     return 2.0 * double(rh_atan2_wave(im, re));
 }
 
-// Wave-uniform integer of a 0/1/2 code stored as a double: its high word
-// (0, 0x3ff00000, 0x40000000) read into a scalar register.
+// Wave-uniform integer of a 0/1/2/3 code stored as a double: its high word
+// (0, 0x3ff00000, 0x40000000, 0x40080000) read into a scalar register.
 RH_HD int mesh_code(const double* __restrict__ slot)
 {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -81,7 +83,7 @@ RH_HD int mesh_code(const double* __restrict__ slot)
 #else
     const int hi = int(uint64_t(__builtin_bit_cast(uint64_t, *slot)) >> 32);
 #endif
-    return hi == 0 ? 0 : (hi == 0x3ff00000 ? 1 : 2);
+    return hi == 0 ? 0 : (hi == 0x3ff00000 ? 1 : (hi == 0x40000000 ? 2 : 3));
 }
 
 // The owned-side mask of a face record (an integer in the slot's low word),
