@@ -313,14 +313,15 @@ constexpr int kTetTile = 32;
 #define RH_TET_WAVES 3
 #endif
 // mesh sweep: occupancy floor (waves per SIMD) and edge and face records
-// staged per LDS tile.  FP64: 5 waves (96 VGPRs, 16 bytes of spill) with
-// 64-record tiles measured 53.3 ms against 54.5 ms at 4 waves with 128-record
-// tiles on the C-yoke; FP32 runs 7 waves at 67 VGPRs.
+// staged per LDS tile.  FP64: 6 waves (80 VGPRs, 8 bytes of spill outside
+// the record loops) measured 50.9 / 566 ms on the C-yoke / Halbach ring
+// against 51.3 / 569 ms at 5 waves (85 VGPRs); FP32: 7 waves (72 VGPRs),
+// the records near a wave evaluated in FP64 after each tile's loop.
 #ifndef RH_MESH_WAVES32
 #define RH_MESH_WAVES32 7
 #endif
 #ifndef RH_MESH_WAVES
-#define RH_MESH_WAVES 5
+#define RH_MESH_WAVES 6
 #endif
 #ifndef RH_MESH_STAGE32
 #define RH_MESH_STAGE32 64
