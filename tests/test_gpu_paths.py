@@ -93,11 +93,15 @@ def test_assembly_fp32_storage(oracle_off):
     default the far blocks (beyond two circumradii) are integrated in FP32:
     near blocks stay within FP32 rounding, and a far block's FP32 integral
     errs by up to ~1e-3 of its own size, which is ~1e-8 of the row's scale
-    (the far field is small); every block is within 3e-7 of its row scale."""
+    (the far field is small); every block is within 3e-7 of its row scale.
+    Tetrahedra, blocks and hexahedral general polyhedra (integrated in FP64:
+    their FP32 integrals measured no faster on the Halbach ring)."""
     print("case: assembly fp32 storage", flush=True)
     ids = models.tet_mesh((5, 4, 3), 5.0, jitter=0.1, seed=4)
     ids.append(rad.ObjRecMag([-6, 3, 2], [3, 4, 5], [0, 0, 1]))
     ids.append(rad.ObjRecMag([60, 3, 2], [3, 4, 5], [0, 0, 1]))
+    for c in ([-8, 14, 6], [-8, 18, 6], [45, -10, 4]):
+        ids.append(models.hex_block(c, [3, 4, 5], [0.2, -0.4, 1.0]))
     top = rad.ObjCnt(ids)
     P, items = models.packed(top)
     n = len(P["type"])
