@@ -72,7 +72,8 @@ __device__ __forceinline__ void store_out(T* p, double v) { *p = T(v); }
 #endif
 // symmetric-block tiles: 8 (FP64) or 16 (FP32) sources, 64-byte plane
 // segments; on the 40,000-tetrahedron yoke 49.2 / 38.5 ms against 55.1 / 40.4
-// with 4 / 8 sources and 80.7 / 48.1 with 2 / 4
+// with 4 / 8 sources and 80.7 / 48.1 with 2 / 4 (padded tiles); unpadded
+// swizzled tiles 46.2 / 37.2 against 47.4 / 39.0 ms padded
 #ifndef RH_ASM_P64
 #define RH_ASM_P64 8
 #endif
@@ -86,7 +87,16 @@ struct AsmTile {
     static constexpr int S = PK ? ((sizeof(OUT) == 8) ? RH_ASM_P64 : RH_ASM_P32)
                                 : ((sizeof(OUT) == 8) ? RH_ASM_S64 : RH_ASM_S32);  // sources per tile
     static constexpr int U = VPS * S * int(sizeof(OUT)) / 8;  // 8-byte units per row segment
-    static constexpr int W = (U + 1) * 8 / int(sizeof(OUT));  // LDS row stride (elements)
+    static constexpr int EPU = 8 / int(sizeof(OUT));          // elements per 8-byte unit
+    // LDS row stride (elements): row-major tiles pad a unit against bank
+    // conflicts; plane tiles are unpadded (20 KB per wave, so that two waves
+    // per SIMD fit the CU's LDS) with the units of the rows of lane l stored
+    // at unit index u ^ (l & SWZ) instead
+    static constexpr int W = (PK ? U : U + 1) * EPU;
+    static constexpr int SWZ = PK ? U - 1 : 0;
+    static_assert(!PK || (U & (U - 1)) == 0, "plane tiles swizzle a power-of-two number of units");
+    // element index of element c of tile row r (rows RPE l .. RPE l + RPE - 1 belong to lane l)
+    __device__ static int at(int r, int c) { return r * W + (((c / EPU) ^ ((r / RPE) & SWZ)) * EPU + c % EPU); }
 };
 
 #ifdef RH_ASM_WAVES
@@ -161,11 +171,10 @@ __global__ __launch_bounds__(64) RH_ASM_ATTR void k_assemble(DevModel m, OUT* __
                 if constexpr (PK == 6) v[5] = q[8];
             }
             if (run) {
-                OUT* t = tile + RPE * lane * W + VPS * s;
 #pragma unroll
                 for (int a = 0; a < RPE; a++)
 #pragma unroll
-                    for (int b = 0; b < VPS; b++) t[a * W + b] = OUT(v[VPS * a + b]);
+                    for (int b = 0; b < VPS; b++) tile[TL::at(RPE * lane + a, VPS * s + b)] = OUT(v[VPS * a + b]);
             } else if (live) {
                 OUT* o = row + VPS * j;
 #pragma unroll
@@ -184,18 +193,20 @@ __global__ __launch_bounds__(64) RH_ASM_ATTR void k_assemble(DevModel m, OUT* __
                 // in 32 bits from a uniform base (the tile spans < 2^32 bytes)
                 const int lr = lane / U, lc = lane - U * lr;
                 if (lr < RP) {
-                    constexpr int WU = W * int(sizeof(OUT)) / 8;
-                    const uint64_t* src = reinterpret_cast<const uint64_t*>(tile) + lr * WU + lc;
+                    constexpr int WU = W / TL::EPU;
+                    const uint64_t* src = reinterpret_cast<const uint64_t*>(tile);
+                    // unit lc of tile row r
+                    auto unit = [&](int r) { return src[r * WU + (lc ^ ((r / RPE) & TL::SWZ))]; };
                     char* dst = reinterpret_cast<char*>(base);  // uniform, advanced per pass
                     const int64_t step = int64_t(RP) * ld * int64_t(sizeof(OUT));
                     const uint32_t off = uint32_t(lr) * uint32_t(ld * sizeof(OUT)) + uint32_t(8 * lc);
                     constexpr int full = TR / RP;  // passes with every lane's row in the tile
 #pragma unroll 4
                     for (int it = 0; it < full; it++) {
-                        *reinterpret_cast<uint64_t*>(dst + off) = src[RP * it * WU];
+                        *reinterpret_cast<uint64_t*>(dst + off) = unit(lr + RP * it);
                         dst += step;
                     }
-                    if (RP * full + lr < TR) *reinterpret_cast<uint64_t*>(dst + off) = src[RP * full * WU];
+                    if (RP * full + lr < TR) *reinterpret_cast<uint64_t*>(dst + off) = unit(lr + RP * full);
                 }
             } else {
                 // partial tile: the RPE*nrow stored rows, VPS*cnt values each
@@ -203,7 +214,7 @@ __global__ __launch_bounds__(64) RH_ASM_ATTR void k_assemble(DevModel m, OUT* __
                 const int step_r = 64 / cols, step_c = 64 - step_r * cols;
                 int r = lane / cols, c = lane - r * cols;
                 for (int e = lane; e < total; e += 64) {
-                    base[int64_t(r) * ld + c] = tile[r * W + c];
+                    base[int64_t(r) * ld + c] = tile[TL::at(r, c)];
                     c += step_c;
                     r += step_r;
                     if (c >= cols) {
