@@ -73,7 +73,9 @@ __device__ __forceinline__ void store_out(T* p, double v) { *p = T(v); }
 // symmetric-block tiles: 8 (FP64) or 16 (FP32) sources, 64-byte plane
 // segments; on the 40,000-tetrahedron yoke 49.2 / 38.5 ms against 55.1 / 40.4
 // with 4 / 8 sources and 80.7 / 48.1 with 2 / 4 (padded tiles); unpadded
-// swizzled tiles 46.2 / 37.2 against 47.4 / 39.0 ms padded
+// swizzled tiles 46.2 / 37.2 against 47.4 / 39.0 ms padded.  Three waves
+// per SIMD (RH_ASM_WAVES=3: 154 VGPRs) with 4 / 8 sources measured 58-63 /
+// 40 ms: the wider plane segments matter more than the occupancy
 #ifndef RH_ASM_P64
 #define RH_ASM_P64 8
 #endif
