@@ -18,8 +18,8 @@ Workloads (seeded, synthetic, built through the drop-in API; no files):
   iron and the coils on a 100^3 gap grid, FP64 and mixed.
 
 The roofline is that of the headline's dominant kernel, the FP64 matrix pass
-(HBM-bound).  The CPU baseline is the FP64 oracle on the box's host cores, on
-a bounded sample of the headline workload.
+(HBM-bound).  The CPU baseline is the FP64 oracle on this machine's host
+cores, on a bounded sample of the headline workload.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -491,8 +491,8 @@ def main():
                          "known HBM traffic, for calibrating profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(every box has a single MI355X); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the multi-GPU sharding of the matrix rows is a "
+              "design, not built; run with --gpus 1", file=sys.stderr)
         return 2
     ctx = Ctx()
     cpu = not args.no_cpu and not args.only and args.workload == "all"
