@@ -387,4 +387,27 @@ int rhc_field_mesh(const rh_model_desc* d, const double* M, const double* pts, i
     return RH_OK;
 }
 
+// Sweep-order statistics of the mesh form: out[0..6] = unique edges, edges
+// swept on their own (owned by no face), unique faces, triangles,
+// continued triangles, quadrilaterals, continued quadrilaterals.  Returns 2
+// when the model has no mesh form.
+int rhc_mesh_stats(const rh_model_desc* d, int64_t* out)
+{
+    rh::PackedModel P;
+    rh_status st = rh::pack_model(d, &P, &g_err);
+    if (st != RH_OK) return st;
+    const rh::MeshForm& F = P.mesh;
+    if (!F.active) return 2;
+    int64_t cont[4] = {0, 0, 0, 0};
+    for (double c : F.face_cont) cont[int(c) & 3]++;
+    out[0] = F.n_edges();
+    out[1] = F.ne_free;
+    out[2] = F.n_faces();
+    out[3] = F.nf_tri;
+    out[4] = cont[1];
+    out[5] = F.n_faces() - F.nf_tri;
+    out[6] = cont[3];
+    return RH_OK;
+}
+
 }  // extern "C"

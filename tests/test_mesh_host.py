@@ -125,3 +125,31 @@ def test_mesh_form_with_transform_copies_equals_per_element(L):
     # faces on the two mirror planes are shared: fewer unique faces than without sharing
     assert counts[1] < 4 * counts[2] * 0.6
     rad.UtiDelAll()
+
+
+def _mesh_stats(L, top):
+    items = [it for it in O.walk(rad._reg, top)
+             if isinstance(it[0], (O.RecMag, O.Polyhedron)) and not getattr(it[0], "is_current", False)]
+    desc, keep = _abi.model_desc(O.pack_elements(items))
+    out = np.zeros(7, np.int64)
+    assert L.rhc_mesh_stats(ctypes.byref(desc), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))) == 0
+    return dict(zip(("edges", "free", "faces", "tri", "tri_cont", "quad", "quad_cont"), out.tolist()))
+
+
+def test_sweep_order_reuses_offsets_on_the_bench_meshes(L):
+    """The packer's strips and side ownership: on the C-yoke tetrahedra every
+    edge is evaluated by a triangle and most triangles continue a strip; on the
+    hexahedral Halbach ring almost every edge is owned by a quadrilateral and
+    most quadrilaterals continue a strip (each continued face computes one or
+    two vertex offsets instead of three or four)."""
+    rad.UtiDelAll()
+    _top, iron, _coil, _info = W.c_yoke()
+    s = _mesh_stats(L, iron)
+    assert s["quad"] == 0 and s["tri"] == s["faces"] == 17200 and s["edges"] == 11454
+    assert s["free"] == 0 and s["tri_cont"] >= 0.85 * s["tri"]
+    rad.UtiDelAll()
+    top, _pts = W.halbach(n_points=10)
+    s = _mesh_stats(L, top)
+    assert s["tri"] == 0 and s["quad"] == s["faces"]
+    assert s["free"] <= 0.01 * s["edges"] and s["quad_cont"] >= 0.7 * s["quad"]
+    rad.UtiDelAll()
